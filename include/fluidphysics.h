@@ -1,0 +1,223 @@
+/*
+ * fluidphysics.h -- C ABI of the MI355X unified-particle solver (libfluidphysics.so).
+ *
+ * This is the drop-in boundary for the per-substep solver of the reference
+ * CPU solver `class Solver` (src/solver.js:8-516).  Every entry point below
+ * replaces one method or one public field of that class; the citation on
+ * each declaration names it.  The N-API addon (fluidphysics_amd/addon) and
+ * the ctypes host (fluidphysics_amd/__init__.py) bind exactly these symbols.
+ *
+ * Conventions
+ *   - Every function returns 0 on success, a negative FP_E* code on failure;
+ *     fp_last_error() then holds a message.  Cases the reference ignores
+ *     silently keep the reference's answer (fp_cut reports 0 cuts for a
+ *     degenerate wedge, fp_pick_solid reports id -1 for no hit).
+ *   - Plain C types only: pointers + counts.  Positions crossing the ABI are
+ *     float32 xyz triples (the reference's Float32Array layout, 3 per
+ *     particle); ray/knife/grab inputs are double (JS numbers).
+ *   - Device state stays resident in HBM between calls.  fp_step() enqueues
+ *     all substeps on the solver's own HIP stream and waits for them, like
+ *     the synchronous JS call; fp_step_async() leaves the wait to fp_sync().
+ *   - Not thread-safe per solver; different solvers are independent.
+ */
+#ifndef FLUIDPHYSICS_H
+#define FLUIDPHYSICS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FP_OK 0
+#define FP_EINVAL (-1)     /* bad argument (null pointer, out-of-range size, wrong call order) */
+#define FP_EDEVICE (-2)    /* HIP runtime error */
+#define FP_ENOMEM (-3)     /* device or host allocation failed */
+#define FP_ESTATE (-4)     /* call not valid in the solver's current state */
+
+/* Solver constants.  Defaults (fp_params_default) are the reference's
+ * tuning fields, src/solver.js:14-37, plus cellSize (:50) and the
+ * hard-coded fluid damping (:220).  Mutable between steps like the JS
+ * public fields (fp_set_params). */
+typedef struct fp_params {
+  double gravity;              /* solver.js:15  -10.0 */
+  double solid_radius;         /* :16  0.09 */
+  double fluid_radius;         /* :17  0.075 */
+  double compliance;           /* :18  2.5e-4 */
+  double plastic_threshold;    /* :19  0.32 */
+  double plastic_rate;         /* :20  0.04 */
+  double break_strain;         /* :21  2.6 */
+  double fluid_h;              /* :23  0.19 (interaction radius) */
+  double rest_density;         /* :24  6.0 */
+  double stiffness;            /* :25  0.002 */
+  double near_stiffness;       /* :26  0.008 */
+  double viscosity;            /* :27  0.3 (XSPH) */
+  double solid_damping;        /* :28  1.2 1/s */
+  double fluid_damping;        /* :220 0.6 1/s (hard-coded in the reference) */
+  double floor_friction_solid; /* :29  0.4 */
+  double floor_friction_fluid; /* :30  0.04 */
+  double wall_x;               /* :31  3.5 */
+  double wall_z;               /* :31  3.5 */
+  double wall_height;          /* :31  0.92 */
+  double wall_half_thick;      /* :31  0.07 */
+  double cull_radius;          /* :32  7.0 */
+  double max_speed;            /* :33  14.0 */
+  double grab_stiffness;       /* :36  0.3 */
+  double cell_size;            /* :50  0.22 (must be >= fluid_h and >= 2*max radius) */
+  int32_t constraint_iters;    /* Jacobi XPBD iterations per substep (the reference runs one
+                                  Gauss-Seidel sweep, :215); default 3 */
+  int32_t table_size;          /* hash table size, power of two; 0 = nextpow2(2*capacity) (:51) */
+  int32_t max_neighbors;       /* per-fluid-particle neighbour-list capacity; 0 = 96.  The
+                                  reference caps the global pair list at 32*maxFluid (:57) */
+  int32_t reserved0;
+  double jacobi_omega;         /* over-relaxation of the summed constraint correction, 1.0 */
+  double contact_scale;        /* scale on each Jacobi contact correction, 1.0 */
+} fp_params;
+
+/* Opaque solver handle (one scene, one HIP stream). */
+typedef struct fp_solver fp_solver;
+
+typedef struct fp_info {
+  int32_t capacity;            /* solids + maxFluid slots (solver.js:41) */
+  int32_t solid_count;         /* :89 */
+  int32_t max_fluid;           /* :10 */
+  int32_t table_size;          /* :51 */
+  int32_t num_constraints;     /* consA.length */
+  int32_t fluid_next;          /* ring cursor of emitFluid (:140-141) */
+  int32_t max_neighbors;       /* neighbour-list capacity per fluid particle */
+  int32_t device;              /* HIP device ordinal */
+  int64_t substeps_done;       /* substeps executed since creation */
+  int64_t neighbor_overflow;   /* fluid particles whose neighbour list was truncated (cumulative) */
+  int64_t device_bytes;        /* HBM bytes held by this solver */
+} fp_info;
+
+/* ---- lifetime ------------------------------------------------------- */
+
+/* Fill the reference defaults (solver.js:14-37, :50, :220). */
+void fp_params_default(fp_params* out);
+
+/* new Solver({ maxFluid }) (solver.js:9).  params may be NULL (defaults).
+ * device < 0 selects the current HIP device. */
+int fp_create(const fp_params* params, int32_t max_fluid, int32_t device, fp_solver** out);
+
+/* Frees device memory and the stream.  NULL is accepted. */
+int fp_destroy(fp_solver* s);
+
+/* Message of the last failure on s (s == NULL: last failure of fp_create or
+ * of a call made with a NULL solver).  Never NULL. */
+const char* fp_last_error(const fp_solver* s);
+
+/* Replace / read the tuning fields (the JS public fields, solver.js:14-37). */
+int fp_set_params(fp_solver* s, const fp_params* p);
+int fp_get_params(const fp_solver* s, fp_params* out);
+int fp_get_info(const fp_solver* s, fp_info* out);
+
+/* ---- scene set-up --------------------------------------------------- */
+
+/* addSolidCube(cx, cy, cz, n, spacing, particleMass) (solver.js:72-123):
+ * allocates capacity n^3 + maxFluid, the n^3 lattice (id = (ix*n+iy)*n+iz),
+ * the 26-neighbour distance constraints (b > a, loop order of :99-116) and
+ * their CSR adjacency.  Call once, before stepping. */
+int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, double spacing,
+                      double particle_mass);
+
+/* emitFluid(x, y, z, vx, vy, vz, mass) called n times in order
+ * (solver.js:139-149): ring slots solidCount + fluidNext, overwriting live
+ * particles when the ring wraps.  pos/vel: 3n doubles; mass: n doubles or
+ * NULL for the default 0.03. */
+int fp_emit(fp_solver* s, const double* pos, const double* vel, const double* mass, int32_t n);
+
+/* resetSolid() (solver.js:125-137): restore lattice positions and rest
+ * lengths, zero solid velocities, revive every constraint, groups = 0,
+ * clear the grabs. */
+int fp_reset_solid(fp_solver* s);
+
+/* clearFluid() (solver.js:151-153). */
+int fp_clear_fluid(fp_solver* s);
+
+/* ---- the hot path --------------------------------------------------- */
+
+/* step(dt, substeps) (solver.js:187-242): substeps x [integrate, grab,
+ * hash grid, contacts, double-density relaxation, XPBD constraints, bounds,
+ * velocity + damping, XSPH], then the connected-component labels if a
+ * constraint tore.  Synchronous. */
+int fp_step(fp_solver* s, double dt, int32_t substeps);
+
+/* Same work as fp_step, enqueued without the final wait.  The component
+ * labels are refreshed lazily when fp_read_state asks for them. */
+int fp_step_async(fp_solver* s, double dt, int32_t substeps);
+
+/* Wait for all work enqueued on s. */
+int fp_sync(fp_solver* s);
+
+/* The `grabs` public field (solver.js:35, applied at :205-210): replaces the
+ * list of {i, x, y, z} targets pulled toward each substep.  idx: n particle
+ * indices, xyz: 3n doubles.  n == 0 clears the list. */
+int fp_set_grabs(fp_solver* s, const int32_t* idx, const double* xyz, int32_t n);
+
+/* cut(ox, oy, oz, d1x, d1y, d1z, d2x, d2y, d2z) (solver.js:458-486): kill
+ * every alive constraint whose segment crosses the wedge swept between two
+ * rays.  The predicate is evaluated in IEEE double from the float32
+ * positions, in the reference's operation order.  *ncut = constraints cut
+ * (0 for a degenerate wedge, |d1 x d2| < 1e-8). */
+int fp_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[3], int32_t* ncut);
+
+/* pickSolid(ox, oy, oz, dx, dy, dz, maxPerp) (solver.js:502-515): nearest
+ * solid along the ray with perpendicular distance <= max_perp; ties go to the
+ * lowest index.  *id = -1 (and *t = +inf) when nothing is hit. */
+int fp_pick_solid(fp_solver* s, const double o[3], const double d[3], double max_perp, int32_t* id,
+                  double* t);
+
+/* fluidCount() (solver.js:155-159). */
+int fp_fluid_count(fp_solver* s, int32_t* count);
+
+/* ---- state access (the JS public typed arrays, solver.js:42-48) ------ */
+
+/* Copy state to host.  Any pointer may be NULL.  pos/prev/vel: 3*capacity
+ * floats; inv_mass: capacity floats; active: capacity bytes; group:
+ * solidCount int32 (union-find roots, solver.js:488-499). */
+int fp_read_state(fp_solver* s, float* pos, float* prev, float* vel, float* inv_mass,
+                  uint8_t* active, int32_t* group);
+
+/* Overwrite state (any pointer may be NULL to keep that array).  Used to
+ * start the solver from a given state, as assigning the typed arrays does. */
+int fp_write_state(fp_solver* s, const float* pos, const float* prev, const float* vel,
+                   const float* inv_mass, const uint8_t* active);
+
+/* Constraint topology and state: a, b (consA/consB), rest (consRest), alive
+ * (consAlive), num_constraints entries each; any pointer may be NULL. */
+int fp_read_constraints(fp_solver* s, int32_t* a, int32_t* b, float* rest, uint8_t* alive);
+int fp_write_constraints(fp_solver* s, const float* rest, const uint8_t* alive);
+
+/* _buildGrid() (solver.js:174-184) on the current positions, on the device. */
+int fp_build_grid(fp_solver* s);
+
+/* cellStart (table_size + 1 int32) and cellEntries (capacity int32; only the
+ * first cellStart[table_size] are defined) of the last grid build. */
+int fp_read_grid(fp_solver* s, int32_t* cell_start, int32_t* cell_entries);
+
+/* rho / rhoNear (capacity floats each; fluid slots) of the last substep's
+ * density pass (solver.js:316-317).  Either pointer may be NULL. */
+int fp_read_density(fp_solver* s, float* rho, float* rho_near);
+
+/* ---- measurement ---------------------------------------------------- */
+
+/* Per-pass timing with HIP events recorded on the solver's stream. */
+#define FP_MAX_PASSES 24
+typedef struct fp_pass_stat {
+  char name[32];
+  int64_t launches;
+  double total_ms;
+} fp_pass_stat;
+
+/* enable != 0: bracket every kernel launch with events (adds overhead; for
+ * profiling only).  Resets the accumulated statistics. */
+int fp_profile_enable(fp_solver* s, int32_t enable);
+/* Accumulated statistics; *n = number of passes written (<= max). */
+int fp_profile_read(fp_solver* s, fp_pass_stat* out, int32_t max, int32_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUIDPHYSICS_H */
