@@ -1,0 +1,227 @@
+"""Benchmark: particle-substeps/s of the unified-particle solver on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large]
+
+A step is one frame of the workload: the hose burst, any scheduled slice /
+grab, then `substeps` solver substeps (src/solver.js:187 step(dt, 5)).
+Inputs are resident in HBM when the timed region starts; the region is
+bracketed by a device sync on both sides.  value = (active particles x
+substeps x frames) / wall time.
+
+Workloads (fluidphysics_amd/scenes.py):
+  large       24^3 = 13,824 cube + 262,144 fluid slab + 120/frame hose (default;
+              the configuration the throughput target is quoted on)
+  single_gpu  20^3 = 8,000 cube + 65,536 fluid column
+  plain_cpu   7^3 = 343 cube + 2,600 fluid (the reference's CPU scale)
+
+Printed: one JSON line with the metric, a `roofline` object for the dominant
+kernel (algorithmic bytes per launch / average launch time from HIP events
+on the solver's stream) and a `cpu_baseline` object (the oracle -- the
+plain-C restatement of src/solver.js -- on one host core, bounded sample).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def model_bytes_per_substep(n, nf, ns, c, t, iters):
+    """Algorithmic bytes per substep: each array a pass reads or writes counted
+    once at the float4 SoA layout (integrate 84, grid count 28, scatter 32,
+    contacts 44, bounds 56, velocity 52 per particle; DDR density 36,
+    displacement 52, XSPH 60 per fluid particle; 12 per table entry; per
+    constraint iteration 40 per constraint + 36 per solid)."""
+    return 296 * n + 148 * nf + 12 * t + iters * (40 * c + 36 * ns)
+
+
+# Per-launch algorithmic bytes of each pass under the same model (per-unit
+# figure x the units one launch processes).
+def pass_bytes(name, n, nf, ns, c, t):
+    return {
+        "integrate": 84 * n,
+        "hash_count": 28 * n,
+        "scatter": 32 * n,
+        "contacts": 44 * n,
+        "density": 36 * nf,
+        "displace": 52 * nf,
+        "constraints": 40 * c + 36 * ns,
+        "bounds_velocity": (56 + 52) * n,
+        "xsph": 60 * nf,
+        "scan_reduce": 4 * t,
+        "scan_apply": 8 * t,
+    }.get(name)
+
+
+def run_frames(solver, scene, f0, frames, rest_pos, grab_ids, sync_each=False):
+    from fluidphysics_amd import scenes
+    sp = 1.91 * scene.params_with()["fluid_radius"]
+    for f in range(f0, f0 + frames):
+        if scene.hose:
+            p, v = scenes.hose_burst(scene.hose, f, sp)
+            solver.emit_batch(p, v, scene.fluid_mass)
+        for w in scene.wedges:
+            if w.frame == f:
+                solver.cut(*w.o, *w.d1, *w.d2)
+        if scene.grab and grab_ids is not None:
+            g0, g1 = scene.grab["frames"]
+            if g0 <= f < g1:
+                tg = scenes.grab_targets(scene, f, grab_ids, rest_pos)
+                solver.grabs = [{"i": int(i), "x": float(x[0]), "y": float(x[1]), "z": float(x[2])}
+                                for i, x in zip(grab_ids, tg)]
+            elif f == g1:
+                solver.grabs = []
+        solver.step(scene.dt, scene.substeps, wait=sync_each)
+
+
+def active_count(solver):
+    return solver.solidCount + solver.fluidCount()
+
+
+def cpu_baseline(scene, budget_s=12.0, max_frames=50):
+    """The oracle (plain-C restatement of src/solver.js, Gauss-Seidel mode,
+    float32 stores) on one host core: frames of the same workload from its
+    initial state until ~budget_s of CPU work."""
+    from oracle.oracle import OracleSolver
+    from fluidphysics_amd import scenes
+    p = scene.params_with(constraint_iters=1)
+    o = OracleSolver(p, scene.max_fluid, storage="f32", mode="gs")
+    o.add_solid_cube(*scene.cube)
+    o.emit(scene.fluid_pos, scene.fluid_vel, scene.fluid_mass)
+    sp = 1.91 * p["fluid_radius"]
+    psub = 0
+    frames = 0
+    t0 = time.perf_counter()
+    while frames < max_frames and time.perf_counter() - t0 < budget_s:
+        if scene.hose:
+            hp, hv = scenes.hose_burst(scene.hose, frames, sp)
+            o.emit(hp, hv, scene.fluid_mass)
+        n_act = scene.solid_count + o.fluid_count()
+        o.step(scene.dt, scene.substeps)
+        psub += n_act * scene.substeps
+        frames += 1
+    el = time.perf_counter() - t0
+    return {"value": psub / el, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+            "sample": "%d frame(s) x %d substeps of the %s workload from its initial state (%.1f s), "
+                      "oracle Gauss-Seidel mode (src/solver.js semantics), float32 stores, 1 thread of %d"
+                      % (frames, scene.substeps, scene.name, el, os.cpu_count() or 0)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="large", choices=["large", "single_gpu", "plain_cpu"])
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the solver's path does not span GPUs "
+              "(multi-GPU sharding is a design note only). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from fluidphysics_amd import from_scene, scenes
+    scene = scenes.SCENES[args.workload]()
+    solver = from_scene(scene)
+    inf = solver.info()
+    p = scene.params_with()
+    grab_ids = rest_pos = None
+    if scene.grab:
+        st = solver.read_state()
+        pos = st["pos"].reshape(-1, 3)[: solver.solidCount]
+        d = np.linalg.norm(pos - np.asarray(scene.grab["point"]), axis=1)
+        grab_ids = np.argsort(d, kind="stable")[: scene.grab["count"]]
+        rest_pos = pos[grab_ids].astype(np.float64)
+
+    print("bench: workload %s, capacity %d, table %d, constraints %d" %
+          (scene.name, inf.capacity, inf.table_size, inf.num_constraints), file=sys.stderr, flush=True)
+    run_frames(solver, scene, 0, args.warmup, rest_pos, grab_ids)
+    solver.sync()
+    n0 = active_count(solver)
+
+    f0 = args.warmup
+    solver.sync()
+    t0 = time.perf_counter()
+    run_frames(solver, scene, f0, args.steps, rest_pos, grab_ids)
+    solver.sync()
+    el = time.perf_counter() - t0
+    n1 = active_count(solver)
+    n_mean = 0.5 * (n0 + n1)
+    value = n_mean * scene.substeps * args.steps / el
+    print("bench: %d frames in %.3f s, %.0f active particles" % (args.steps, el, n_mean), file=sys.stderr, flush=True)
+
+    # per-pass timing: HIP events on the solver's stream, a separate region
+    solver.profile(True)
+    run_frames(solver, scene, f0 + args.steps, args.profile_steps, rest_pos, grab_ids)
+    solver.sync()
+    stats = solver.profile_read()
+    solver.profile(False)
+
+    nf = n_mean - solver.solidCount
+    ns, c, t = solver.solidCount, inf.num_constraints, inf.table_size
+    iters = p["constraint_iters"]
+    passes = {}
+    dom, dom_ms = None, -1.0
+    for name, (launches, total_ms) in stats.items():
+        avg = total_ms / launches
+        passes[name] = {"launches": launches, "avg_us": round(avg * 1e3, 3),
+                        "ms_per_frame": round(total_ms / args.profile_steps, 4)}
+        b = pass_bytes(name, n_mean, nf, ns, c, t)
+        if b is not None:
+            if name == "constraints":
+                b = b  # one launch = one iteration
+            passes[name]["gbs"] = round(b / (avg * 1e-3) / 1e9, 1)
+        if total_ms > dom_ms and b is not None:
+            dom, dom_ms = name, total_ms
+    launches, total_ms = stats[dom]
+    avg_s = total_ms / launches * 1e-3
+    bytes_dom = pass_bytes(dom, n_mean, nf, ns, c, t)
+    achieved = bytes_dom / avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "bytes_per_launch": int(bytes_dom), "avg_launch_us": round(avg_s * 1e6, 3)}
+    b_sub = model_bytes_per_substep(n_mean, nf, ns, c, t, iters)
+    model_frac = b_sub * (value / n_mean) / (HBM_PEAK_GBS * 1e9)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene)
+        print("bench: cpu baseline %.3e p-substeps/s" % cpu["value"], file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "particle-substeps/s",
+        "value": round(value, 1),
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": scene.name, "solids": ns, "fluid_active_mean": int(nf),
+                   "capacity": inf.capacity, "constraints": c, "table_size": t,
+                   "substeps": scene.substeps, "constraint_iters": iters,
+                   "hose_per_frame": scene.hose["per_frame"] if scene.hose else 0},
+        "hbm_model_bytes_per_substep": int(b_sub),
+        "hbm_roofline_frac_model": round(model_frac, 4),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "passes": passes,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -70,7 +70,8 @@ typedef struct fp_params {
   int32_t max_neighbors;       /* per-fluid-particle neighbour-list capacity; 0 = 96.  The
                                   reference caps the global pair list at 32*maxFluid (:57) */
   int32_t reserved0;
-  double jacobi_omega;         /* over-relaxation of the summed constraint correction, 1.0 */
+  double jacobi_omega;         /* Jacobi relaxation: each constraint's correction is scaled by
+                                  jacobi_omega / max(deg_a, deg_b); default 4.0 */
   double contact_scale;        /* scale on each Jacobi contact correction, 1.0 */
 } fp_params;
 

@@ -75,6 +75,9 @@ typedef struct Oracle {
 
 /* ECMAScript ToInt32 followed by >>> 0 (ToUint32) of a double. */
 static uint32_t js_touint32(double x) {
+  /* every double of magnitude >= 2^53 is an integer, so below 2^63 the
+   * truncated value fits int64 and wraps to the same residue mod 2^32 */
+  if (fabs(x) < 9.2e18) return (uint32_t)(int64_t)x;
   if (!isfinite(x)) return 0u;
   double t = trunc(x);
   double m = fmod(t, 4294967296.0);

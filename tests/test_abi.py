@@ -1,0 +1,62 @@
+"""The C ABI library loads on a host without a GPU and exports every entry
+point include/fluidphysics.h declares; the ctypes host binds the same set.
+No compute calls (no device here)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from fluidphysics_amd import _abi
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "fluidphysics.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(fp_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_declares_the_reference_surface():
+    names = _declared()
+    for fn in ["fp_create", "fp_destroy", "fp_add_solid_cube", "fp_step", "fp_cut", "fp_set_grabs",
+               "fp_emit", "fp_pick_solid", "fp_reset_solid", "fp_clear_fluid", "fp_fluid_count",
+               "fp_read_state", "fp_read_grid", "fp_read_constraints"]:
+        assert fn in names
+
+
+@pytest.mark.skipif(not os.path.exists(_abi.LIB_PATH), reason="library not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_abi.LIB_PATH)
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", _abi.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (fp_[a-z_0-9]+)", out))
+    assert set(_declared()) <= exported
+
+
+def test_ctypes_host_binds_exactly_the_header():
+    assert sorted(_abi.SIGNATURES) == _declared()
+
+
+@pytest.mark.skipif(not os.path.exists(_abi.LIB_PATH), reason="library not built")
+def test_params_default_and_error_paths_without_device():
+    lib = _abi.load_library()
+    p = _abi.FpParams()
+    lib.fp_params_default(ctypes.byref(p))
+    ref = dict(_abi.REFERENCE_DEFAULTS)
+    for k, v in ref.items():
+        assert getattr(p, k) == v, k
+    # argument errors are reported without touching a device
+    assert lib.fp_create(ctypes.byref(p), -1, 0, ctypes.byref(ctypes.c_void_p())) == _abi.FP_EINVAL
+    assert b"max_fluid" in lib.fp_last_error(None)
+    bad = _abi.make_params(cell_size=0.1, fluid_h=0.19)
+    assert lib.fp_create(ctypes.byref(bad), 16, 0, ctypes.byref(ctypes.c_void_p())) == _abi.FP_EINVAL
+    assert lib.fp_step(None, 0.01, 1) == _abi.FP_EINVAL
+
+
+def test_params_struct_layout_matches_header():
+    # 24 doubles, 4 int32, 2 doubles
+    assert ctypes.sizeof(_abi.FpParams) == 24 * 8 + 4 * 4 + 2 * 8
