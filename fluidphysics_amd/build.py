@@ -24,18 +24,20 @@ HIP_FLAGS = [
 
 NODE_INCLUDE = "/usr/include/node"
 
+SOURCES = ["fp_api.hip", "fp_grid.hip", "fp_fluid.hip", "fp_solid.hip", "fp_step.hip"]
+
 
 def build_library(verbose=False):
-    src = os.path.join(HERE, "csrc", "fp_solver.hip")
+    srcs = [os.path.join(HERE, "csrc", f) for f in SOURCES]
     out = os.path.join(HERE, "libfluidphysics.so")
     tmp = out + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp, src]
+    cmd = [HIPCC] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed for %s" % src)
+        raise RuntimeError("hipcc failed for %s" % " ".join(SOURCES))
     os.replace(tmp, out)
     return out
 
@@ -47,7 +49,7 @@ def build_addon(verbose=False):
         return None
     out = os.path.join(HERE, "addon", "fp_napi.node")
     cxx = shutil.which("g++") or "g++"
-    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-DNAPI_VERSION=8",
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-DNAPI_VERSION=6", "-DNODE_GYP_MODULE_NAME=fp_napi",
            "-I", NODE_INCLUDE, "-I", os.path.join(ROOT, "include"),
            "-o", out + ".tmp", src, "-L", HERE, "-lfluidphysics", "-Wl,-rpath,$ORIGIN/.."]
     if verbose:

@@ -1,0 +1,823 @@
+// fp_api.hip -- the C ABI of include/fluidphysics.h: scene set-up, state
+// transfers, sync points, component labels, profiling.  Kernels live in
+// fp_grid.hip, fp_fluid.hip, fp_solid.hip and fp_step.hip.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "fp_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error = "";
+
+const char* kPassNames[P_NUM] = {"integrate", "grab",        "hash_count",      "scan_reduce", "scan_apply",
+                                 "scatter",   "rank",        "contacts",        "density",     "displace",
+                                 "constraints", "bounds_velocity", "xsph",        "cut",         "pick",
+                                 "count_active", "emit"};
+
+// Host <-> device copies ordered on the solver's (non-blocking) stream.
+hipError_t d2h(fp_solver* s, void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream);
+  return e == hipSuccess ? hipStreamSynchronize(s->stream) : e;
+}
+
+hipError_t h2d(fp_solver* s, void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream);
+  return e == hipSuccess ? hipStreamSynchronize(s->stream) : e;
+}
+
+void profile_collect(fp_solver* s) {
+  for (auto& e : s->ev_live) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+      s->prof_ms[e.pass] += ms;
+      s->prof_n[e.pass] += 1;
+    }
+    s->ev_pool.push_back(e.a);
+    s->ev_pool.push_back(e.b);
+  }
+  s->ev_live.clear();
+}
+
+template <typename T>
+int dalloc(fp_solver* s, T** p, size_t count) {
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess) return fp_fail(s, FP_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  s->device_bytes += (int64_t)bytes;
+  e = hipMemsetAsync(*p, 0, bytes, s->stream);
+  if (e != hipSuccess) return fp_fail(s, FP_EDEVICE, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+  return FP_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void free_all(fp_solver* s) {
+  dfree(s->pos), dfree(s->prev), dfree(s->vel), dfree(s->active), dfree(s->rho), dfree(s->phash);
+  dfree(s->lcount), dfree(s->lstart), dfree(s->lblock_sums), dfree(s->ltmp), dfree(s->lkey), dfree(s->lslot);
+  dfree(s->sid), dfree(s->slot), dfree(s->spos), dfree(s->spos2), dfree(s->svtmp), dfree(s->sphash), dfree(s->sP);
+  dfree(s->mcount), dfree(s->mlist), dfree(s->sbox);
+  dfree(s->pslot), dfree(s->count), dfree(s->cell_start), dfree(s->entries_tmp), dfree(s->cell_entries);
+  dfree(s->block_sums);
+  dfree(s->cons_ab), dfree(s->rest[0]), dfree(s->rest[1]), dfree(s->alive[0]), dfree(s->alive[1]);
+  dfree(s->adj_off), dfree(s->adj), dfree(s->spos_c[0]), dfree(s->spos_c[1]), dfree(s->dflags), dfree(s->dpick_t);
+  dfree(s->dpick_i), dfree(s->grab_idx), dfree(s->grab_tgt), dfree(s->emit_idx), dfree(s->emit_p), dfree(s->emit_v);
+}
+
+int log2_pow2(uint32_t v) {
+  int k = 0;
+  while ((1u << k) < v) k++;
+  return k;
+}
+
+uint32_t next_pow2(double v, uint32_t lo) {
+  uint32_t t = lo;
+  while ((double)t < v) t <<= 1;
+  return t;
+}
+
+// Cell-grid dimensions: NZ and NY cover the basin (walls + 1 m) and 4 m of
+// height without aliasing; the table holds at least 4 z-y planes and the
+// reference table size, at most 2^25 keys (far cells beyond that alias and
+// are rejected by the distance tests).
+void setup_cell_grid(fp_solver* s) {
+  const fp_params& P = s->P;
+  const double cs = P.cell_size;
+  const double half = std::max(std::fabs(P.wall_x), std::fabs(P.wall_z)) + 1.0;
+  const uint32_t NZ = next_pow2(std::ceil(2.0 * half / cs), 4);
+  const uint32_t NY = next_pow2(std::ceil(4.0 / cs), 4);
+  const uint64_t plane = (uint64_t)NZ * NY;
+  uint64_t M = std::max<uint64_t>(plane * NZ, (uint64_t)s->T);
+  M = std::max<uint64_t>(M, 4 * plane);
+  M = std::min<uint64_t>(M, 1ull << 25);
+  uint32_t m32 = next_pow2((double)M, 4);
+  s->cg.inv_cell = 1.0 / cs;
+  s->cg.hmask = (uint32_t)(s->T - 1);
+  s->cg.M = m32;
+  s->cg.kmask = m32 - 1;
+  s->cg.nz_shift = (uint32_t)log2_pow2(NZ);
+  s->cg.nyz_shift = (uint32_t)log2_pow2((uint32_t)plane);
+  s->lscan_blocks = (int)((m32 + SCAN_TILE - 1) / SCAN_TILE);
+  // moved table: >= 4 z-y planes keeps the 27 neighbour buckets distinct
+  uint32_t tm = s->P.moved_table > 0 ? (uint32_t)s->P.moved_table : next_pow2((double)s->n / 8.0, 1024);
+  tm = std::max<uint32_t>(tm, (uint32_t)std::min<uint64_t>(4 * plane, m32));
+  s->TM = (int)std::min<uint32_t>(tm, m32);
+}
+
+int set_device(fp_solver* s) {
+  FP_TRY(s, hipSetDevice(s->device));
+  return FP_OK;
+}
+
+// union-find with path halving, parent[ra] = rb (solver.js:488-499): the
+// labels are the reference's roots.
+void recompute_groups_host(fp_solver* s, const std::vector<uint8_t>& alive) {
+  const int n = s->ns;
+  std::vector<int32_t> parent(n);
+  for (int i = 0; i < n; i++) parent[i] = i;
+  auto find = [&](int x) {
+    while (parent[x] != x) {
+      parent[x] = parent[parent[x]];
+      x = parent[x];
+    }
+    return x;
+  };
+  for (int c = 0; c < s->C; c++) {
+    if (!alive[c]) continue;
+    const int ra = find(s->h_cons_a[c]), rb = find(s->h_cons_b[c]);
+    if (ra != rb) parent[ra] = rb;
+  }
+  s->h_group.resize(n);
+  for (int i = 0; i < n; i++) s->h_group[i] = find(i);
+}
+
+// Sync point: wait for the stream, pick up the tear flag of finished steps,
+// relabel the components when a tear or a cut happened (solver.js:238-241).
+int settle(fp_solver* s) {
+  FP_TRY(s, hipStreamSynchronize(s->stream));
+  if (s->profile) profile_collect(s);
+  if (s->steps_pending && s->C > 0) {
+    int tear = 0;
+    FP_TRY(s, d2h(s, &tear, s->dflags + DF_TEAR, sizeof(int)));
+    if (tear) {
+      s->groups_dirty = true;
+      FP_TRY(s, hipMemsetAsync(s->dflags + DF_TEAR, 0, sizeof(int), s->stream));
+    }
+    if (s->cut_dirty_at_step) s->groups_dirty = true;
+  }
+  s->cut_dirty_at_step = false;
+  s->steps_pending = false;
+  if (s->groups_dirty) {
+    std::vector<uint8_t> al(s->C);
+    if (s->C) FP_TRY(s, d2h(s, al.data(), s->alive[s->cur], s->C));
+    recompute_groups_host(s, al);
+    s->groups_dirty = false;
+  }
+  return FP_OK;
+}
+
+int check_params(fp_solver* s, const fp_params& p) {
+  const double rmax = std::max(p.solid_radius, p.fluid_radius);
+  if (!(p.cell_size > 0) || !(p.fluid_h > 0) || p.cell_size < p.fluid_h || p.cell_size < 2 * rmax)
+    return fp_fail(s, FP_EINVAL, "cell_size must be >= fluid_h and >= 2 * max(solid_radius, fluid_radius)");
+  if (p.table_size < 0 || (p.table_size & (p.table_size - 1)))
+    return fp_fail(s, FP_EINVAL, "table_size must be 0 or a power of two");
+  if (p.moved_table < 0 || (p.moved_table & (p.moved_table - 1)))
+    return fp_fail(s, FP_EINVAL, "moved_table must be 0 or a power of two");
+  if (p.constraint_iters < 0 || p.constraint_iters > 1000)
+    return fp_fail(s, FP_EINVAL, "constraint_iters out of range");
+  return FP_OK;
+}
+
+}  // namespace
+
+int fp_fail(fp_solver* s, int code, const std::string& msg) {
+  if (s)
+    s->err = msg;
+  else
+    g_last_error = msg;
+  return code;
+}
+
+hipEvent_t fp_event_get(fp_solver* s) {
+  if (!s->ev_pool.empty()) {
+    hipEvent_t e = s->ev_pool.back();
+    s->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+extern "C" {
+
+void fp_params_default(fp_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->gravity = -10.0;
+  p->solid_radius = 0.09;
+  p->fluid_radius = 0.075;
+  p->compliance = 2.5e-4;
+  p->plastic_threshold = 0.32;
+  p->plastic_rate = 0.04;
+  p->break_strain = 2.6;
+  p->fluid_h = 0.19;
+  p->rest_density = 6.0;
+  p->stiffness = 0.002;
+  p->near_stiffness = 0.008;
+  p->viscosity = 0.3;
+  p->solid_damping = 1.2;
+  p->fluid_damping = 0.6;
+  p->floor_friction_solid = 0.4;
+  p->floor_friction_fluid = 0.04;
+  p->wall_x = 3.5;
+  p->wall_z = 3.5;
+  p->wall_height = 0.92;
+  p->wall_half_thick = 0.07;
+  p->cull_radius = 7.0;
+  p->max_speed = 14.0;
+  p->grab_stiffness = 0.3;
+  p->cell_size = 0.22;
+  p->constraint_iters = 3;
+  p->table_size = 0;
+  p->moved_table = 0;
+  p->jacobi_omega = 4.0;
+  p->contact_scale = 1.0;
+}
+
+int fp_create(const fp_params* params, int32_t max_fluid, int32_t device, fp_solver** out) {
+  if (!out) return fp_fail(nullptr, FP_EINVAL, "fp_create: out is NULL");
+  *out = nullptr;
+  if (max_fluid < 0) return fp_fail(nullptr, FP_EINVAL, "fp_create: max_fluid < 0");
+  fp_solver* s = new (std::nothrow) fp_solver();
+  if (!s) return fp_fail(nullptr, FP_ENOMEM, "fp_create: out of host memory");
+  if (params)
+    s->P = *params;
+  else
+    fp_params_default(&s->P);
+  int rc = check_params(s, s->P);
+  if (rc) {
+    g_last_error = s->err;
+    delete s;
+    return rc;
+  }
+  if (device < 0 && hipGetDevice(&device) != hipSuccess) device = 0;
+  s->device = device;
+  s->max_fluid = max_fluid;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_last_error = std::string("fp_create: ") + hipGetErrorString(e);
+    delete s;
+    return FP_EDEVICE;
+  }
+  *out = s;
+  return FP_OK;
+}
+
+int fp_destroy(fp_solver* s) {
+  if (!s) return FP_OK;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  free_all(s);
+  for (auto& e : s->ev_live) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  for (auto e : s->ev_pool) (void)hipEventDestroy(e);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return FP_OK;
+}
+
+const char* fp_last_error(const fp_solver* s) { return s ? s->err.c_str() : g_last_error.c_str(); }
+
+int fp_set_params(fp_solver* s, const fp_params* p) {
+  if (!s || !p) return fp_fail(s, FP_EINVAL, "fp_set_params: NULL argument");
+  int rc = check_params(s, *p);
+  if (rc) return rc;
+  if (s->has_cube && (p->table_size != s->P.table_size || p->moved_table != s->P.moved_table))
+    return fp_fail(s, FP_ESTATE, "table_size / moved_table are fixed once the scene is allocated");
+  s->P = *p;
+  return FP_OK;
+}
+
+int fp_get_params(const fp_solver* s, fp_params* out) {
+  if (!s || !out) return FP_EINVAL;
+  *out = s->P;
+  return FP_OK;
+}
+
+int fp_get_info(const fp_solver* s, fp_info* out) {
+  if (!s || !out) return FP_EINVAL;
+  fp_solver* m = const_cast<fp_solver*>(s);
+  out->capacity = s->n;
+  out->solid_count = s->ns;
+  out->max_fluid = s->max_fluid;
+  out->table_size = s->T;
+  out->num_constraints = s->C;
+  out->fluid_next = s->fluid_next;
+  out->moved_table = s->TM;
+  out->device = s->device;
+  out->substeps_done = s->substeps_done;
+  out->moved_overflow = 0;
+  out->device_bytes = s->device_bytes;
+  if (s->dflags) {
+    (void)hipSetDevice(s->device);
+    int ov = 0;
+    if (d2h(m, &ov, s->dflags + DF_MOVED_OVF, sizeof(int)) == hipSuccess) out->moved_overflow = ov;
+  }
+  return FP_OK;
+}
+
+int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, double spacing,
+                      double particle_mass) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_add_solid_cube: NULL solver");
+  if (s->has_cube) return fp_fail(s, FP_ESTATE, "addSolidCube may be called once");
+  if (n < 1 || n > 1024) return fp_fail(s, FP_EINVAL, "cube dimension out of range");
+  const int64_t cap64 = (int64_t)n * n * n + s->max_fluid;
+  if (cap64 >= (1ll << 27)) return fp_fail(s, FP_EINVAL, "capacity exceeds 2^27 particles");
+  int rc = set_device(s);
+  if (rc) return rc;
+  const int ns = n * n * n;
+  s->ns = ns;
+  s->n = (int)cap64;
+  s->nf = s->max_fluid;
+  if (s->P.table_size > 0) {
+    s->T = s->P.table_size;
+  } else {
+    int t = 1;
+    while (t < 2 * s->n) t <<= 1;  // solver.js:51
+    s->T = t;
+  }
+  s->scan_blocks = (s->T + SCAN_TILE - 1) / SCAN_TILE;
+
+  // lattice (solver.js:74-89): double arithmetic, float32 stores
+  const double half = (n - 1) * spacing * 0.5;
+  std::vector<float4> hpos(s->n, make_float4(0.f, 0.f, 0.f, 0.f));
+  std::vector<uint8_t> hact(s->n, 0);
+  s->h_snap_pos.resize(3 * (size_t)ns);
+  const float inv_m = (float)(1 / particle_mass);
+  int id = 0;
+  for (int ix = 0; ix < n; ix++)
+    for (int iy = 0; iy < n; iy++)
+      for (int iz = 0; iz < n; iz++) {
+        const float x = (float)(cx - half + ix * spacing), y = (float)(cy - half + iy * spacing),
+                    z = (float)(cz - half + iz * spacing);
+        hpos[id] = make_float4(x, y, z, inv_m);
+        s->h_snap_pos[3 * id] = x;
+        s->h_snap_pos[3 * id + 1] = y;
+        s->h_snap_pos[3 * id + 2] = z;
+        hact[id] = 1;
+        id++;
+      }
+  // constraints in the reference's loop order (solver.js:97-116)
+  const double maxD2 = 3 * spacing * spacing + 1e-6;
+  std::vector<float> hrest;
+  s->h_cons_a.clear();
+  s->h_cons_b.clear();
+  auto idx = [n](int ax, int ay, int az) { return (ax * n + ay) * n + az; };
+  for (int ix = 0; ix < n; ix++)
+    for (int iy = 0; iy < n; iy++)
+      for (int iz = 0; iz < n; iz++)
+        for (int dx = -1; dx <= 1; dx++)
+          for (int dy = -1; dy <= 1; dy++)
+            for (int dz = -1; dz <= 1; dz++) {
+              const int jx = ix + dx, jy = iy + dy, jz = iz + dz;
+              if (jx < 0 || jy < 0 || jz < 0 || jx >= n || jy >= n || jz >= n) continue;
+              const int a = idx(ix, iy, iz), b = idx(jx, jy, jz);
+              if (b <= a) continue;
+              const double d2 = (dx * dx + dy * dy + dz * dz) * spacing * spacing;
+              if (d2 > maxD2) continue;
+              s->h_cons_a.push_back(a);
+              s->h_cons_b.push_back(b);
+              hrest.push_back((float)std::sqrt(d2));
+            }
+  const int C = (int)hrest.size();
+  s->C = C;
+  s->h_snap_rest = hrest;
+  // static CSR adjacency: entries (other endpoint, id, bits of 1/max(deg_a, deg_b))
+  std::vector<int> hoff(ns + 1, 0);
+  for (int c = 0; c < C; c++) {
+    hoff[s->h_cons_a[c] + 1]++;
+    hoff[s->h_cons_b[c] + 1]++;
+  }
+  for (int i = 0; i < ns; i++) hoff[i + 1] += hoff[i];
+  std::vector<int4> hadj(std::max(1, 2 * C));
+  std::vector<int> fill(ns, 0);
+  std::vector<int2> hab(std::max(1, C));
+  auto deg = [&](int i) { return hoff[i + 1] - hoff[i]; };
+  for (int c = 0; c < C; c++) {
+    const int a = s->h_cons_a[c], b = s->h_cons_b[c];
+    float sc = 1.f / (float)std::max(deg(a), deg(b));
+    int scb;
+    std::memcpy(&scb, &sc, 4);
+    hadj[hoff[a] + fill[a]++] = make_int4(b, c, scb, 0);
+    hadj[hoff[b] + fill[b]++] = make_int4(a, c, scb, 0);
+    hab[c] = make_int2(a, b);
+  }
+  s->h_group.assign(ns, 0);
+
+  setup_cell_grid(s);
+  const int N = s->n, T = s->T;
+  const size_t M = s->cg.M;
+  if ((rc = dalloc(s, &s->pos, N)) || (rc = dalloc(s, &s->prev, N)) || (rc = dalloc(s, &s->vel, N)) ||
+      (rc = dalloc(s, &s->active, N)) || (rc = dalloc(s, &s->rho, N)) || (rc = dalloc(s, &s->phash, N)) ||
+      (rc = dalloc(s, &s->lcount, M)) || (rc = dalloc(s, &s->lstart, M + 1)) ||
+      (rc = dalloc(s, &s->lblock_sums, s->lscan_blocks)) || (rc = dalloc(s, &s->ltmp, N)) ||
+      (rc = dalloc(s, &s->lkey, N)) || (rc = dalloc(s, &s->lslot, N)) || (rc = dalloc(s, &s->sid, N)) ||
+      (rc = dalloc(s, &s->slot, N)) || (rc = dalloc(s, &s->spos, N)) || (rc = dalloc(s, &s->spos2, N)) ||
+      (rc = dalloc(s, &s->svtmp, N)) || (rc = dalloc(s, &s->sphash, N)) || (rc = dalloc(s, &s->sP, N)) ||
+      (rc = dalloc(s, &s->mcount, s->TM)) || (rc = dalloc(s, &s->mlist, (size_t)s->TM * MCAP)) ||
+      (rc = dalloc(s, &s->sbox, 6)) ||
+      (rc = dalloc(s, &s->pslot, N)) || (rc = dalloc(s, &s->count, T)) ||
+      (rc = dalloc(s, &s->cell_start, (size_t)T + 1)) || (rc = dalloc(s, &s->entries_tmp, N)) ||
+      (rc = dalloc(s, &s->cell_entries, N)) || (rc = dalloc(s, &s->block_sums, s->scan_blocks)) ||
+      (rc = dalloc(s, &s->cons_ab, std::max(C, 1))) || (rc = dalloc(s, &s->rest[0], std::max(C, 1))) ||
+      (rc = dalloc(s, &s->rest[1], std::max(C, 1))) || (rc = dalloc(s, &s->alive[0], std::max(C, 1))) ||
+      (rc = dalloc(s, &s->alive[1], std::max(C, 1))) || (rc = dalloc(s, &s->adj_off, ns + 1)) ||
+      (rc = dalloc(s, &s->adj, std::max(2 * C, 1))) || (rc = dalloc(s, &s->spos_c[0], ns)) ||
+      (rc = dalloc(s, &s->spos_c[1], ns)) || (rc = dalloc(s, &s->dflags, DF_WORDS)) ||
+      (rc = dalloc(s, &s->dpick_t, 1)) || (rc = dalloc(s, &s->dpick_i, 1))) {
+    return rc;
+  }
+  {
+    // phash = "not in the grid" until the first build; an empty solid box
+    std::vector<uint32_t> none(N, 0xffffffffu);
+    FP_TRY(s, h2d(s, s->phash, none.data(), N * sizeof(uint32_t)));
+    const int box[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    FP_TRY(s, h2d(s, s->sbox, box, sizeof(box)));
+  }
+  std::vector<uint8_t> ones(std::max(C, 1), 1);
+  FP_TRY(s, h2d(s, s->pos, hpos.data(), N * sizeof(float4)));
+  FP_TRY(s, h2d(s, s->active, hact.data(), N));
+  if (C) {
+    FP_TRY(s, h2d(s, s->cons_ab, hab.data(), C * sizeof(int2)));
+    FP_TRY(s, h2d(s, s->rest[0], hrest.data(), C * sizeof(float)));
+    FP_TRY(s, h2d(s, s->alive[0], ones.data(), C));
+    FP_TRY(s, h2d(s, s->adj, hadj.data(), 2 * C * sizeof(int4)));
+  }
+  FP_TRY(s, h2d(s, s->adj_off, hoff.data(), (ns + 1) * sizeof(int)));
+  // prev stays zero until the first integrate (solver.js:43)
+  s->cur = 0;
+  s->has_cube = true;
+  return FP_OK;
+}
+
+int fp_emit(fp_solver* s, const double* pos, const double* vel, const double* mass, int32_t n) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_emit: NULL solver");
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_emit before fp_add_solid_cube");
+  if (n < 0 || (n > 0 && (!pos || !vel))) return fp_fail(s, FP_EINVAL, "fp_emit: bad arguments");
+  if (n == 0) return FP_OK;
+  if (s->max_fluid == 0) return fp_fail(s, FP_EINVAL, "fp_emit: maxFluid is 0");
+  int rc = set_device(s);
+  if (rc) return rc;
+  // only the last maxFluid emits of a batch survive the ring (solver.js:140-141)
+  const int first = n > s->max_fluid ? n - s->max_fluid : 0;
+  const int m = n - first;
+  if (m > s->emit_cap) {
+    FP_TRY(s, hipStreamSynchronize(s->stream));
+    dfree(s->emit_idx);
+    dfree(s->emit_p);
+    dfree(s->emit_v);
+    const int cap = std::max(m, 1024);
+    if ((rc = dalloc(s, &s->emit_idx, cap)) || (rc = dalloc(s, &s->emit_p, cap)) || (rc = dalloc(s, &s->emit_v, cap)))
+      return rc;
+    s->emit_cap = cap;
+  }
+  std::vector<int> idx(m);
+  std::vector<float4> hp(m), hv(m);
+  int next = (int)(((int64_t)s->fluid_next + first) % s->max_fluid);
+  for (int k = 0; k < m; k++) {
+    const int j = first + k;
+    const double ms = mass ? mass[j] : 0.03;
+    idx[k] = s->ns + next;
+    hp[k] = make_float4((float)pos[3 * j], (float)pos[3 * j + 1], (float)pos[3 * j + 2], (float)(1 / ms));
+    hv[k] = make_float4((float)vel[3 * j], (float)vel[3 * j + 1], (float)vel[3 * j + 2], 0.f);
+    next = (next + 1) % s->max_fluid;
+  }
+  s->fluid_next = (int)(((int64_t)s->fluid_next + n) % s->max_fluid);
+  FP_TRY(s, hipMemcpyAsync(s->emit_idx, idx.data(), m * sizeof(int), hipMemcpyHostToDevice, s->stream));
+  FP_TRY(s, hipMemcpyAsync(s->emit_p, hp.data(), m * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+  FP_TRY(s, hipMemcpyAsync(s->emit_v, hv.data(), m * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+  if ((rc = fp_launch_emit(s, m))) return rc;
+  // the host vectors and the staging buffers are reused: wait for the copies
+  FP_TRY(s, hipStreamSynchronize(s->stream));
+  return FP_OK;
+}
+
+int fp_reset_solid(fp_solver* s) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_reset_solid: NULL solver");
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_reset_solid before fp_add_solid_cube");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  const int ns = s->ns;
+  std::vector<float4> p(ns);
+  FP_TRY(s, d2h(s, p.data(), s->pos, ns * sizeof(float4)));
+  std::vector<float4> pv(ns), v(ns, make_float4(0.f, 0.f, 0.f, 0.f));
+  for (int i = 0; i < ns; i++) {
+    p[i] = make_float4(s->h_snap_pos[3 * i], s->h_snap_pos[3 * i + 1], s->h_snap_pos[3 * i + 2], p[i].w);
+    pv[i] = make_float4(p[i].x, p[i].y, p[i].z, 0.f);
+  }
+  FP_TRY(s, h2d(s, s->pos, p.data(), ns * sizeof(float4)));
+  FP_TRY(s, h2d(s, s->prev, pv.data(), ns * sizeof(float4)));
+  FP_TRY(s, h2d(s, s->vel, v.data(), ns * sizeof(float4)));
+  if (s->C) {
+    std::vector<uint8_t> ones(s->C, 1);
+    FP_TRY(s, h2d(s, s->rest[s->cur], s->h_snap_rest.data(), s->C * sizeof(float)));
+    FP_TRY(s, h2d(s, s->alive[s->cur], ones.data(), s->C));
+  }
+  s->h_group.assign(ns, 0);
+  s->groups_dirty = false;
+  s->cut_dirty = false;
+  s->grab_round_len.clear();
+  return FP_OK;
+}
+
+int fp_clear_fluid(fp_solver* s) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_clear_fluid: NULL solver");
+  if (!s->has_cube) return FP_OK;
+  int rc = set_device(s);
+  if (rc) return rc;
+  if (s->nf) FP_TRY(s, hipMemsetAsync(s->active + s->ns, 0, s->nf, s->stream));
+  return FP_OK;
+}
+
+static int step_impl(fp_solver* s, double dt, int32_t substeps, bool wait) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_step: NULL solver");
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_step before fp_add_solid_cube");
+  if (substeps < 1) return fp_fail(s, FP_EINVAL, "fp_step: substeps < 1");
+  int rc = set_device(s);
+  if (rc) return rc;
+  const double h = dt / substeps;
+  if (s->cut_dirty) {
+    s->cut_dirty_at_step = true;
+    s->cut_dirty = false;
+  }
+  for (int k = 0; k < substeps; k++)
+    if ((rc = fp_launch_substep(s, h))) return rc;
+  s->steps_pending = true;
+  return wait ? settle(s) : FP_OK;
+}
+
+int fp_step(fp_solver* s, double dt, int32_t substeps) { return step_impl(s, dt, substeps, true); }
+int fp_step_async(fp_solver* s, double dt, int32_t substeps) { return step_impl(s, dt, substeps, false); }
+
+int fp_sync(fp_solver* s) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_sync: NULL solver");
+  int rc = set_device(s);
+  if (rc) return rc;
+  return settle(s);
+}
+
+int fp_set_grabs(fp_solver* s, const int32_t* idx, const double* xyz, int32_t n) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_set_grabs: NULL solver");
+  if (n < 0 || (n > 0 && (!idx || !xyz))) return fp_fail(s, FP_EINVAL, "fp_set_grabs: bad arguments");
+  for (int g = 0; g < n; g++)
+    if (idx[g] < 0 || idx[g] >= s->n) return fp_fail(s, FP_EINVAL, "fp_set_grabs: particle index out of range");
+  int rc = set_device(s);
+  if (rc) return rc;
+  s->grab_round_len.clear();
+  if (n == 0) return FP_OK;
+  // rounds of distinct indices; an index listed k times is pulled in rounds 0..k-1 in list order
+  std::vector<int> round_of(n);
+  {
+    std::vector<std::pair<int, int>> key(n);
+    for (int g = 0; g < n; g++) key[g] = {idx[g], g};
+    std::sort(key.begin(), key.end());
+    for (int g = 0; g < n; g++)
+      round_of[key[g].second] = (g > 0 && key[g].first == key[g - 1].first) ? round_of[key[g - 1].second] + 1 : 0;
+  }
+  int rounds = 0;
+  for (int g = 0; g < n; g++) rounds = std::max(rounds, round_of[g] + 1);
+  std::vector<int> hidx;
+  std::vector<float4> htgt;
+  for (int r = 0; r < rounds; r++) {
+    int len = 0;
+    for (int g = 0; g < n; g++)
+      if (round_of[g] == r) {
+        hidx.push_back(idx[g]);
+        htgt.push_back(make_float4((float)xyz[3 * g], (float)xyz[3 * g + 1], (float)xyz[3 * g + 2], 0.f));
+        len++;
+      }
+    s->grab_round_len.push_back(len);
+  }
+  if (n > s->grab_cap) {
+    FP_TRY(s, hipStreamSynchronize(s->stream));
+    dfree(s->grab_idx);
+    dfree(s->grab_tgt);
+    const int cap = std::max(n, 256);
+    if ((rc = dalloc(s, &s->grab_idx, cap)) || (rc = dalloc(s, &s->grab_tgt, cap))) return rc;
+    s->grab_cap = cap;
+  }
+  FP_TRY(s, hipMemcpyAsync(s->grab_idx, hidx.data(), n * sizeof(int), hipMemcpyHostToDevice, s->stream));
+  FP_TRY(s, hipMemcpyAsync(s->grab_tgt, htgt.data(), n * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+  FP_TRY(s, hipStreamSynchronize(s->stream));
+  return FP_OK;
+}
+
+int fp_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[3], int32_t* ncut) {
+  if (!s || !o || !d1 || !d2) return fp_fail(s, FP_EINVAL, "fp_cut: NULL argument");
+  if (ncut) *ncut = 0;
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_cut before fp_add_solid_cube");
+  const double nx = d1[1] * d2[2] - d1[2] * d2[1];
+  const double ny = d1[2] * d2[0] - d1[0] * d2[2];
+  const double nz = d1[0] * d2[1] - d1[1] * d2[0];
+  const double nlen = std::sqrt(nx * nx + ny * ny + nz * nz);
+  if (nlen < 1e-8 || s->C == 0) return FP_OK;
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  FP_TRY(s, hipMemsetAsync(s->dflags + DF_CUT, 0, sizeof(int), s->stream));
+  if ((rc = fp_launch_cut(s, o, d1, d2, nx, ny, nz))) return rc;
+  int cnt = 0;
+  FP_TRY(s, d2h(s, &cnt, s->dflags + DF_CUT, sizeof(int)));
+  if (s->profile) profile_collect(s);
+  if (ncut) *ncut = cnt;
+  // the reference marks components dirty; they are relabelled at the end
+  // of the next step (solver.js:238-241, :386)
+  if (cnt) s->cut_dirty = true;
+  return FP_OK;
+}
+
+int fp_pick_solid(fp_solver* s, const double o[3], const double d[3], double max_perp, int32_t* id, double* t) {
+  if (!s || !o || !d || !id) return fp_fail(s, FP_EINVAL, "fp_pick_solid: NULL argument");
+  *id = -1;
+  if (t) *t = INFINITY;
+  if (!s->has_cube || s->ns == 0) return FP_OK;
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = fp_launch_pick(s, o, d, max_perp * max_perp))) return rc;
+  double ht = INFINITY;
+  int hi = -1;
+  FP_TRY(s, hipMemcpyAsync(&ht, s->dpick_t, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  FP_TRY(s, hipMemcpyAsync(&hi, s->dpick_i, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  if ((rc = settle(s))) return rc;
+  *id = hi;
+  if (t) *t = hi >= 0 ? ht : INFINITY;
+  return FP_OK;
+}
+
+int fp_fluid_count(fp_solver* s, int32_t* count) {
+  if (!s || !count) return fp_fail(s, FP_EINVAL, "fp_fluid_count: NULL argument");
+  *count = 0;
+  if (!s->has_cube || s->nf == 0) return FP_OK;
+  int rc = set_device(s);
+  if (rc) return rc;
+  FP_TRY(s, hipMemsetAsync(s->dflags + DF_SCRATCH, 0, sizeof(int), s->stream));
+  if ((rc = fp_launch_count_active(s))) return rc;
+  int c = 0;
+  FP_TRY(s, hipMemcpyAsync(&c, s->dflags + DF_SCRATCH, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  if ((rc = settle(s))) return rc;
+  *count = c;
+  return FP_OK;
+}
+
+int fp_read_state(fp_solver* s, float* pos, float* prev, float* vel, float* inv_mass, uint8_t* active,
+                  int32_t* group) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_read_state: NULL solver");
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_read_state before fp_add_solid_cube");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  const int N = s->n;
+  std::vector<float4> tmp(N);
+  if (pos || inv_mass) {
+    FP_TRY(s, d2h(s, tmp.data(), s->pos, N * sizeof(float4)));
+    for (int i = 0; i < N; i++) {
+      if (pos) {
+        pos[3 * i] = tmp[i].x;
+        pos[3 * i + 1] = tmp[i].y;
+        pos[3 * i + 2] = tmp[i].z;
+      }
+      if (inv_mass) inv_mass[i] = tmp[i].w;
+    }
+  }
+  float4* src[2] = {s->prev, s->vel};
+  float* dst[2] = {prev, vel};
+  for (int k = 0; k < 2; k++) {
+    if (!dst[k]) continue;
+    FP_TRY(s, d2h(s, tmp.data(), src[k], N * sizeof(float4)));
+    for (int i = 0; i < N; i++) {
+      dst[k][3 * i] = tmp[i].x;
+      dst[k][3 * i + 1] = tmp[i].y;
+      dst[k][3 * i + 2] = tmp[i].z;
+    }
+  }
+  if (active) FP_TRY(s, d2h(s, active, s->active, N));
+  if (group) std::memcpy(group, s->h_group.data(), s->ns * sizeof(int32_t));
+  return FP_OK;
+}
+
+int fp_write_state(fp_solver* s, const float* pos, const float* prev, const float* vel, const float* inv_mass,
+                   const uint8_t* active) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_write_state: NULL solver");
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_write_state before fp_add_solid_cube");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  const int N = s->n;
+  std::vector<float4> tmp(N);
+  if (pos || inv_mass) {
+    FP_TRY(s, d2h(s, tmp.data(), s->pos, N * sizeof(float4)));
+    for (int i = 0; i < N; i++) {
+      if (pos) tmp[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], tmp[i].w);
+      if (inv_mass) tmp[i].w = inv_mass[i];
+    }
+    FP_TRY(s, h2d(s, s->pos, tmp.data(), N * sizeof(float4)));
+  }
+  float4* dstp[2] = {s->prev, s->vel};
+  const float* srcp[2] = {prev, vel};
+  for (int k = 0; k < 2; k++) {
+    if (!srcp[k]) continue;
+    for (int i = 0; i < N; i++) tmp[i] = make_float4(srcp[k][3 * i], srcp[k][3 * i + 1], srcp[k][3 * i + 2], 0.f);
+    FP_TRY(s, h2d(s, dstp[k], tmp.data(), N * sizeof(float4)));
+  }
+  if (active) FP_TRY(s, h2d(s, s->active, active, N));
+  return FP_OK;
+}
+
+int fp_read_constraints(fp_solver* s, int32_t* a, int32_t* b, float* rest, uint8_t* alive) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_read_constraints: NULL solver");
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_read_constraints before fp_add_solid_cube");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  if (a) std::memcpy(a, s->h_cons_a.data(), s->C * sizeof(int32_t));
+  if (b) std::memcpy(b, s->h_cons_b.data(), s->C * sizeof(int32_t));
+  if (rest && s->C) FP_TRY(s, d2h(s, rest, s->rest[s->cur], s->C * sizeof(float)));
+  if (alive && s->C) FP_TRY(s, d2h(s, alive, s->alive[s->cur], s->C));
+  return FP_OK;
+}
+
+int fp_write_constraints(fp_solver* s, const float* rest, const uint8_t* alive) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_write_constraints: NULL solver");
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_write_constraints before fp_add_solid_cube");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  if (rest && s->C) FP_TRY(s, h2d(s, s->rest[s->cur], rest, s->C * sizeof(float)));
+  if (alive && s->C) FP_TRY(s, h2d(s, s->alive[s->cur], alive, s->C));
+  return FP_OK;
+}
+
+int fp_build_grid(fp_solver* s) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_build_grid: NULL solver");
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_build_grid before fp_add_solid_cube");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = fp_launch_ref_grid(s, true))) return rc;
+  return settle(s);
+}
+
+int fp_read_grid(fp_solver* s, int32_t* cell_start, int32_t* cell_entries) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_read_grid: NULL solver");
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_read_grid before fp_add_solid_cube");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if (!s->ref_grid_valid && (rc = fp_launch_ref_grid(s, false))) return rc;
+  if ((rc = settle(s))) return rc;
+  if (cell_start) FP_TRY(s, d2h(s, cell_start, s->cell_start, ((size_t)s->T + 1) * sizeof(int)));
+  if (cell_entries) FP_TRY(s, d2h(s, cell_entries, s->cell_entries, (size_t)s->n * sizeof(int)));
+  return FP_OK;
+}
+
+int fp_read_density(fp_solver* s, float* rho, float* rho_near) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_read_density: NULL solver");
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_read_density before fp_add_solid_cube");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  std::vector<float2> tmp(s->n);
+  FP_TRY(s, d2h(s, tmp.data(), s->rho, s->n * sizeof(float2)));
+  for (int i = 0; i < s->n; i++) {
+    if (rho) rho[i] = tmp[i].x;
+    if (rho_near) rho_near[i] = tmp[i].y;
+  }
+  return FP_OK;
+}
+
+int fp_profile_enable(fp_solver* s, int32_t enable) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_profile_enable: NULL solver");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  s->profile = enable != 0;
+  for (int k = 0; k < P_NUM; k++) {
+    s->prof_ms[k] = 0;
+    s->prof_n[k] = 0;
+  }
+  return FP_OK;
+}
+
+int fp_profile_read(fp_solver* s, fp_pass_stat* out, int32_t max, int32_t* n) {
+  if (!s || !out || !n) return fp_fail(s, FP_EINVAL, "fp_profile_read: NULL argument");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  int m = 0;
+  for (int k = 0; k < P_NUM && m < max; k++) {
+    if (!s->prof_n[k]) continue;
+    std::memset(out[m].name, 0, sizeof(out[m].name));
+    std::strncpy(out[m].name, kPassNames[k], sizeof(out[m].name) - 1);
+    out[m].launches = s->prof_n[k];
+    out[m].total_ms = s->prof_ms[k];
+    m++;
+  }
+  *n = m;
+  return FP_OK;
+}
+
+}  // extern "C"

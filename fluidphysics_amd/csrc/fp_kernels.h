@@ -77,6 +77,29 @@ __device__ __forceinline__ void probe_hashes(const Cell& c, uint32_t mask, uint3
       for (int d = 0; d < 3; d++) h[a * 9 + b * 3 + d] = (X[a] ^ Y[b] ^ Z[d]) & mask;
 }
 
+// Does any pair of the 27 probes share a bucket?  Probes (a,b,c) and
+// (a',b',c') collide iff (X[a]^X[a']) ^ (Y[b]^Y[b']) ^ (Z[c]^Z[c']) has no
+// bits under the mask, so 63 XOR tests over the per-axis differences replace
+// the 351 pairwise comparisons.
+__device__ __forceinline__ bool probe_any_dup(const Cell& c, uint32_t mask) {
+  uint32_t X[3], Y[3], Z[3];
+  axis_terms(c.x, FP_K1, X);
+  axis_terms(c.y, FP_K2, Y);
+  axis_terms(c.z, FP_K3, Z);
+  const uint32_t dx[4] = {0u, (X[0] ^ X[1]) & mask, (X[0] ^ X[2]) & mask, (X[1] ^ X[2]) & mask};
+  const uint32_t dy[4] = {0u, (Y[0] ^ Y[1]) & mask, (Y[0] ^ Y[2]) & mask, (Y[1] ^ Y[2]) & mask};
+  const uint32_t dz[4] = {0u, (Z[0] ^ Z[1]) & mask, (Z[0] ^ Z[2]) & mask, (Z[1] ^ Z[2]) & mask};
+  bool hit = false;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int d = 0; d < 4; d++)
+        if (a + b + d > 0) hit |= ((dx[a] ^ dy[b] ^ dz[d]) == 0u);
+  return hit;
+}
+
 // Bit k set when probe k hits the same bucket as an earlier probe.
 __device__ __forceinline__ uint32_t probe_dup_mask(const uint32_t h[27]) {
   uint32_t m = 0;
@@ -104,6 +127,67 @@ __device__ __forceinline__ int probe_multiplicity(const Cell& c, uint32_t mask, 
 #pragma unroll
       for (int d = 0; d < 3; d++) n += (((X[a] ^ Y[b] ^ Z[d]) & mask) == target) ? 1 : 0;
   return n;
+}
+
+// ---- the cell grid of the hot path -------------------------------------------
+
+// Key of cell c: (cx * NY*NZ + cy * NZ + cz) mod M with 32-bit wrap; the three
+// z-neighbours of a cell are consecutive keys.  Coordinates beyond +-2^30
+// (and NaN) share one sentinel cell, so such particles still meet each other.
+struct CellGrid {
+  double inv_cell;
+  uint32_t hmask;      // reference hash table: T - 1
+  uint32_t M, kmask;   // cell table size (power of 2), M - 1
+  uint32_t nz_shift;   // log2 NZ
+  uint32_t nyz_shift;  // log2 (NY * NZ)
+};
+
+__device__ __forceinline__ uint32_t cell_coord_u32(double c) {
+  return fabs(c) < 1073741824.0 ? (uint32_t)(int32_t)c : 0x40000000u;
+}
+
+__device__ __forceinline__ uint32_t cell_key(const CellGrid& g, const Cell& c) {
+  return ((cell_coord_u32(c.x) << g.nyz_shift) + (cell_coord_u32(c.y) << g.nz_shift) + cell_coord_u32(c.z)) &
+         g.kmask;
+}
+
+// First key of the z-run of neighbour row (a, b) in {-1,0,1}^2 of key k.
+__device__ __forceinline__ uint32_t row_key(const CellGrid& g, uint32_t k, int a, int b) {
+  return (k + ((uint32_t)a << g.nyz_shift) + ((uint32_t)b << g.nz_shift) - 1u) & g.kmask;
+}
+
+// ---- solids' bounding box ------------------------------------------------------
+
+// Floats as ints whose signed order is the floats' order.
+__device__ __forceinline__ int order_float(float f) {
+  const int b = __float_as_int(f);
+  return b >= 0 ? b : b ^ 0x7fffffff;
+}
+
+// Fold the solid positions of this wave into box[0..5] (min xyz, max xyz,
+// ordered-int bits): a wave64 min/max, then one atomic per wave.  Every lane
+// of the wave must call it.
+__device__ __forceinline__ void box_update(bool has, float px, float py, float pz, int* __restrict__ box) {
+  if (!__any(has)) return;
+  float mnx = has ? px : INFINITY, mny = has ? py : INFINITY, mnz = has ? pz : INFINITY;
+  float mxx = has ? px : -INFINITY, mxy = has ? py : -INFINITY, mxz = has ? pz : -INFINITY;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    mnx = fminf(mnx, __shfl_xor(mnx, off));
+    mny = fminf(mny, __shfl_xor(mny, off));
+    mnz = fminf(mnz, __shfl_xor(mnz, off));
+    mxx = fmaxf(mxx, __shfl_xor(mxx, off));
+    mxy = fmaxf(mxy, __shfl_xor(mxy, off));
+    mxz = fmaxf(mxz, __shfl_xor(mxz, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&box[0], order_float(mnx));
+    atomicMin(&box[1], order_float(mny));
+    atomicMin(&box[2], order_float(mnz));
+    atomicMax(&box[3], order_float(mxx));
+    atomicMax(&box[4], order_float(mxy));
+    atomicMax(&box[5], order_float(mxz));
+  }
 }
 
 // ---- wave64 scan primitives ------------------------------------------------

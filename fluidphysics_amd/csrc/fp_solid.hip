@@ -1,0 +1,196 @@
+// fp_solid.hip -- distance constraints, the knife, pick.
+//
+// constraints  one Jacobi XPBD iteration over the static CSR adjacency
+//              (solver.js:356-381 per-constraint formula).  32 lanes work on
+//              one solid: lane k takes the solid's k-th incident constraint,
+//              the corrections are summed across the 32 lanes with xor
+//              shuffles.  Each constraint's correction is scaled by
+//              omega / max(deg_a, deg_b) -- the same factor at both
+//              endpoints, so the pair's corrections stay equal and opposite.
+//              Tearing (d > rest * breakStrain) and plasticity (|C/rest| >
+//              threshold -> rest += C * rate) are decided from the iteration's
+//              snapshot, so both endpoints agree; the first endpoint (consA)
+//              writes the next rest / alive.
+// cut          the knife wedge test of solver.js:458-486 in IEEE double with
+//              the reference's operation order (the library is built with
+//              -ffp-contract=off), one thread per constraint.
+// pick         pickSolid (solver.js:502-515): one workgroup, lexicographic
+//              minimum of (t, index) over the solids that pass both tests.
+#include "fp_internal.h"
+#include "fp_kernels.h"
+
+#define CLANES 32
+
+__global__ void __launch_bounds__(BLOCK) k_constraints(ConsParams cp, const float4* __restrict__ sin,
+                                                       float4* __restrict__ sout, const int* __restrict__ adj_off,
+                                                       const int4* __restrict__ adj,
+                                                       const float* __restrict__ rest_in,
+                                                       const uint8_t* __restrict__ alive_in,
+                                                       float* __restrict__ rest_out, uint8_t* __restrict__ alive_out,
+                                                       int* __restrict__ dflags) {
+  const int lane = threadIdx.x & (CLANES - 1);
+  const int a = (blockIdx.x * BLOCK + threadIdx.x) / CLANES;
+  const bool valid = a < cp.ns;
+  float4 pa = make_float4(0.f, 0.f, 0.f, 0.f);
+  int k0 = 0, k1 = 0;
+  if (valid) {
+    pa = sin[a];
+    k0 = adj_off[a];
+    k1 = adj_off[a + 1];
+  }
+  float cx = 0.f, cy = 0.f, cz = 0.f;
+  bool tore = false;
+  for (int k = k0 + lane; k < k1; k += CLANES) {
+    const int4 e = adj[k];
+    const int b = e.x, c = e.y;
+    const bool owner = a < b;  // the lattice builder always has consA < consB
+    const float rest = rest_in[c];
+    if (!alive_in[c]) {
+      if (owner) {
+        rest_out[c] = rest;
+        alive_out[c] = 0;
+      }
+      continue;
+    }
+    const float4 pb = sin[b];
+    const float w = pa.w + pb.w;
+    const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+    const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+    float nrest = rest;
+    uint8_t nal = 1;
+    if (w != 0.f && d >= 1e-9f) {
+      if (d > rest * cp.break_strain) {
+        nal = 0;
+        tore = true;
+      } else {
+        const float C = d - rest;
+        const float dl = -C / (w + cp.alpha);
+        const float s = dl / d * pa.w * (cp.omega * __int_as_float(e.z));
+        cx += dx * s;
+        cy += dy * s;
+        cz += dz * s;
+        if (fabsf(C / rest) > cp.plastic_thr) nrest = rest + C * cp.plastic_rate;
+      }
+    }
+    if (owner) {
+      rest_out[c] = nrest;
+      alive_out[c] = nal;
+    }
+  }
+#pragma unroll
+  for (int off = CLANES / 2; off > 0; off >>= 1) {
+    cx += __shfl_xor(cx, off, CLANES);
+    cy += __shfl_xor(cy, off, CLANES);
+    cz += __shfl_xor(cz, off, CLANES);
+  }
+  if (valid && lane == 0) sout[a] = make_float4(pa.x + cx, pa.y + cy, pa.z + cz, pa.w);
+  if (tore) dflags[DF_TEAR] = 1;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cut(int nc, const float4* __restrict__ pos, const int2* __restrict__ ab,
+                                               uint8_t* __restrict__ alive, double ox, double oy, double oz,
+                                               double d1x, double d1y, double d1z, double d2x, double d2y,
+                                               double d2z, double nx, double ny, double nz, int* __restrict__ ncut) {
+  const int c = blockIdx.x * BLOCK + threadIdx.x;
+  if (c >= nc || !alive[c]) return;
+  const int2 e = ab[c];
+  const float4 fa = pos[e.x], fb = pos[e.y];
+  const double ax = fa.x, ay = fa.y, az = fa.z, bx = fb.x, by = fb.y, bz = fb.z;
+  const double sa = (ax - ox) * nx + (ay - oy) * ny + (az - oz) * nz;
+  const double sb = (bx - ox) * nx + (by - oy) * ny + (bz - oz) * nz;
+  if (sa * sb >= 0) return;
+  const double t = sa / (sa - sb);
+  const double px = ax + (bx - ax) * t - ox;
+  const double py = ay + (by - ay) * t - oy;
+  const double pz = az + (bz - az) * t - oz;
+  const double c1 = (d1y * pz - d1z * py) * nx + (d1z * px - d1x * pz) * ny + (d1x * py - d1y * px) * nz;
+  const double c2 = (py * d2z - pz * d2y) * nx + (pz * d2x - px * d2z) * ny + (px * d2y - py * d2x) * nz;
+  if (c1 < 0 || c2 < 0) return;
+  alive[c] = 0;
+  atomicAdd(ncut, 1);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_pick(int ns, const float4* __restrict__ pos, double ox, double oy,
+                                                double oz, double dx, double dy, double dz, double max_perp2,
+                                                double* __restrict__ out_t, int* __restrict__ out_i) {
+  __shared__ double st[BLOCK];
+  __shared__ int si[BLOCK];
+  double bt = INFINITY;
+  int bi = -1;
+  for (int i = threadIdx.x; i < ns; i += BLOCK) {
+    const float4 p = pos[i];
+    const double px = (double)p.x - ox, py = (double)p.y - oy, pz = (double)p.z - oz;
+    const double t = px * dx + py * dy + pz * dz;
+    if (t < 0 || t >= bt) continue;
+    const double qx = px - dx * t, qy = py - dy * t, qz = pz - dz * t;
+    if (qx * qx + qy * qy + qz * qz > max_perp2) continue;
+    bt = t;
+    bi = i;
+  }
+  st[threadIdx.x] = bt;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double t2 = st[threadIdx.x + s];
+      const int i2 = si[threadIdx.x + s];
+      const double t1 = st[threadIdx.x];
+      const int i1 = si[threadIdx.x];
+      if (i2 >= 0 && (i1 < 0 || t2 < t1 || (t2 == t1 && i2 < i1))) {
+        st[threadIdx.x] = t2;
+        si[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *out_t = st[0];
+    *out_i = si[0];
+  }
+}
+
+// iters Jacobi iterations; *solid_src receives the buffer holding the
+// solids' final positions (pos itself when there is nothing to solve).
+int fp_launch_constraints(fp_solver* s, double h, const float4** solid_src) {
+  *solid_src = s->pos;
+  if (s->ns == 0 || s->C == 0) return FP_OK;
+  const fp_params& P = s->P;
+  const int iters = P.constraint_iters > 0 ? P.constraint_iters : 1;
+  ConsParams cp;
+  cp.ns = s->ns;
+  cp.alpha = (float)(P.compliance / (h * h));
+  cp.omega = (float)P.jacobi_omega;
+  cp.break_strain = (float)P.break_strain;
+  cp.plastic_thr = (float)P.plastic_threshold;
+  cp.plastic_rate = (float)P.plastic_rate;
+  const int blocks = (int)(((int64_t)s->ns * CLANES + BLOCK - 1) / BLOCK);
+  PassScope ps(s, P_CONSTRAINTS);
+  for (int it = 0; it < iters; it++) {
+    const float4* in = it == 0 ? s->pos : s->spos_c[(it - 1) & 1];
+    float4* out = s->spos_c[it & 1];
+    hipLaunchKernelGGL(k_constraints, dim3(blocks), dim3(BLOCK), 0, s->stream, cp, in, out, s->adj_off, s->adj,
+                       s->rest[s->cur], s->alive[s->cur], s->rest[s->cur ^ 1], s->alive[s->cur ^ 1], s->dflags);
+    s->cur ^= 1;
+    *solid_src = out;
+  }
+  FP_TRY(s, hipGetLastError());
+  return FP_OK;
+}
+
+int fp_launch_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[3], double nx, double ny,
+                  double nz) {
+  PassScope ps(s, P_CUT);
+  hipLaunchKernelGGL(k_cut, dim3(fp_blocks(s->C)), dim3(BLOCK), 0, s->stream, s->C, s->pos, s->cons_ab,
+                     s->alive[s->cur], o[0], o[1], o[2], d1[0], d1[1], d1[2], d2[0], d2[1], d2[2], nx, ny, nz,
+                     s->dflags + DF_CUT);
+  FP_TRY(s, hipGetLastError());
+  return FP_OK;
+}
+
+int fp_launch_pick(fp_solver* s, const double o[3], const double d[3], double max_perp2) {
+  PassScope ps(s, P_PICK);
+  hipLaunchKernelGGL(k_pick, dim3(1), dim3(BLOCK), 0, s->stream, s->ns, s->pos, o[0], o[1], o[2], d[0], d[1], d[2],
+                     max_perp2, s->dpick_t, s->dpick_i);
+  FP_TRY(s, hipGetLastError());
+  return FP_OK;
+}

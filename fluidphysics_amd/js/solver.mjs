@@ -1,0 +1,199 @@
+// solver.mjs -- drop-in for the reference CPU solver's module (src/solver.js):
+// `export class Solver`, `FLUID`, `SOLID`, the same method names, defaults,
+// return shapes and mutable tuning fields, with every substep solved by the
+// HIP kernels of libfluidphysics.so through the N-API addon (fp_napi.node).
+//
+//   import { Solver } from './fluidphysics_amd/js/solver.mjs';
+//   const s = new Solver({ maxFluid: 2600 });
+//   s.addSolidCube(0, 2.0, 0, 7, 0.18);
+//   s.emitFluid(x, y, z, vx, vy, vz);
+//   s.grabs = [{ i, x, y, z }];
+//   s.step(1 / 60, 5);
+//   const n = s.cut(ox, oy, oz, d1x, d1y, d1z, d2x, d2y, d2z);
+//   const hit = s.pickSolid(ox, oy, oz, dx, dy, dz, 0.2);   // { id, t } | null
+//   s.pos, s.vel, s.active, s.group                          // typed arrays
+//
+// Differences from the reference, all at the boundary:
+//  * pos / prev / vel / invMass / active / group are refreshed from the
+//    device on first access after a step (one device->host copy, then cached
+//    until the next mutating call).  Writing into them does not reach the
+//    device; use writeState() for that.
+//  * emitFluid is buffered and sent in one batch before the next call that
+//    needs it.
+//  * device errors throw (the reference has none to throw).
+import { createRequire } from 'module';
+import { fileURLToPath } from 'url';
+import path from 'path';
+
+const require = createRequire(import.meta.url);
+const HERE = path.dirname(fileURLToPath(import.meta.url));
+const addon = require(path.join(HERE, '..', 'addon', 'fp_napi.node'));
+
+export const FLUID = 0;
+export const SOLID = 1;
+
+// reference field names (src/solver.js:14-37) forwarded to fp_params
+const TUNING = ['gravity', 'solidRadius', 'fluidRadius', 'compliance', 'plasticThreshold',
+  'plasticRate', 'breakStrain', 'fluidH', 'restDensity', 'stiffness', 'nearStiffness',
+  'viscosity', 'solidDamping', 'fluidDamping', 'floorFrictionSolid', 'floorFrictionFluid',
+  'cullRadius', 'maxSpeed', 'grabStiffness', 'cellSize', 'constraintIters', 'jacobiOmega'];
+
+export class Solver {
+  constructor({ maxFluid = 2600, params = {}, device = -1 } = {}) {
+    this.maxFluid = maxFluid;
+    this.solidCount = 0;
+    this.capacity = 0;
+    this._p = Object.assign(addon.defaults(), params);
+    this._h = addon.create(maxFluid, this._p, device);
+    this._dirtyParams = false;
+    this._emits = [];
+    this._views = null;
+    this.grabs = [];
+    for (const k of TUNING) {
+      Object.defineProperty(this, k, {
+        get: () => this._p[k],
+        set: (v) => { this._p[k] = v; this._dirtyParams = true; },
+        enumerable: true,
+      });
+    }
+    const p = this._p;
+    const self = this;
+    this.wall = {
+      get x() { return p.wallX; }, set x(v) { p.wallX = v; self._dirtyParams = true; },
+      get z() { return p.wallZ; }, set z(v) { p.wallZ = v; self._dirtyParams = true; },
+      get height() { return p.wallHeight; }, set height(v) { p.wallHeight = v; self._dirtyParams = true; },
+      get halfThick() { return p.wallHalfThick; }, set halfThick(v) { p.wallHalfThick = v; self._dirtyParams = true; },
+    };
+  }
+
+  _flush() {
+    if (this._dirtyParams) { addon.setParams(this._h, this._p); this._dirtyParams = false; }
+    if (this._emits.length) {
+      const n = this._emits.length;
+      const pos = new Float64Array(3 * n), vel = new Float64Array(3 * n), mass = new Float64Array(n);
+      for (let k = 0; k < n; k++) {
+        const e = this._emits[k];
+        pos[3 * k] = e[0]; pos[3 * k + 1] = e[1]; pos[3 * k + 2] = e[2];
+        vel[3 * k] = e[3]; vel[3 * k + 1] = e[4]; vel[3 * k + 2] = e[5];
+        mass[k] = e[6];
+      }
+      this._emits = [];
+      addon.emit(this._h, pos, vel, mass, n);
+    }
+  }
+
+  _stale() { this._views = null; }
+
+  // addSolidCube (src/solver.js:72-123)
+  addSolidCube(cx, cy, cz, n = 7, spacing = 0.18, particleMass = 0.08) {
+    this._flush();
+    addon.addSolidCube(this._h, cx, cy, cz, n, spacing, particleMass);
+    const i = addon.info(this._h);
+    this.solidCount = i.solidCount;
+    this.capacity = i.capacity;
+    this.tableSize = i.tableSize;
+    this._stale();
+  }
+
+  // emitFluid (src/solver.js:139-149)
+  emitFluid(x, y, z, vx, vy, vz, mass = 0.03) {
+    this._emits.push([x, y, z, vx, vy, vz, mass]);
+    this._stale();
+  }
+
+  // emitFluid over arrays in one transfer: pos/vel Float64Array(3n), mass number|Float64Array(n)
+  emitBatch(pos, vel, mass = 0.03) {
+    this._flush();
+    const n = pos.length / 3;
+    const m = typeof mass === 'number' ? new Float64Array(n).fill(mass) : mass;
+    addon.emit(this._h, pos, vel, m, n);
+    this._stale();
+  }
+
+  resetSolid() { this._flush(); addon.resetSolid(this._h); this.grabs.length = 0; this._stale(); }
+
+  clearFluid() { this._flush(); addon.clearFluid(this._h); this._stale(); }
+
+  fluidCount() { this._flush(); return addon.fluidCount(this._h); }
+
+  // step (src/solver.js:187-242); wait = false leaves the wait to sync()
+  step(dt, substeps = 5, wait = true) {
+    this._flush();
+    const g = this.grabs;
+    const idx = new Int32Array(g.length), xyz = new Float64Array(3 * g.length);
+    for (let k = 0; k < g.length; k++) {
+      idx[k] = g[k].i; xyz[3 * k] = g[k].x; xyz[3 * k + 1] = g[k].y; xyz[3 * k + 2] = g[k].z;
+    }
+    addon.setGrabs(this._h, idx, xyz, g.length);
+    addon.step(this._h, dt, substeps, wait ? 1 : 0);
+    this._stale();
+  }
+
+  sync() { addon.sync(this._h); }
+
+  // cut (src/solver.js:458-486)
+  cut(ox, oy, oz, d1x, d1y, d1z, d2x, d2y, d2z) {
+    this._flush();
+    const n = addon.cut(this._h, ox, oy, oz, d1x, d1y, d1z, d2x, d2y, d2z);
+    this._stale();
+    return n;
+  }
+
+  // pickSolid (src/solver.js:502-515)
+  pickSolid(ox, oy, oz, dx, dy, dz, maxPerp = 0.2) {
+    this._flush();
+    return addon.pickSolid(this._h, ox, oy, oz, dx, dy, dz, maxPerp);
+  }
+
+  _state() {
+    if (!this._views) {
+      this._flush();
+      const n = this.capacity;
+      const v = {
+        pos: new Float32Array(3 * n), prev: new Float32Array(3 * n), vel: new Float32Array(3 * n),
+        invMass: new Float32Array(n), active: new Uint8Array(n), group: new Int32Array(this.solidCount),
+      };
+      addon.readState(this._h, v.pos, v.prev, v.vel, v.invMass, v.active, v.group);
+      this._views = v;
+    }
+    return this._views;
+  }
+
+  get pos() { return this._state().pos; }
+  get prev() { return this._state().prev; }
+  get vel() { return this._state().vel; }
+  get invMass() { return this._state().invMass; }
+  get active() { return this._state().active; }
+  get group() { return this._state().group; }
+  get type() {
+    const t = new Uint8Array(this.capacity);
+    t.fill(SOLID, 0, this.solidCount);
+    return t;
+  }
+
+  get consAlive() {
+    const c = addon.info(this._h).numConstraints;
+    const alive = new Uint8Array(c);
+    addon.readConstraints(this._h, null, null, null, alive);
+    return alive;
+  }
+
+  writeState({ pos = null, prev = null, vel = null, invMass = null, active = null } = {}) {
+    this._flush();
+    addon.writeState(this._h, pos, prev, vel, invMass, active);
+    this._stale();
+  }
+
+  readGrid() {
+    const i = addon.info(this._h);
+    const cs = new Int32Array(i.tableSize + 1), ce = new Int32Array(i.capacity);
+    addon.readGrid(this._h, cs, ce);
+    return { cellStart: cs, cellEntries: ce };
+  }
+
+  _buildGrid() { this._flush(); addon.buildGrid(this._h); }
+
+  info() { return addon.info(this._h); }
+
+  destroy() { if (this._h) { addon.destroy(this._h); this._h = null; } }
+}

@@ -67,8 +67,8 @@ typedef struct fp_params {
   int32_t constraint_iters;    /* Jacobi XPBD iterations per substep (the reference runs one
                                   Gauss-Seidel sweep, :215); default 3 */
   int32_t table_size;          /* hash table size, power of two; 0 = nextpow2(2*capacity) (:51) */
-  int32_t max_neighbors;       /* per-fluid-particle neighbour-list capacity; 0 = 96.  The
-                                  reference caps the global pair list at 32*maxFluid (:57) */
+  int32_t moved_table;         /* buckets of the table of fluid particles that contacts moved to
+                                  another cell, power of two; 0 = nextpow2(capacity/8), >= 1024 */
   int32_t reserved0;
   double jacobi_omega;         /* Jacobi relaxation: each constraint's correction is scaled by
                                   jacobi_omega / max(deg_a, deg_b); default 4.0 */
@@ -85,10 +85,11 @@ typedef struct fp_info {
   int32_t table_size;          /* :51 */
   int32_t num_constraints;     /* consA.length */
   int32_t fluid_next;          /* ring cursor of emitFluid (:140-141) */
-  int32_t max_neighbors;       /* neighbour-list capacity per fluid particle */
+  int32_t moved_table;         /* moved-particle table size in use */
   int32_t device;              /* HIP device ordinal */
   int64_t substeps_done;       /* substeps executed since creation */
-  int64_t neighbor_overflow;   /* fluid particles whose neighbour list was truncated (cumulative) */
+  int64_t moved_overflow;      /* moved-table insertions that did not fit a bucket (cumulative;
+                                  non-zero means pairs of those particles were missed) */
   int64_t device_bytes;        /* HBM bytes held by this solver */
 } fp_info;
 

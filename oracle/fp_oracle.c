@@ -136,7 +136,7 @@ void or_params_default(fp_params* p) {
   p->cell_size = 0.22;
   p->constraint_iters = 1;
   p->table_size = 0;
-  p->max_neighbors = 0;
+  p->moved_table = 0;
   p->jacobi_omega = 1.0;
   p->contact_scale = 1.0;
 }
@@ -831,6 +831,28 @@ void or_substep(Oracle* o, double h) {
     }
   }
   apply_viscosity(o);
+}
+
+/* Test hook: the first stages of a substep (integrate, grab, grid, contacts),
+ * leaving the post-contact positions in pos -- what the fluid passes see. */
+void or_substep_until_contacts(Oracle* o, double h) {
+  ST *p = o->pos, *pv = o->prev, *v = o->vel;
+  for (int i = 0; i < o->capacity; i++) {
+    if (!o->active[i]) continue;
+    const int i3 = i * 3;
+    v[i3 + 1] = (ST)(v[i3 + 1] + o->P.gravity * h);
+    pv[i3] = p[i3];
+    pv[i3 + 1] = p[i3 + 1];
+    pv[i3 + 2] = p[i3 + 2];
+    p[i3] = (ST)(p[i3] + (double)v[i3] * h);
+    p[i3 + 1] = (ST)(p[i3 + 1] + (double)v[i3 + 1] * h);
+    p[i3 + 2] = (ST)(p[i3 + 2] + (double)v[i3 + 2] * h);
+  }
+  or_build_grid(o);
+  if (o->mode == OR_MODE_JACOBI)
+    solve_contacts_jacobi(o);
+  else
+    solve_contacts_gs(o);
 }
 
 void or_step(Oracle* o, double dt, int substeps) {

@@ -53,6 +53,7 @@ def _lib(storage):
         "or_set_grabs": (None, [P, IP, DP, I]),
         "or_build_grid": (None, [P]),
         "or_substep": (None, [P, D]),
+        "or_substep_until_contacts": (None, [P, D]),
         "or_step": (None, [P, D, I]),
         "or_recompute_groups": (None, [P]),
         "or_cut": (I, [P, D, D, D, D, D, D, D, D, D]),
@@ -150,6 +151,9 @@ class OracleSolver:
 
     def substep(self, h):
         self.lib.or_substep(self.h, h)
+
+    def substep_until_contacts(self, h):
+        self.lib.or_substep_until_contacts(self.h, h)
 
     def step(self, dt=1.0 / 60.0, substeps=5):
         self.lib.or_step(self.h, dt, int(substeps))
