@@ -189,6 +189,7 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
                 "bytes_per_launch": int(bytes_dom), "avg_launch_us": round(avg_s * 1e6, 3)}
+    inf_end = solver.info()
     b_sub = model_bytes_per_substep(n_mean, nf, ns, c, t, iters)
     model_frac = b_sub * (value / n_mean) / (HBM_PEAK_GBS * 1e9)
 
@@ -219,6 +220,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "passes": passes,
+        "pair_slots": inf_end.pair_slots,
+        "pair_overflow": inf_end.pair_overflow,
     }
     print(json.dumps(out))
 

@@ -47,7 +47,7 @@ class FpParams(ctypes.Structure):
         ("cell_size", ctypes.c_double),
         ("constraint_iters", ctypes.c_int32),
         ("table_size", ctypes.c_int32),
-        ("moved_table", ctypes.c_int32),
+        ("pair_slots", ctypes.c_int32),
         ("reserved0", ctypes.c_int32),
         ("jacobi_omega", ctypes.c_double),
         ("contact_scale", ctypes.c_double),
@@ -73,8 +73,8 @@ REFERENCE_DEFAULTS = dict(
     solid_damping=1.2, fluid_damping=0.6, floor_friction_solid=0.4,
     floor_friction_fluid=0.04, wall_x=3.5, wall_z=3.5, wall_height=0.92,
     wall_half_thick=0.07, cull_radius=7.0, max_speed=14.0, grab_stiffness=0.3,
-    cell_size=0.22, constraint_iters=3, table_size=0, moved_table=0,
-    reserved0=0, jacobi_omega=4.0, contact_scale=1.0,
+    cell_size=0.22, constraint_iters=3, table_size=0,
+    pair_slots=0, reserved0=0, jacobi_omega=4.0, contact_scale=1.0,
 )
 
 
@@ -93,11 +93,11 @@ class FpInfo(ctypes.Structure):
         ("table_size", ctypes.c_int32),
         ("num_constraints", ctypes.c_int32),
         ("fluid_next", ctypes.c_int32),
-        ("moved_table", ctypes.c_int32),
         ("device", ctypes.c_int32),
+        ("pair_slots", ctypes.c_int32),
         ("substeps_done", ctypes.c_int64),
-        ("moved_overflow", ctypes.c_int64),
         ("device_bytes", ctypes.c_int64),
+        ("pair_overflow", ctypes.c_int64),
     ]
 
 

@@ -142,7 +142,7 @@ const Field kFields[] = {
     FD("cellSize", cell_size),
     FI("constraintIters", constraint_iters),
     FI("tableSize", table_size),
-    FI("movedTable", moved_table),
+    FI("pairSlots", pair_slots),
     FD("jacobiOmega", jacobi_omega),
     FD("contactScale", contact_scale),
 };
@@ -249,10 +249,10 @@ napi_value Info(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, o, "tableSize", make_int(env, i.table_size));
   napi_set_named_property(env, o, "numConstraints", make_int(env, i.num_constraints));
   napi_set_named_property(env, o, "fluidNext", make_int(env, i.fluid_next));
-  napi_set_named_property(env, o, "movedTable", make_int(env, i.moved_table));
   napi_set_named_property(env, o, "substepsDone", make_int(env, i.substeps_done));
-  napi_set_named_property(env, o, "movedOverflow", make_int(env, i.moved_overflow));
   napi_set_named_property(env, o, "deviceBytes", make_int(env, i.device_bytes));
+  napi_set_named_property(env, o, "pairOverflow", make_int(env, i.pair_overflow));
+  napi_set_named_property(env, o, "pairSlots", make_int(env, i.pair_slots));
   return o;
 }
 

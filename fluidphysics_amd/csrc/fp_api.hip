@@ -16,10 +16,11 @@ namespace {
 
 thread_local std::string g_last_error = "";
 
-const char* kPassNames[P_NUM] = {"integrate", "grab",        "hash_count",      "scan_reduce", "scan_apply",
-                                 "scatter",   "rank",        "contacts",        "density",     "displace",
-                                 "constraints", "bounds_velocity", "xsph",        "cut",         "pick",
-                                 "count_active", "emit"};
+const char* kPassNames[] = {"integrate",   "grab",        "hash_count",      "scan_reduce", "scan_apply",
+                            "scatter",     "rank",        "contacts",        "resort",      "density",
+                            "displace",    "constraints", "bounds_velocity", "xsph",        "cut",
+                            "pick",        "count_active", "emit"};
+static_assert(sizeof(kPassNames) / sizeof(kPassNames[0]) == P_NUM, "one name per PassId");
 
 // Host <-> device copies ordered on the solver's (non-blocking) stream.
 hipError_t d2h(fp_solver* s, void* dst, const void* src, size_t bytes) {
@@ -65,8 +66,10 @@ void dfree(T*& p) {
 void free_all(fp_solver* s) {
   dfree(s->pos), dfree(s->prev), dfree(s->vel), dfree(s->active), dfree(s->rho), dfree(s->phash);
   dfree(s->lcount), dfree(s->lstart), dfree(s->lblock_sums), dfree(s->ltmp), dfree(s->lkey), dfree(s->lslot);
-  dfree(s->sid), dfree(s->slot), dfree(s->spos), dfree(s->spos2), dfree(s->svtmp), dfree(s->sphash), dfree(s->sP);
-  dfree(s->mcount), dfree(s->mlist), dfree(s->sbox);
+  dfree(s->sid), dfree(s->spos), dfree(s->spos2), dfree(s->svtmp), dfree(s->sphash), dfree(s->sP);
+  dfree(s->tpos), dfree(s->tkey), dfree(s->tslot), dfree(s->fsid), dfree(s->fslot), dfree(s->sphash2);
+  dfree(s->sbox);
+  dfree(s->nbr), dfree(s->ncount);
   dfree(s->pslot), dfree(s->count), dfree(s->cell_start), dfree(s->entries_tmp), dfree(s->cell_entries);
   dfree(s->block_sums);
   dfree(s->cons_ab), dfree(s->rest[0]), dfree(s->rest[1]), dfree(s->alive[0]), dfree(s->alive[1]);
@@ -108,10 +111,6 @@ void setup_cell_grid(fp_solver* s) {
   s->cg.nz_shift = (uint32_t)log2_pow2(NZ);
   s->cg.nyz_shift = (uint32_t)log2_pow2((uint32_t)plane);
   s->lscan_blocks = (int)((m32 + SCAN_TILE - 1) / SCAN_TILE);
-  // moved table: >= 4 z-y planes keeps the 27 neighbour buckets distinct
-  uint32_t tm = s->P.moved_table > 0 ? (uint32_t)s->P.moved_table : next_pow2((double)s->n / 8.0, 1024);
-  tm = std::max<uint32_t>(tm, (uint32_t)std::min<uint64_t>(4 * plane, m32));
-  s->TM = (int)std::min<uint32_t>(tm, m32);
 }
 
 int set_device(fp_solver* s) {
@@ -172,10 +171,9 @@ int check_params(fp_solver* s, const fp_params& p) {
     return fp_fail(s, FP_EINVAL, "cell_size must be >= fluid_h and >= 2 * max(solid_radius, fluid_radius)");
   if (p.table_size < 0 || (p.table_size & (p.table_size - 1)))
     return fp_fail(s, FP_EINVAL, "table_size must be 0 or a power of two");
-  if (p.moved_table < 0 || (p.moved_table & (p.moved_table - 1)))
-    return fp_fail(s, FP_EINVAL, "moved_table must be 0 or a power of two");
   if (p.constraint_iters < 0 || p.constraint_iters > 1000)
     return fp_fail(s, FP_EINVAL, "constraint_iters out of range");
+  if (p.pair_slots < 0 || p.pair_slots > 1024) return fp_fail(s, FP_EINVAL, "pair_slots out of range [0, 1024]");
   return FP_OK;
 }
 
@@ -231,7 +229,6 @@ void fp_params_default(fp_params* p) {
   p->cell_size = 0.22;
   p->constraint_iters = 3;
   p->table_size = 0;
-  p->moved_table = 0;
   p->jacobi_omega = 4.0;
   p->contact_scale = 1.0;
 }
@@ -287,8 +284,8 @@ int fp_set_params(fp_solver* s, const fp_params* p) {
   if (!s || !p) return fp_fail(s, FP_EINVAL, "fp_set_params: NULL argument");
   int rc = check_params(s, *p);
   if (rc) return rc;
-  if (s->has_cube && (p->table_size != s->P.table_size || p->moved_table != s->P.moved_table))
-    return fp_fail(s, FP_ESTATE, "table_size / moved_table are fixed once the scene is allocated");
+  if (s->has_cube && (p->table_size != s->P.table_size || p->pair_slots != s->P.pair_slots))
+    return fp_fail(s, FP_ESTATE, "table_size / pair_slots are fixed once the scene is allocated");
   s->P = *p;
   return FP_OK;
 }
@@ -308,15 +305,15 @@ int fp_get_info(const fp_solver* s, fp_info* out) {
   out->table_size = s->T;
   out->num_constraints = s->C;
   out->fluid_next = s->fluid_next;
-  out->moved_table = s->TM;
   out->device = s->device;
   out->substeps_done = s->substeps_done;
-  out->moved_overflow = 0;
   out->device_bytes = s->device_bytes;
+  out->pair_overflow = 0;
+  out->pair_slots = s->pair_slots;
   if (s->dflags) {
     (void)hipSetDevice(s->device);
     int ov = 0;
-    if (d2h(m, &ov, s->dflags + DF_MOVED_OVF, sizeof(int)) == hipSuccess) out->moved_overflow = ov;
+    if (d2h(m, &ov, s->dflags + DF_PAIR_OVF, sizeof(ov)) == hipSuccess) out->pair_overflow = ov;
   }
   return FP_OK;
 }
@@ -410,6 +407,7 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
   s->h_group.assign(ns, 0);
 
   setup_cell_grid(s);
+  s->pair_slots = s->P.pair_slots > 0 ? s->P.pair_slots : 64;
   const int N = s->n, T = s->T;
   const size_t M = s->cg.M;
   if ((rc = dalloc(s, &s->pos, N)) || (rc = dalloc(s, &s->prev, N)) || (rc = dalloc(s, &s->vel, N)) ||
@@ -417,10 +415,12 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
       (rc = dalloc(s, &s->lcount, M)) || (rc = dalloc(s, &s->lstart, M + 1)) ||
       (rc = dalloc(s, &s->lblock_sums, s->lscan_blocks)) || (rc = dalloc(s, &s->ltmp, N)) ||
       (rc = dalloc(s, &s->lkey, N)) || (rc = dalloc(s, &s->lslot, N)) || (rc = dalloc(s, &s->sid, N)) ||
-      (rc = dalloc(s, &s->slot, N)) || (rc = dalloc(s, &s->spos, N)) || (rc = dalloc(s, &s->spos2, N)) ||
+      (rc = dalloc(s, &s->spos, N)) || (rc = dalloc(s, &s->spos2, N)) ||
       (rc = dalloc(s, &s->svtmp, N)) || (rc = dalloc(s, &s->sphash, N)) || (rc = dalloc(s, &s->sP, N)) ||
-      (rc = dalloc(s, &s->mcount, s->TM)) || (rc = dalloc(s, &s->mlist, (size_t)s->TM * MCAP)) ||
-      (rc = dalloc(s, &s->sbox, 6)) ||
+      (rc = dalloc(s, &s->tpos, N)) || (rc = dalloc(s, &s->tkey, N)) || (rc = dalloc(s, &s->tslot, N)) ||
+      (rc = dalloc(s, &s->fsid, N)) || (rc = dalloc(s, &s->fslot, N)) || (rc = dalloc(s, &s->sphash2, N)) ||
+      (rc = dalloc(s, &s->sbox, 6)) || (rc = dalloc(s, &s->nbr, (size_t)s->pair_slots * N)) ||
+      (rc = dalloc(s, &s->ncount, N)) ||
       (rc = dalloc(s, &s->pslot, N)) || (rc = dalloc(s, &s->count, T)) ||
       (rc = dalloc(s, &s->cell_start, (size_t)T + 1)) || (rc = dalloc(s, &s->entries_tmp, N)) ||
       (rc = dalloc(s, &s->cell_entries, N)) || (rc = dalloc(s, &s->block_sums, s->scan_blocks)) ||

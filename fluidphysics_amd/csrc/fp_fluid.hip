@@ -13,9 +13,9 @@
 //           particle gathers from the grid-time snapshot spos: solids from
 //           non-bonded solids and all fluid, fluid from solids (only fluid
 //           inside the solids' bounding box, grown by the contact range,
-//           can touch one).  Writes spos2 (post-contact xyz; w = particle
-//           index | flags << 30) and enters fluid that changed cell into the
-//           moved table.
+//           can touch one).  Solids write their corrected position to pos;
+//           fluid stages it (tpos) and counts itself under the key of its
+//           post-contact cell for the fluid order (fp_grid.hip).
 // density   (solver.js:284-327) rho = sum w q^2, rhoNear = sum w q^3 over the
 //           reference's pair list; P = k(rho - rho0), Pn = kNear rhoNear.
 // displace  (solver.js:329-353) sum of w (p_x - p_y) D / d with the per-pair
@@ -29,13 +29,11 @@
 // |p_i - p_j|^2 < H^2 in double at the current positions.  x therefore pairs with
 //   y > x  with weight #{probes of cell(x) hashing to bucket(y)},
 //   y < x  with weight #{probes of cell(y) hashing to bucket(x)},
-// both 1 unless a cell moved or a probe set repeats a bucket (PF_MOVED,
-// PF_DUP), when the 27 reference hashes are counted.  Every neighbour within
-// H sits in one of the 27 cells around x's current cell: the cell grid (keyed
-// by grid-time cells) holds the ones that kept their cell, the moved table
-// (keyed by current cell) the ones contacts moved.  Moved neighbours' terms
-// are summed in 64-bit fixed point, so the (atomic) order of the moved table
-// cannot change the result.  The inclusion test d^2 < H^2 is decided in
+// both 1 unless a probe set repeats a bucket (PF_DUP) or the particle whose
+// bucket is probed changed cell in contacts (PF_MOVED): then the 27
+// reference hashes are counted.  The passes gather from the fluid order
+// (sorted by post-contact cell), so every neighbour within H sits in one of
+// the 27 cells around x.  The inclusion test d^2 < H^2 is decided in
 // double, from the float32 positions, whenever the float32 d^2 lies within
 // 1e-5 of H^2 -- the reference's own test (XSPH counts every pair, so the
 // test's outcome matters even where q ~ 0).
@@ -44,9 +42,6 @@
 #include "fp_internal.h"
 #include "fp_kernels.h"
 
-#define FX_RHO 4294967296.0f    // 2^32
-#define FX_DISP 1099511627776.0f  // 2^40
-#define FX_VEL 4294967296.0f
 #define STAGE_CAP 2560            // float4 slots of LDS per block (40 KB)
 
 __device__ __forceinline__ uint32_t fbits(float w) { return __float_as_uint(w); }
@@ -157,15 +152,18 @@ __device__ __forceinline__ void stage_rows(RowStage& st, T* buf, const T* __rest
 
 // ---- contacts -----------------------------------------------------------------
 
-// Is the solid pair (x, y) joined by an alive constraint?  Scans x's CSR row.
+// Is the solid pair (x, y) joined by an alive constraint?  Scans x's CSR
+// row without an early exit, so the row's loads are independent and issue
+// back to back; one dependent load of the alive flag follows.
 __device__ __forceinline__ bool is_bonded(int x, int y, const int* __restrict__ adj_off, const int4* __restrict__ adj,
                                           const uint8_t* __restrict__ alive) {
   const int e = adj_off[x + 1];
+  int c = -1;
   for (int k = adj_off[x]; k < e; k++) {
     const int4 a = adj[k];
-    if (a.x == y && alive[a.y]) return true;
+    c = a.x == y ? a.y : c;
   }
-  return false;
+  return c >= 0 && alive[c] != 0;
 }
 
 struct ContactCtx {
@@ -219,8 +217,9 @@ __device__ __forceinline__ void contact_rows(const FluidParams& fp, const Contac
 }
 
 __global__ void __launch_bounds__(BLOCK) k_contacts(FluidParams fp, ContactCtx cc, const int* __restrict__ lstart,
-                                                    float4* __restrict__ spos2, int* __restrict__ mcount,
-                                                    int* __restrict__ mlist, int* __restrict__ dflags) {
+                                                    float4* __restrict__ pos, float4* __restrict__ tpos,
+                                                    uint32_t* __restrict__ tkey, uint32_t* __restrict__ tslot,
+                                                    int* __restrict__ lcount) {
   __shared__ RowStage st;
   __shared__ float4 buf[STAGE_CAP];
   __shared__ int bid[STAGE_CAP];
@@ -255,7 +254,10 @@ __global__ void __launch_bounds__(BLOCK) k_contacts(FluidParams fp, ContactCtx c
     }
   }
   __syncthreads();
-  if (!valid) return;
+  if (!valid) {
+    if (e < fp.n) tkey[e] = NO_KEY;
+    return;
+  }
   const float rx = solid_x ? fp.rs : fp.rf;
   float ax = 0.f, ay = 0.f, az = 0.f;
   if (want) {
@@ -265,35 +267,27 @@ __global__ void __launch_bounds__(BLOCK) k_contacts(FluidParams fp, ContactCtx c
       contact_rows<false>(fp, cc, lstart, st, buf, bid, e, x, solid_x, px, rx, k0, ax, ay, az);
   }
   const float ox = px.x + ax, oy = px.y + ay, oz = px.z + az;
-  uint32_t fl = 0;
-  if (!solid_x) {
-    const Cell c1 = cell_of(ox, oy, oz, fp.cg.inv_cell);
-    if (c1.x != c0.x || c1.y != c0.y || c1.z != c0.z) {
-      fl |= PF_MOVED;
-      if (probe_any_dup(c1, fp.cg.hmask)) fl |= PF_DUP;
-      const uint32_t mb = cell_key(fp.cg, c1) & fp.mmask;
-      const int s = atomicAdd(&mcount[mb], 1);
-      if (s < MCAP)
-        mlist[mb * MCAP + s] = e;
-      else
-        atomicAdd(&dflags[DF_MOVED_OVF], 1);
-      atomicAdd(&dflags[DF_MOVED], 1);
-    } else if (probe_any_dup(c0, fp.cg.hmask)) {
-      fl |= PF_DUP;
-    }
+  if (solid_x) {
+    pos[x] = make_float4(ox, oy, oz, px.w);
+    tkey[e] = NO_KEY;
+    return;
   }
-  spos2[e] = make_float4(ox, oy, oz, __uint_as_float((uint32_t)x | (fl << PF_SHIFT)));
+  const Cell c1 = cell_of(ox, oy, oz, fp.cg.inv_cell);
+  uint32_t fl = 0;
+  if (c1.x != c0.x || c1.y != c0.y || c1.z != c0.z) fl |= PF_MOVED;
+  if (probe_any_dup(c1, fp.cg.hmask)) fl |= PF_DUP;
+  const uint32_t k1 = cell_key(fp.cg, c1);
+  tkey[e] = k1;
+  tslot[e] = (uint32_t)atomicAdd(&lcount[k1], 1);
+  tpos[e] = make_float4(ox, oy, oz, __uint_as_float((uint32_t)x | (fl << PF_SHIFT)));
 }
 
 // ---- the reference's fluid pairs of the particle in slot e ------------------
 
 struct PairSource {
-  const float4* __restrict__ spos2;
-  const uint32_t* __restrict__ sphash;
-  const int* __restrict__ lstart;
-  const int* __restrict__ mcount;
-  const int* __restrict__ mlist;
-  const int* __restrict__ dflags;
+  const float4* __restrict__ spos2;     // fluid order: post-contact xyz, w = index | flags
+  const uint32_t* __restrict__ sphash;  // fluid order: bucket at grid time
+  const int* __restrict__ lstart;       // fluid-order cell starts
 };
 
 struct PairSelf {
@@ -327,11 +321,11 @@ __device__ __forceinline__ int pair_weight(const FluidParams& fp, const PairSour
   dy = me.p.y - py.y;
   dz = me.p.z - py.z;
   d2 = dx * dx + dy * dy + dz * dz;
-  // solids and moved fluid (it comes from the moved table) are not grid pairs
-  bool ok = (q != me.e) & (y >= fp.ns) & ((fy & PF_MOVED) == 0u) & (d2 < fp.H2_hi);
+  bool ok = (q != me.e) & (d2 < fp.H2_hi);
   if (__builtin_expect(ok && d2 >= fp.H2_lo, 0)) ok = within_h(fp, me.p, py, d2);
   if (!ok) return 0;
-  const bool special = (y > me.x) ? me.dup : (me.moved | ((fy & PF_DUP) != 0u));
+  // y > x: x's probes against y's grid bucket; y < x: y's probes against x's
+  const bool special = (y > me.x) ? (me.dup | ((fy & PF_MOVED) != 0u)) : (me.moved | ((fy & PF_DUP) != 0u));
   if (__builtin_expect(special, 0)) {
     return y > me.x ? multiplicity(me.c, fp.cg.hmask, src.sphash[q])
                     : multiplicity_of(py.x, py.y, py.z, fp.cg.inv_cell, fp.cg.hmask, me.gh);
@@ -339,10 +333,10 @@ __device__ __forceinline__ int pair_weight(const FluidParams& fp, const PairSour
   return 1;
 }
 
-// The pairs of x: f.grid(q, w, dx, dy, dz, d2) for neighbours from the cell
-// grid (deterministic order), f.moved(...) for moved neighbours.  LDS: the
-// block's rows are staged in buf (a separate instantiation, so each path
-// keeps its own address space: ds_read for LDS, global loads otherwise).
+// The pairs of x, in the deterministic order of the fluid order's rows:
+// f.pair(q, w, dx, dy, dz, d2).  LDS: the block's rows are staged in buf (a
+// separate instantiation, so each path keeps its own address space:
+// ds_read for LDS, global loads otherwise).
 template <bool LDS, class F>
 __device__ __forceinline__ void visit_rows(const FluidParams& fp, const PairSource& src, const PairSelf& me,
                                            const RowStage& st, const float4* buf, F& f) {
@@ -355,107 +349,131 @@ __device__ __forceinline__ void visit_rows(const FluidParams& fp, const PairSour
     for (int q = r0; q < r1; q++) {
       float dx, dy, dz, d2;
       const int w = pair_weight(fp, src, me, q, LDS ? buf[base + q] : src.spos2[q], dx, dy, dz, d2);
-      if (w) f.grid(q, w, dx, dy, dz, d2);
+      if (w) f.pair(q, w, dx, dy, dz, d2);
     }
     for (int q = 0; q < w1; q++) {
       float dx, dy, dz, d2;
       const int w = pair_weight(fp, src, me, q, src.spos2[q], dx, dy, dz, d2);
-      if (w) f.grid(q, w, dx, dy, dz, d2);
+      if (w) f.pair(q, w, dx, dy, dz, d2);
     }
   }
 }
 
+// ---- the pair list ----------------------------------------------------------------
+//
+// The density pass records the pairs it finds for the displacement and XSPH
+// passes (the reference freezes its pair list at density time too,
+// solver.js:311, :431-445): entry k of fluid slot e lives at
+// nbr[k * stride + e], so a wave reading its lanes' k-th pairs reads one
+// contiguous run.  An entry packs the neighbour's fluid slot (27 bits) and
+// the weight - 1 (5 bits).  A particle with more than pair_slots pairs gets
+// the count pair_slots + 1 and is gathered again from the grid by the later
+// passes (same pairs, same order).
+
+#define NB_QBITS 27
+#define NB_QMASK ((1u << NB_QBITS) - 1u)
+
+struct PairList {
+  uint32_t* __restrict__ nbr;
+  int* __restrict__ ncount;
+  int stride, cap;
+};
+
+// Replays the recorded pairs of me into f, in the order they were found.
 template <class F>
-__device__ __forceinline__ void visit_pairs(const FluidParams& fp, const PairSource& src, const PairSelf& me,
-                                            const RowStage& st, const float4* buf, F& f) {
-  if (st.ok)
-    visit_rows<true>(fp, src, me, st, buf, f);
-  else
-    visit_rows<false>(fp, src, me, st, buf, f);
-  if (src.dflags[DF_MOVED] == 0) return;
-  const uint32_t kx = cell_key(fp.cg, me.c);
-  // the moved table: 27 distinct buckets (it spans at least 4 z-y planes)
-  for (int a = -1; a <= 1; a++)
-    for (int b = -1; b <= 1; b++)
-      for (int c = -1; c <= 1; c++) {
-        const uint32_t mb = (row_key(fp.cg, kx, a, b) + (uint32_t)(c + 1)) & fp.mmask;
-        const int cnt = src.mcount[mb];
-        const int m = cnt < MCAP ? cnt : MCAP;
-        for (int j = 0; j < m; j++) {
-          const int q = src.mlist[mb * MCAP + j];
-          if (q == me.e) continue;
-          const float4 py = src.spos2[q];
-          const int y = (int)(fbits(py.w) & SID_MASK);
-          const float dx = me.p.x - py.x, dy = me.p.y - py.y, dz = me.p.z - py.z;
-          const float d2 = dx * dx + dy * dy + dz * dz;
-          if (!within_h(fp, me.p, py, d2)) continue;
-          const int w = y > me.x ? multiplicity(me.c, fp.cg.hmask, src.sphash[q])
-                                 : multiplicity_of(py.x, py.y, py.z, fp.cg.inv_cell, fp.cg.hmask, me.gh);
-          if (w == 0) continue;
-          f.moved(q, w, dx, dy, dz, d2);
-        }
-      }
+__device__ __forceinline__ void replay_pairs(const PairSource& src, const PairList& pl, const PairSelf& me, int cnt,
+                                             F& f) {
+  const uint32_t* __restrict__ row = pl.nbr + me.e;
+  for (int k = 0; k < cnt; k++) {
+    const uint32_t v = row[(size_t)k * pl.stride];
+    const int q = (int)(v & NB_QMASK);
+    const int w = (int)(v >> NB_QBITS) + 1;
+    const float4 py = src.spos2[q];
+    const float dx = me.p.x - py.x, dy = me.p.y - py.y, dz = me.p.z - py.z;
+    f.pair(q, w, dx, dy, dz, dx * dx + dy * dy + dz * dz);
+  }
 }
 
-// Common prologue of the three fluid passes: who am I, the block staging.
-// Returns false for threads without a fluid particle (they still took part
-// in the block's synchronisation).
-__device__ __forceinline__ bool fluid_prologue(const FluidParams& fp, const PairSource& src, RowStage& st,
-                                               float4* buf, PairSelf& me) {
-  me.e = blockIdx.x * BLOCK + threadIdx.x;
-  bool valid = me.e < src.lstart[fp.cg.M];
-  me.x = 0;
-  uint32_t kx = 0;
-  if (valid) {
-    me.p = src.spos2[me.e];
-    const uint32_t b = fbits(me.p.w);
-    me.x = (int)(b & SID_MASK);
-    me.moved = (b >> PF_SHIFT) & PF_MOVED;
-    me.dup = (b >> PF_SHIFT) & PF_DUP;
-    valid = me.x >= fp.ns;
+// Records every pair it is handed, then forwards it.
+template <class F>
+struct Recorder {
+  F& f;
+  const PairList& pl;
+  int e;
+  int cnt = 0;
+  __device__ Recorder(F& f_, const PairList& pl_, int e_) : f(f_), pl(pl_), e(e_) {}
+  __device__ __forceinline__ void pair(int q, int w, float dx, float dy, float dz, float d2) {
+    if (cnt < pl.cap) pl.nbr[(size_t)cnt * pl.stride + e] = (uint32_t)q | ((uint32_t)(w - 1) << NB_QBITS);
+    cnt++;
+    f.pair(q, w, dx, dy, dz, d2);
   }
-  if (valid) {
-    me.c = cell_of(me.p.x, me.p.y, me.p.z, fp.cg.inv_cell);
-    me.gh = src.sphash[me.e];
-    kx = cell_key(fp.cg, me.c);
+};
+
+// The pairs of me for the passes after density: the recorded list, or the
+// grid again (global reads) for a particle whose list overflowed.
+template <class F>
+__device__ __forceinline__ void recorded_pairs(const FluidParams& fp, const PairSource& src, const PairList& pl,
+                                               const PairSelf& me, F& f) {
+  const int cnt = pl.ncount[me.e];
+  if (__builtin_expect(cnt <= pl.cap, 1)) {
+    replay_pairs(src, pl, me, cnt, f);
+  } else {
+    RowStage none;
+    visit_rows<false>(fp, src, me, none, nullptr, f);
   }
-  stage_rows(st, buf, src.spos2, fp.cg, src.lstart, valid, kx);
-  return valid;
+}
+
+// Who is in fluid slot e (false past the last one).
+__device__ __forceinline__ bool pair_self(const FluidParams& fp, const PairSource& src, int e, PairSelf& me) {
+  me.e = e;
+  if (e >= src.lstart[fp.cg.M]) return false;
+  me.p = src.spos2[e];
+  const uint32_t b = fbits(me.p.w);
+  me.x = (int)(b & SID_MASK);
+  me.moved = (b >> PF_SHIFT) & PF_MOVED;
+  me.dup = (b >> PF_SHIFT) & PF_DUP;
+  me.c = cell_of(me.p.x, me.p.y, me.p.z, fp.cg.inv_cell);
+  me.gh = src.sphash[e];
+  return true;
 }
 
 struct DensityAcc {
   float inv_H;
   float rho = 0.f, rhon = 0.f;
-  long long frho = 0, frhon = 0;
-  __device__ void grid(int, int w, float, float, float, float d2) {
+  __device__ __forceinline__ void pair(int, int w, float, float, float, float d2) {
     const float q = 1.f - sqrtf(d2) * inv_H;
     const float q2 = q * q;
     rho += (float)w * q2;
     rhon += (float)w * q2 * q;
   }
-  __device__ void moved(int, int w, float, float, float, float d2) {
-    const float q = 1.f - sqrtf(d2) * inv_H;
-    const float q2 = q * q;
-    frho += __float2ll_rn((float)w * q2 * FX_RHO);
-    frhon += __float2ll_rn((float)w * q2 * q * FX_RHO);
-  }
 };
 
-__global__ void __launch_bounds__(BLOCK) k_density(FluidParams fp, PairSource src, float2* __restrict__ sP,
-                                                   float2* __restrict__ rho_out, int* __restrict__ sbox) {
+__global__ void __launch_bounds__(BLOCK) k_density(FluidParams fp, PairSource src, PairList pl,
+                                                   float2* __restrict__ sP, float2* __restrict__ rho_out,
+                                                   int* __restrict__ sbox, int* __restrict__ dflags) {
   __shared__ RowStage st;
   __shared__ float4 buf[STAGE_CAP];
   // the solids' bounding box is rebuilt by the next integrate
   if (blockIdx.x == 0 && threadIdx.x < 6) sbox[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : INT_MIN;
   PairSelf me;
-  if (!fluid_prologue(fp, src, st, buf, me)) return;
+  const bool valid = pair_self(fp, src, blockIdx.x * BLOCK + threadIdx.x, me);
+  stage_rows(st, buf, src.spos2, fp.cg, src.lstart, valid, valid ? cell_key(fp.cg, me.c) : 0u);
+  if (!valid) return;
   DensityAcc acc;
   acc.inv_H = fp.inv_H;
-  visit_pairs(fp, src, me, st, buf, acc);
-  const float rho = acc.rho + (float)((double)acc.frho * (1.0 / 4294967296.0));
-  const float rhon = acc.rhon + (float)((double)acc.frhon * (1.0 / 4294967296.0));
-  rho_out[me.x] = make_float2(rho, rhon);
-  sP[me.e] = make_float2(fp.stiff * (rho - fp.rest_density), fp.near_stiff * rhon);
+  Recorder<DensityAcc> rec(acc, pl, me.e);
+  if (st.ok)
+    visit_rows<true>(fp, src, me, st, buf, rec);
+  else
+    visit_rows<false>(fp, src, me, st, buf, rec);
+  if (rec.cnt > pl.cap) {
+    pl.ncount[me.e] = pl.cap + 1;
+    atomicAdd(&dflags[DF_PAIR_OVF], 1);
+  } else {
+    pl.ncount[me.e] = rec.cnt;
+  }
+  rho_out[me.x] = make_float2(acc.rho, acc.rhon);
+  sP[me.e] = make_float2(fp.stiff * (acc.rho - fp.rest_density), fp.near_stiff * acc.rhon);
 }
 
 struct DisplaceAcc {
@@ -463,53 +481,33 @@ struct DisplaceAcc {
   float2 Px;
   float inv_H, max_pair;
   float sx = 0.f, sy = 0.f, sz = 0.f;
-  long long fx = 0, fy = 0, fz = 0;
-  __device__ float coef(int q, int w, float d2) const {
+  __device__ __forceinline__ void pair(int q, int w, float dx, float dy, float dz, float d2) {
     const float d = sqrtf(d2);
-    if (d < 1e-9f) return 0.f;
+    if (d < 1e-9f) return;
     const float u = 1.f - d * inv_H;
     const float2 Py = sP[q];
     float D = 0.25f * ((Px.x + Py.x) * u + (Px.y + Py.y) * u * u);
     D = fminf(fmaxf(D, -max_pair), max_pair);
-    return (float)w * D / d;
-  }
-  __device__ void grid(int q, int w, float dx, float dy, float dz, float d2) {
-    const float s = coef(q, w, d2);
+    const float s = (float)w * D / d;
     sx += dx * s;
     sy += dy * s;
     sz += dz * s;
   }
-  __device__ void moved(int q, int w, float dx, float dy, float dz, float d2) {
-    const float s = coef(q, w, d2);
-    fx += __float2ll_rn(dx * s * FX_DISP);
-    fy += __float2ll_rn(dy * s * FX_DISP);
-    fz += __float2ll_rn(dz * s * FX_DISP);
-  }
 };
 
-// spos2 -> pos for every active particle: fluid gets the clamped
-// displacement; the inverse mass comes back from spos.w.
-__global__ void __launch_bounds__(BLOCK) k_displace(FluidParams fp, PairSource src, const float4* __restrict__ spos,
+// spos2 + the clamped displacement -> pos for the fluid; the inverse mass
+// is kept from pos.w.
+__global__ void __launch_bounds__(BLOCK) k_displace(FluidParams fp, PairSource src, PairList pl,
                                                     const float2* __restrict__ sP, float4* __restrict__ pos) {
-  __shared__ RowStage st;
-  __shared__ float4 buf[STAGE_CAP];
-  const int e = blockIdx.x * BLOCK + threadIdx.x;
-  if (e < src.lstart[fp.cg.M]) {
-    const float4 q = src.spos2[e];
-    const int x = (int)(fbits(q.w) & SID_MASK);
-    if (x < fp.ns) pos[x] = make_float4(q.x, q.y, q.z, spos[e].w);
-  }
   PairSelf me;
-  if (!fluid_prologue(fp, src, st, buf, me)) return;
+  if (!pair_self(fp, src, blockIdx.x * BLOCK + threadIdx.x, me)) return;
   DisplaceAcc acc;
   acc.sP = sP;
   acc.Px = sP[me.e];
   acc.inv_H = fp.inv_H;
   acc.max_pair = fp.max_pair;
-  visit_pairs(fp, src, me, st, buf, acc);
-  float sx = acc.sx + (float)((double)acc.fx * (1.0 / 1099511627776.0));
-  float sy = acc.sy + (float)((double)acc.fy * (1.0 / 1099511627776.0));
-  float sz = acc.sz + (float)((double)acc.fz * (1.0 / 1099511627776.0));
+  recorded_pairs(fp, src, pl, me, acc);
+  float sx = acc.sx, sy = acc.sy, sz = acc.sz;
   const float s2 = sx * sx + sy * sy + sz * sz;
   if (s2 > fp.max_total * fp.max_total) {
     const float g = fp.max_total / sqrtf(s2);
@@ -517,15 +515,15 @@ __global__ void __launch_bounds__(BLOCK) k_displace(FluidParams fp, PairSource s
     sy *= g;
     sz *= g;
   }
-  pos[me.x] = make_float4(me.p.x + sx, me.p.y + sy, me.p.z + sz, spos[me.e].w);
+  const float w = pos[me.x].w;
+  pos[me.x] = make_float4(me.p.x + sx, me.p.y + sy, me.p.z + sz, w);
 }
 
 struct XsphAcc {
   const float4* __restrict__ svtmp;
   float sx = 0.f, sy = 0.f, sz = 0.f;
   int m = 0;
-  long long fx = 0, fy = 0, fz = 0;
-  __device__ void grid(int q, int w, float, float, float, float) {
+  __device__ __forceinline__ void pair(int q, int w, float, float, float, float) {
     const float4 v = svtmp[q];
     const float fw = (float)w;
     sx += fw * v.x;
@@ -533,39 +531,27 @@ struct XsphAcc {
     sz += fw * v.z;
     m += w;
   }
-  __device__ void moved(int q, int w, float, float, float, float) {
-    const float4 v = svtmp[q];
-    const float fw = (float)w;
-    fx += __float2ll_rn(fw * v.x * FX_VEL);
-    fy += __float2ll_rn(fw * v.y * FX_VEL);
-    fz += __float2ll_rn(fw * v.z * FX_VEL);
-    m += w;
-  }
 };
 
 // XSPH: svtmp -> vel for the fluid that is still active (the reference skips
 // particles culled since the density pass, whose velocity stays as it was).
-__global__ void __launch_bounds__(BLOCK) k_xsph(FluidParams fp, PairSource src, const float4* __restrict__ svtmp,
-                                                float4* __restrict__ vel, const uint8_t* __restrict__ active) {
-  __shared__ RowStage st;
-  __shared__ float4 buf[STAGE_CAP];
+__global__ void __launch_bounds__(BLOCK) k_xsph(FluidParams fp, PairSource src, PairList pl,
+                                                const float4* __restrict__ svtmp, float4* __restrict__ vel,
+                                                const uint8_t* __restrict__ active) {
   PairSelf me;
-  if (!fluid_prologue(fp, src, st, buf, me)) return;
+  if (!pair_self(fp, src, blockIdx.x * BLOCK + threadIdx.x, me)) return;
   if (!active[me.x]) return;
   const float4 v = svtmp[me.e];
   XsphAcc acc;
   acc.svtmp = svtmp;
-  visit_pairs(fp, src, me, st, buf, acc);
+  recorded_pairs(fp, src, pl, me, acc);
   if (acc.m == 0) {
     vel[me.x] = v;
     return;
   }
-  const float sx = acc.sx + (float)((double)acc.fx * (1.0 / 4294967296.0));
-  const float sy = acc.sy + (float)((double)acc.fy * (1.0 / 4294967296.0));
-  const float sz = acc.sz + (float)((double)acc.fz * (1.0 / 4294967296.0));
   const float inv_m = 1.f / (float)acc.m;
-  vel[me.x] = make_float4(v.x + fp.visc * (sx * inv_m - v.x), v.y + fp.visc * (sy * inv_m - v.y),
-                          v.z + fp.visc * (sz * inv_m - v.z), 0.f);
+  vel[me.x] = make_float4(v.x + fp.visc * (acc.sx * inv_m - v.x), v.y + fp.visc * (acc.sy * inv_m - v.y),
+                          v.z + fp.visc * (acc.sz * inv_m - v.z), 0.f);
 }
 
 // ---- launchers ------------------------------------------------------------------
@@ -574,7 +560,6 @@ FluidParams fp_fluid_params(const fp_solver* s) {
   FluidParams fp;
   const fp_params& P = s->P;
   fp.cg = s->cg;
-  fp.mmask = (uint32_t)(s->TM - 1);
   fp.ns = s->ns;
   fp.n = s->n;
   fp.nf = s->nf;
@@ -597,20 +582,26 @@ FluidParams fp_fluid_params(const fp_solver* s) {
 static PairSource pair_source(const fp_solver* s) {
   PairSource src;
   src.spos2 = s->spos2;
-  src.sphash = s->sphash;
+  src.sphash = s->sphash2;
   src.lstart = s->lstart;
-  src.mcount = s->mcount;
-  src.mlist = s->mlist;
-  src.dflags = s->dflags;
   return src;
+}
+
+static PairList pair_list(const fp_solver* s) {
+  PairList pl;
+  pl.nbr = s->nbr;
+  pl.ncount = s->ncount;
+  pl.stride = s->n;
+  pl.cap = s->pair_slots;
+  return pl;
 }
 
 int fp_launch_contacts(fp_solver* s) {
   const FluidParams fp = fp_fluid_params(s);
   ContactCtx cc{s->spos, s->sid, s->adj_off, s->adj, s->alive[s->cur], s->sbox};
   PassScope ps(s, P_CONTACTS);
-  hipLaunchKernelGGL(k_contacts, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, cc, s->lstart, s->spos2,
-                     s->mcount, s->mlist, s->dflags);
+  hipLaunchKernelGGL(k_contacts, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, cc, s->lstart, s->pos,
+                     s->tpos, s->tkey, s->tslot, s->lcount);
   FP_TRY(s, hipGetLastError());
   return FP_OK;
 }
@@ -618,8 +609,8 @@ int fp_launch_contacts(fp_solver* s) {
 int fp_launch_density(fp_solver* s) {
   const FluidParams fp = fp_fluid_params(s);
   PassScope ps(s, P_DENSITY);
-  hipLaunchKernelGGL(k_density, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, pair_source(s), s->sP, s->rho,
-                     s->sbox);
+  hipLaunchKernelGGL(k_density, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, pair_source(s), pair_list(s),
+                     s->sP, s->rho, s->sbox, s->dflags);
   FP_TRY(s, hipGetLastError());
   return FP_OK;
 }
@@ -627,8 +618,8 @@ int fp_launch_density(fp_solver* s) {
 int fp_launch_displace(fp_solver* s) {
   const FluidParams fp = fp_fluid_params(s);
   PassScope ps(s, P_DISPLACE);
-  hipLaunchKernelGGL(k_displace, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, pair_source(s), s->spos, s->sP,
-                     s->pos);
+  hipLaunchKernelGGL(k_displace, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, pair_source(s),
+                     pair_list(s), s->sP, s->pos);
   FP_TRY(s, hipGetLastError());
   return FP_OK;
 }
@@ -637,8 +628,8 @@ int fp_launch_xsph(fp_solver* s) {
   if (s->nf == 0) return FP_OK;
   const FluidParams fp = fp_fluid_params(s);
   PassScope ps(s, P_XSPH);
-  hipLaunchKernelGGL(k_xsph, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, pair_source(s), s->svtmp, s->vel,
-                     s->active);
+  hipLaunchKernelGGL(k_xsph, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, pair_source(s), pair_list(s),
+                     s->svtmp, s->vel, s->active);
   FP_TRY(s, hipGetLastError());
   return FP_OK;
 }

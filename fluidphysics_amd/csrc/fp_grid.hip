@@ -8,10 +8,11 @@
 //    are removed by the distance tests of the passes.  The build is a
 //    counting sort: count (fused into integrate) -> exclusive scan ->
 //    scatter in arrival order -> stable rank (ascending particle index
-//    inside a cell), which also writes the sorted copies the gather passes
-//    read contiguously (sid = particle of each sorted slot, slot = sorted
-//    slot of each particle, spos = position + inverse mass, sphash = the
-//    particle's bucket in the reference's hash table).
+//    inside a cell), which also writes the sorted copies the contact pass
+//    reads contiguously (sid = particle of each sorted slot, spos = position
+//    + inverse mass, sphash = the particle's bucket in the reference's hash
+//    table).  After contacts the fluid is sorted again by its post-contact
+//    cell (fp_launch_resort) for the fluid passes.
 //
 // 2. The reference's hash grid (_buildGrid, src/solver.js:174-184), built on
 //    demand for fp_build_grid / fp_read_grid from the per-particle buckets
@@ -57,11 +58,9 @@ __global__ void __launch_bounds__(BLOCK) k_scan_reduce(int t, const int* __restr
 
 // Exclusive scan of one tile plus the carry of the tiles before it; writes
 // start[k] and zeroes count[k]; the tile holding the last entry writes
-// start[t] = total.  With tm > 0 it also clears the moved table and the
-// per-substep counters (cell-grid build only).
+// start[t] = total.
 __global__ void __launch_bounds__(BLOCK) k_scan_apply(int t, int* __restrict__ count,
-                                                      const int* __restrict__ block_sums, int* __restrict__ start,
-                                                      int tm, int* __restrict__ mcount, int* __restrict__ dflags) {
+                                                      const int* __restrict__ block_sums, int* __restrict__ start) {
   __shared__ int wsum[BLOCK / 64];
   __shared__ int carry_s;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -79,11 +78,6 @@ __global__ void __launch_bounds__(BLOCK) k_scan_apply(int t, int* __restrict__ c
   __syncthreads();
   const int carry = carry_s;
   __syncthreads();
-
-  if (tm > 0) {
-    for (int k = blockIdx.x * BLOCK + threadIdx.x; k < tm; k += gridDim.x * BLOCK) mcount[k] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) dflags[DF_MOVED] = 0;
-  }
 
   const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
@@ -179,8 +173,7 @@ __global__ void __launch_bounds__(BLOCK) k_cell_rank(int n, const uint32_t* __re
                                                      const uint32_t* __restrict__ lkey,
                                                      const int* __restrict__ lstart, const int* __restrict__ tmp,
                                                      const float4* __restrict__ pos, int* __restrict__ sid,
-                                                     int* __restrict__ slot, float4* __restrict__ spos,
-                                                     uint32_t* __restrict__ sphash) {
+                                                     float4* __restrict__ spos, uint32_t* __restrict__ sphash) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
   const uint32_t h = phash[i];
@@ -191,7 +184,6 @@ __global__ void __launch_bounds__(BLOCK) k_cell_rank(int n, const uint32_t* __re
   for (int q = s; q < e; q++) r += tmp[q] < i ? 1 : 0;
   const int d = s + r;
   sid[d] = i;
-  slot[i] = d;
   spos[d] = pos[i];
   sphash[d] = h;
 }
@@ -212,7 +204,7 @@ int fp_launch_cell_grid(fp_solver* s, bool count_done) {
   {
     PassScope ps(s, P_SCAN_APPLY);
     hipLaunchKernelGGL(k_scan_apply, dim3(s->lscan_blocks), dim3(BLOCK), 0, s->stream, M, s->lcount,
-                       s->lblock_sums, s->lstart, s->TM, s->mcount, s->dflags);
+                       s->lblock_sums, s->lstart);
   }
   {
     PassScope ps(s, P_SCATTER);
@@ -222,10 +214,66 @@ int fp_launch_cell_grid(fp_solver* s, bool count_done) {
   {
     PassScope ps(s, P_RANK);
     hipLaunchKernelGGL(k_cell_rank, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->phash, s->lkey, s->lstart,
-                       s->ltmp, s->pos, s->sid, s->slot, s->spos, s->sphash);
+                       s->ltmp, s->pos, s->sid, s->spos, s->sphash);
   }
   FP_TRY(s, hipGetLastError());
   s->ref_grid_valid = false;
+  return FP_OK;
+}
+
+// ---- fluid order (after contacts) --------------------------------------------
+//
+// The contact pass counted each active fluid particle into lcount under the
+// key of its post-contact cell (tkey, tslot per grid slot; the first scan
+// left lcount zero).  Scan, scatter, then the same stable rank (ascending
+// particle index inside a cell) give the fluid order the density,
+// displacement and XSPH passes gather from.
+
+__global__ void __launch_bounds__(BLOCK) k_fluid_scatter(int n, const int* __restrict__ sid,
+                                                         const uint32_t* __restrict__ tkey,
+                                                         const uint32_t* __restrict__ tslot,
+                                                         const int* __restrict__ lstart, int* __restrict__ tmp) {
+  const int e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= n) return;
+  const uint32_t k = tkey[e];
+  if (k == NO_KEY) return;
+  tmp[lstart[k] + (int)tslot[e]] = sid[e];
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fluid_rank(int n, const int* __restrict__ sid,
+                                                      const uint32_t* __restrict__ tkey,
+                                                      const int* __restrict__ lstart, const int* __restrict__ tmp,
+                                                      const float4* __restrict__ tpos,
+                                                      const uint32_t* __restrict__ sphash,
+                                                      int* __restrict__ fsid, int* __restrict__ fslot,
+                                                      float4* __restrict__ spos2, uint32_t* __restrict__ sphash2) {
+  const int e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= n) return;
+  const uint32_t k = tkey[e];
+  if (k == NO_KEY) return;
+  const int i = sid[e];
+  const int s = lstart[k], t = lstart[k + 1];
+  int r = 0;
+  for (int q = s; q < t; q++) r += tmp[q] < i ? 1 : 0;
+  const int d = s + r;
+  fsid[d] = i;
+  fslot[i] = d;
+  spos2[d] = tpos[e];
+  sphash2[d] = sphash[e];
+}
+
+int fp_launch_resort(fp_solver* s) {
+  const int n = s->n;
+  const int M = (int)s->cg.M;
+  PassScope ps(s, P_RESORT);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(s->lscan_blocks), dim3(BLOCK), 0, s->stream, M, s->lcount, s->lblock_sums);
+  hipLaunchKernelGGL(k_scan_apply, dim3(s->lscan_blocks), dim3(BLOCK), 0, s->stream, M, s->lcount, s->lblock_sums,
+                     s->lstart);
+  hipLaunchKernelGGL(k_fluid_scatter, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->sid, s->tkey, s->tslot,
+                     s->lstart, s->ltmp);
+  hipLaunchKernelGGL(k_fluid_rank, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->sid, s->tkey, s->lstart,
+                     s->ltmp, s->tpos, s->sphash, s->fsid, s->fslot, s->spos2, s->sphash2);
+  FP_TRY(s, hipGetLastError());
   return FP_OK;
 }
 
@@ -287,7 +335,7 @@ int fp_launch_ref_grid(fp_solver* s, bool rehash) {
   hipLaunchKernelGGL(k_ref_count, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->phash, s->count, s->pslot);
   hipLaunchKernelGGL(k_scan_reduce, dim3(s->scan_blocks), dim3(BLOCK), 0, s->stream, s->T, s->count, s->block_sums);
   hipLaunchKernelGGL(k_scan_apply, dim3(s->scan_blocks), dim3(BLOCK), 0, s->stream, s->T, s->count, s->block_sums,
-                     s->cell_start, 0, nullptr, nullptr);
+                     s->cell_start);
   hipLaunchKernelGGL(k_ref_scatter, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->phash, s->pslot,
                      s->cell_start, s->entries_tmp);
   hipLaunchKernelGGL(k_ref_rank, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->phash, s->cell_start,

@@ -4,17 +4,22 @@
 //
 // Layout in HBM (structure of arrays, 16-byte vectors; N = capacity,
 // Ns = solids, C = constraints, T = reference hash table size, M = cell
-// table size, A = active particles):
+// table size):
 //   pos, prev, vel  float4[N] particle-index order (pos.w = inverse mass)
 //   active u8[N]; rho float2[N]; phash u32[N] reference bucket at grid time
 //   lcount i32[M] (zero between builds), lstart i32[M+1], ltmp/lkey/lslot [N]
-//   sorted-slot order (slots 0..A-1, cell by cell):
-//     sid i32 (particle of the slot), spos float4 (xyz + inverse mass at grid
-//     time), spos2 float4 (post-contact xyz; w = bits of particle index |
-//     PF_* flags << 30), sphash u32, sP float2 (P, Pnear), svtmp float4
-//     (velocity before XSPH); slot i32[N] is the inverse of sid
-//   mcount i32[TM], mlist i32[TM*MCAP]: sorted slots of the fluid moved to
-//     another cell by contacts, keyed by current cell
+//   grid order (every active particle, sorted by its cell after integrate;
+//   read by contacts): sid i32 (particle of the slot), spos float4 (xyz +
+//     inverse mass), sphash u32 (bucket)
+//   contacts -> fluid order staging, per grid slot: tpos float4 (post-contact
+//     xyz, w = bits of particle index | PF_* flags << 30), tkey u32 (cell key
+//     of the post-contact position, ~0 for slots without fluid), tslot u32
+//   fluid order (active fluid sorted by its post-contact cell; read by
+//   density, displacement, XSPH): fsid, fslot i32[N] (inverse), spos2
+//     float4 (= tpos), sphash2 u32, sP float2 (P, Pnear), svtmp float4
+//     (velocity before XSPH)
+//   nbr u32[K*N] (entry k of fluid slot e at k*N + e), ncount i32[N]: the
+//     pairs the density pass found, replayed by displacement and XSPH
 //   count/cell_start/entries_tmp/cell_entries: the reference's hash grid,
 //     rebuilt from phash only when read
 //   cons_ab int2[C]; rest[2] f32[C]; alive[2] u8[C] (ping-pong per iteration);
@@ -38,14 +43,13 @@
 #define PF_DUP 2u    // its current cell's 27 reference probes repeat a bucket
 #define SID_MASK 0x3fffffffu
 #define PF_SHIFT 30
-#define MCAP 8       // moved-table bucket capacity
+#define NO_KEY 0xffffffffu
 
 // device flag words (dflags)
 #define DF_TEAR 0        // a constraint tore this step
 #define DF_CUT 1         // knife count
 #define DF_SCRATCH 2     // fluid count
-#define DF_MOVED 3       // moved fluid particles this substep
-#define DF_MOVED_OVF 4   // moved-table insertions that did not fit
+#define DF_PAIR_OVF 3    // particle-substeps whose pair list overflowed
 #define DF_WORDS 8
 
 enum PassId {
@@ -57,6 +61,7 @@ enum PassId {
   P_SCATTER,
   P_RANK,
   P_CONTACTS,
+  P_RESORT,
   P_DENSITY,
   P_DISPLACE,
   P_CONSTRAINTS,
@@ -74,7 +79,7 @@ struct fp_solver {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  int max_fluid = 0, ns = 0, n = 0, nf = 0, T = 0, C = 0, TM = 0;
+  int max_fluid = 0, ns = 0, n = 0, nf = 0, T = 0, C = 0;
   int fluid_next = 0;
   bool has_cube = false;
   int64_t substeps_done = 0;
@@ -89,11 +94,18 @@ struct fp_solver {
   int lscan_blocks = 0;
   int *lcount = nullptr, *lstart = nullptr, *lblock_sums = nullptr, *ltmp = nullptr;
   uint32_t *lkey = nullptr, *lslot = nullptr;
-  int *sid = nullptr, *slot = nullptr;
-  float4 *spos = nullptr, *spos2 = nullptr, *svtmp = nullptr;
+  int* sid = nullptr;
+  float4* spos = nullptr;
   uint32_t* sphash = nullptr;
+  float4* tpos = nullptr;
+  uint32_t *tkey = nullptr, *tslot = nullptr;
+  int *fsid = nullptr, *fslot = nullptr;
+  float4 *spos2 = nullptr, *svtmp = nullptr;
+  uint32_t* sphash2 = nullptr;
   float2* sP = nullptr;
-  int *mcount = nullptr, *mlist = nullptr;
+  uint32_t* nbr = nullptr;  // pair list, pair_slots x capacity (slot-major)
+  int* ncount = nullptr;
+  int pair_slots = 0;
   int* sbox = nullptr;  // solids' bounding box at grid time (6 ordered ints)
   // the reference's hash grid, built on demand
   bool ref_grid_valid = false;
@@ -141,7 +153,6 @@ struct fp_solver {
 
 struct FluidParams {
   CellGrid cg;
-  uint32_t mmask;  // moved table
   int ns, n, nf;
   float rs, rf;
   float inv_H, H2_lo, H2_hi, max_pair, max_total;
@@ -200,6 +211,7 @@ static inline int fp_blocks(int n) { return (n + BLOCK - 1) / BLOCK; }
 
 // fp_grid.hip
 int fp_launch_cell_grid(fp_solver* s, bool count_done);
+int fp_launch_resort(fp_solver* s);
 int fp_launch_ref_grid(fp_solver* s, bool rehash);
 
 // fp_fluid.hip

@@ -76,7 +76,7 @@ __device__ __forceinline__ float sign_or_one(float q) { return q < 0.f ? -1.f : 
 
 // Bounds (solver.js:389-427) then velocity + damping + clamp (:219-234).
 // Solids read their positions from `solid_src` (the last constraint
-// iteration) and write vel; fluid velocities go to the sorted slot in svtmp,
+// iteration) and write vel; fluid velocities go to the fluid-order slot in svtmp,
 // where XSPH reads them (a particle culled here leaves its velocity as it
 // was, in vel and in svtmp).
 __global__ void __launch_bounds__(BLOCK) k_bounds_velocity(BoundsParams bp, float4* __restrict__ pos,
@@ -209,6 +209,7 @@ int fp_launch_substep(fp_solver* s, double h) {
   int rc = fp_launch_cell_grid(s, !grabs);
   if (rc) return rc;
   if ((rc = fp_launch_contacts(s))) return rc;
+  if ((rc = fp_launch_resort(s))) return rc;
   if ((rc = fp_launch_density(s))) return rc;
   if ((rc = fp_launch_displace(s))) return rc;
   const float4* solid_src = nullptr;
@@ -217,7 +218,7 @@ int fp_launch_substep(fp_solver* s, double h) {
     const BoundsParams bp = bounds_params(s, h);
     PassScope ps(s, P_BOUNDS_VELOCITY);
     hipLaunchKernelGGL(k_bounds_velocity, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, bp, s->pos, solid_src,
-                       s->prev, s->vel, s->svtmp, s->slot, s->active);
+                       s->prev, s->vel, s->svtmp, s->fslot, s->active);
   }
   if ((rc = fp_launch_xsph(s))) return rc;
   FP_TRY(s, hipGetLastError());

@@ -67,9 +67,11 @@ typedef struct fp_params {
   int32_t constraint_iters;    /* Jacobi XPBD iterations per substep (the reference runs one
                                   Gauss-Seidel sweep, :215); default 3 */
   int32_t table_size;          /* hash table size, power of two; 0 = nextpow2(2*capacity) (:51) */
-  int32_t moved_table;         /* buckets of the table of fluid particles that contacts moved to
-                                  another cell, power of two; 0 = nextpow2(capacity/8), >= 1024 */
-  int32_t reserved0;
+  int32_t pair_slots;          /* pairs recorded per fluid particle by the density pass for the
+                                  displacement and XSPH passes (the reference's pair list,
+                                  :284-327); a particle with more is gathered again from the
+                                  grid.  0 = 64 */
+  int32_t reserved0;           /* 0 */
   double jacobi_omega;         /* Jacobi relaxation: each constraint's correction is scaled by
                                   jacobi_omega / max(deg_a, deg_b); default 4.0 */
   double contact_scale;        /* scale on each Jacobi contact correction, 1.0 */
@@ -85,12 +87,12 @@ typedef struct fp_info {
   int32_t table_size;          /* :51 */
   int32_t num_constraints;     /* consA.length */
   int32_t fluid_next;          /* ring cursor of emitFluid (:140-141) */
-  int32_t moved_table;         /* moved-particle table size in use */
   int32_t device;              /* HIP device ordinal */
+  int32_t pair_slots;          /* pair_slots in use */
   int64_t substeps_done;       /* substeps executed since creation */
-  int64_t moved_overflow;      /* moved-table insertions that did not fit a bucket (cumulative;
-                                  non-zero means pairs of those particles were missed) */
   int64_t device_bytes;        /* HBM bytes held by this solver */
+  int64_t pair_overflow;       /* particle-substeps whose pairs did not fit pair_slots
+                                  (cumulative; those were gathered again, results unchanged) */
 } fp_info;
 
 /* ---- lifetime ------------------------------------------------------- */

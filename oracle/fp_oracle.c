@@ -136,7 +136,6 @@ void or_params_default(fp_params* p) {
   p->cell_size = 0.22;
   p->constraint_iters = 1;
   p->table_size = 0;
-  p->moved_table = 0;
   p->jacobi_omega = 1.0;
   p->contact_scale = 1.0;
 }

@@ -60,3 +60,28 @@ def test_params_default_and_error_paths_without_device():
 def test_params_struct_layout_matches_header():
     # 24 doubles, 4 int32, 2 doubles
     assert ctypes.sizeof(_abi.FpParams) == 24 * 8 + 4 * 4 + 2 * 8
+
+
+def _c_layout(struct, fields, tmp_path):
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fluidphysics.h"', 'int main(void) {',
+             '  printf("size %%zu\\n", sizeof(%s));' % struct]
+    lines += ['  printf("%s %%zu\\n", offsetof(%s, %s));' % (f, struct, f) for f in fields]
+    lines += ['  return 0;', '}']
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    return {k: int(v) for k, v in (ln.split() for ln in out if ln)}
+
+
+@pytest.mark.parametrize("struct,cls", [("fp_params", "FpParams"), ("fp_info", "FpInfo"),
+                                        ("fp_pass_stat", "FpPassStat")])
+def test_struct_offsets_match_the_c_compiler(struct, cls, tmp_path):
+    """Every field of the ctypes mirror sits where gcc puts it."""
+    c = getattr(_abi, cls)
+    fields = [f for f, _ in c._fields_]
+    lay = _c_layout(struct, fields, tmp_path)
+    assert lay.pop("size") == ctypes.sizeof(c)
+    for f in fields:
+        assert lay[f] == getattr(c, f).offset, f

@@ -344,3 +344,41 @@ def test_deterministic_repeat():
         g.close()
     for k in ("pos", "vel", "active"):
         assert np.array_equal(outs[0][k], outs[1][k])
+
+
+@pytest.mark.gpu
+def test_pair_list_overflow_regathers_bit_identically():
+    """Particles whose pairs do not fit the recorded list are gathered again
+    from the grid by the displacement and XSPH passes: a run with a tiny
+    list (most particles overflow) equals the default run bit for bit."""
+    log("case pair-list overflow single_gpu, pair_slots 8 vs default")
+    from fluidphysics_amd import from_scene, scenes
+    sc = scenes.single_gpu_scene()
+    outs, ovf = [], []
+    for slots in (0, 8):
+        g = from_scene(sc, pair_slots=slots)
+        for _f in range(4):
+            g.step(sc.dt, sc.substeps)
+        outs.append(g.read_state())
+        ovf.append(g.info().pair_overflow)
+        g.close()
+    log("  pair_overflow default %d, pair_slots 8 %d" % (ovf[0], ovf[1]))
+    assert ovf[1] > 0
+    for k in ("pos", "prev", "vel", "active"):
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+def test_pass_profile_names():
+    """Per-pass event timing reports every pass of a substep under its name."""
+    log("case pass profile plain_cpu")
+    d = G.scenario("plain_cpu")
+    g = make_gpu(d)
+    g.profile(True)
+    g.step(1.0 / 60.0, d["substeps"])
+    g.sync()
+    stats = g.profile_read()
+    g.profile(False)
+    for name in ("integrate", "contacts", "resort", "density", "displace", "constraints", "bounds_velocity",
+                 "xsph"):
+        assert name in stats and stats[name][0] == d["substeps"], name
+    g.close()
