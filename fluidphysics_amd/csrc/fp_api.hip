@@ -407,7 +407,7 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
   s->h_group.assign(ns, 0);
 
   setup_cell_grid(s);
-  s->pair_slots = s->P.pair_slots > 0 ? s->P.pair_slots : 64;
+  s->pair_slots = s->P.pair_slots > 0 ? s->P.pair_slots : 128;
   const int N = s->n, T = s->T;
   const size_t M = s->cg.M;
   if ((rc = dalloc(s, &s->pos, N)) || (rc = dalloc(s, &s->prev, N)) || (rc = dalloc(s, &s->vel, N)) ||

@@ -1,19 +1,22 @@
 // fp_fluid.hip -- contacts and the double-density-relaxation fluid passes,
 // run over the cell-sorted particles.
 //
-// Thread e handles the particle in sorted slot e.  The 27 neighbour cells of
-// a particle are 9 z-runs of 3 consecutive cell keys, i.e. 9 contiguous slot
-// ranges ("rows").  A block of 256 consecutive slots covers a compact piece
-// of space, so the union of its threads' rows is 9 short slot ranges: the
-// block stages them into LDS with coalesced 16-byte loads, then every thread
-// tests its ~190 candidates from LDS.  Blocks whose rows do not fit (sparse
-// spray, the table's wrap point) read the candidates from global memory.
+// The 27 neighbour cells of a particle are 9 z-runs of 3 consecutive cell
+// keys, i.e. 9 contiguous slot ranges ("rows") of a cell-sorted order.
+// Candidates are read straight from the sorted arrays, four per step so
+// their loads overlap; neighbouring threads read overlapping rows, which the
+// caches serve (staging the rows in LDS measured slower: it costs occupancy
+// and the rows of a block that spans several cell rows do not fit).
 //
-// contacts  (solver.js:246-279 per-pair formula, Jacobi): every active
-//           particle gathers from the grid-time snapshot spos: solids from
-//           non-bonded solids and all fluid, fluid from solids (only fluid
-//           inside the solids' bounding box, grown by the contact range,
-//           can touch one).  Solids write their corrected position to pos;
+// contacts  (solver.js:246-279 per-pair formula, Jacobi) from the grid-time
+//           snapshot spos.  Solids (k_contacts_solid, 4 lanes per solid,
+//           rows split between the lanes) against non-bonded solids and all
+//           fluid; the solid's alive bonds are held in registers.  Fluid
+//           (k_contacts_fluid) against solids only: cells list their
+//           particles by ascending index, so a cell's solids come first and
+//           the scan of a cell stops at its first fluid particle; only fluid
+//           inside the solids' bounding box, grown by the contact range, can
+//           touch a solid.  Solids write their corrected position to pos;
 //           fluid stages it (tpos) and counts itself under the key of its
 //           post-contact cell for the fluid order (fp_grid.hip).
 // density   (solver.js:284-327) rho = sum w q^2, rhoNear = sum w q^3 over the
@@ -42,7 +45,8 @@
 #include "fp_internal.h"
 #include "fp_kernels.h"
 
-#define STAGE_CAP 2560            // float4 slots of LDS per block (40 KB)
+#define CONTACT_LANES 4  // lanes per solid in the contact pass
+#define MAXB 32          // bonds per solid held in registers
 
 __device__ __forceinline__ uint32_t fbits(float w) { return __float_as_uint(w); }
 
@@ -69,94 +73,39 @@ __device__ __forceinline__ bool within_h(const FluidParams& fp, const float4& a,
   return dx * dx + dy * dy + dz * dz < fp.H2d;
 }
 
-// ---- block staging of the 9 neighbour rows ------------------------------------
-
-struct RowStage {
-  int lo[9], hi[9], off[10];
-  int ok;
-  uint32_t kmin, kmax;
-};
-
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off));
-  return v;
-}
-
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
-  return v;
-}
-
-// Every thread of the block calls this (it synchronises).  want: the thread
-// needs candidates, around cell key k.  On return st.ok says whether buf
-// holds the rows: for each of the 9 rows, the slots from the row of the
-// smallest key through the row of the largest key among the wanting threads
-// (a superset of every wanting thread's row, unless a row wraps the table:
-// then st.ok = 0).  The key span is reduced per wave with shuffles, so only
-// one lane per wave touches the shared bounds.
-template <class T>
-__device__ __forceinline__ void stage_rows(RowStage& st, T* buf, const T* __restrict__ src, const CellGrid& g,
-                                           const int* __restrict__ lstart, bool want, uint32_t k) {
-  if (threadIdx.x == 0) {
-    st.kmin = UINT_MAX;
-    st.kmax = 0u;
-    st.ok = 1;
-  }
-  __syncthreads();
-  const uint32_t wmin = wave_min_u32(want ? k : UINT_MAX);
-  const uint32_t wmax = wave_max_u32(want ? k : 0u);
-  if ((threadIdx.x & 63u) == 0u && wmin <= wmax) {
-    atomicMin(&st.kmin, wmin);
-    atomicMax(&st.kmax, wmax);
-  }
-  __syncthreads();
-  if (threadIdx.x < 9) {
-    const int r = threadIdx.x;
-    if (st.kmin > st.kmax) {
-      st.ok = 0;  // no wanting thread
-    } else {
-      const uint32_t a = row_key(g, st.kmin, r / 3 - 1, r % 3 - 1);
-      const uint32_t b = row_key(g, st.kmax, r / 3 - 1, r % 3 - 1);
-      if (b < a || b - a != st.kmax - st.kmin || b + 2u > g.kmask) {
-        st.ok = 0;
-      } else {
-        st.lo[r] = lstart[a];
-        st.hi[r] = lstart[b + 3u];
-      }
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0 && st.ok) {
-    int tot = 0;
-    for (int r = 0; r < 9; r++) {
-      st.off[r] = tot;
-      tot += st.hi[r] - st.lo[r];
-    }
-    st.off[9] = tot;
-    if (tot == 0 || tot > STAGE_CAP) st.ok = 0;
-  }
-  __syncthreads();
-  if (st.ok) {
-    const int tot = st.off[9];
-    for (int i = threadIdx.x; i < tot; i += BLOCK) {
-      int r = 0;
-#pragma unroll
-      for (int q = 1; q < 9; q++) r += (i >= st.off[q]) ? 1 : 0;
-      buf[i] = src[st.lo[r] + (i - st.off[r])];
-    }
-  }
-  __syncthreads();
-}
-
 // ---- contacts -----------------------------------------------------------------
 
-// Is the solid pair (x, y) joined by an alive constraint?  Scans x's CSR
-// row without an early exit, so the row's loads are independent and issue
-// back to back; one dependent load of the alive flag follows.
-__device__ __forceinline__ bool is_bonded(int x, int y, const int* __restrict__ adj_off, const int4* __restrict__ adj,
-                                          const uint8_t* __restrict__ alive) {
+// The alive bonds of solid x, in registers: the other endpoints of x's
+// alive constraints (-1 past the end).  Rows longer than MAXB are not held;
+// is_bonded scans the CSR row for them.
+struct Bonds {
+  int id[MAXB];
+  bool held;
+};
+
+__device__ __forceinline__ void load_bonds(int x, const int* __restrict__ adj_off, const int4* __restrict__ adj,
+                                           const uint8_t* __restrict__ alive, Bonds& b) {
+  const int k0 = adj_off[x], k1 = adj_off[x + 1];
+  b.held = k1 - k0 <= MAXB;
+#pragma unroll
+  for (int j = 0; j < MAXB; j++) {
+    int v = -1;
+    if (b.held && k0 + j < k1) {
+      const int4 a = adj[k0 + j];
+      if (alive[a.y]) v = a.x;
+    }
+    b.id[j] = v;
+  }
+}
+
+__device__ __forceinline__ bool is_bonded(int x, int y, const Bonds& b, const int* __restrict__ adj_off,
+                                          const int4* __restrict__ adj, const uint8_t* __restrict__ alive) {
+  if (b.held) {
+    bool hit = false;
+#pragma unroll
+    for (int j = 0; j < MAXB; j++) hit |= b.id[j] == y;
+    return hit;
+  }
   const int e = adj_off[x + 1];
   int c = -1;
   for (int k = adj_off[x]; k < e; k++) {
@@ -167,8 +116,9 @@ __device__ __forceinline__ bool is_bonded(int x, int y, const int* __restrict__ 
 }
 
 struct ContactCtx {
-  const float4* __restrict__ spos;  // xyz + inverse mass
+  const float4* __restrict__ spos;  // grid order: xyz + inverse mass
   const int* __restrict__ sid;
+  const int* __restrict__ lstart;   // grid-order cell starts
   const int* __restrict__ adj_off;
   const int4* __restrict__ adj;
   const uint8_t* __restrict__ alive;
@@ -177,16 +127,13 @@ struct ContactCtx {
 
 __device__ __forceinline__ float unorder(int v) { return __int_as_float(v >= 0 ? v : v ^ 0x7fffffff); }
 
-__device__ __forceinline__ void contact_pair(const FluidParams& fp, const ContactCtx& c, int x, bool solid_x,
-                                             const float4& px, float rx, int y, const float4& py, float& ax,
-                                             float& ay, float& az) {
-  const bool solid_y = y < fp.ns;
-  if (!solid_x && !solid_y) return;
+// The Jacobi correction of x from y (solver.js:267-275 with x's share).
+__device__ __forceinline__ void contact_add(const FluidParams& fp, const float4& px, float rx, const float4& py,
+                                            float ry, float& ax, float& ay, float& az) {
   const float dx = px.x - py.x, dy = px.y - py.y, dz = px.z - py.z;
   const float d2 = dx * dx + dy * dy + dz * dz;
-  const float rsum = rx + (solid_y ? fp.rs : fp.rf);
+  const float rsum = rx + ry;
   if (d2 >= rsum * rsum || d2 < 1e-12f) return;
-  if (solid_x && solid_y && is_bonded(x, y, c.adj_off, c.adj, c.alive)) return;
   const float d = sqrtf(d2);
   const float f = (rsum - d) / (d * (px.w + py.w)) * px.w * fp.contact_scale;
   ax += dx * f;
@@ -194,84 +141,97 @@ __device__ __forceinline__ void contact_pair(const FluidParams& fp, const Contac
   az += dz * f;
 }
 
-template <bool LDS>
-__device__ __forceinline__ void contact_rows(const FluidParams& fp, const ContactCtx& cc, const int* __restrict__ lstart,
-                                             const RowStage& st, const float4* buf, const int* bid, int e, int x,
-                                             bool solid_x, const float4& px, float rx, uint32_t k0, float& ax,
-                                             float& ay, float& az) {
-#pragma unroll 1
-  for (int r = 0; r < 9; r++) {
-    int r0, r1, w1;
-    run_ranges(fp.cg, lstart, row_key(fp.cg, k0, r / 3 - 1, r % 3 - 1), r0, r1, w1);
-    const int base = LDS ? st.off[r] - st.lo[r] : 0;
-    for (int q = r0; q < r1; q++) {
-      const int y = LDS ? bid[base + q] : cc.sid[q];
-      if (q == e || (!solid_x && y >= fp.ns)) continue;
-      contact_pair(fp, cc, x, solid_x, px, rx, y, LDS ? buf[base + q] : cc.spos[q], ax, ay, az);
-    }
-    for (int q = 0; q < w1; q++) {
-      if (q == e) continue;
-      contact_pair(fp, cc, x, solid_x, px, rx, cc.sid[q], cc.spos[q], ax, ay, az);
+// Solid x: CONTACT_LANES lanes split its 9 rows (row r to lane r % lanes),
+// each sums its candidates in slot order, then the lanes' sums are added
+// pairwise.  Writes pos[x] = grid-time position + correction.
+__global__ void __launch_bounds__(BLOCK) k_contacts_solid(FluidParams fp, ContactCtx cc, const uint8_t* __restrict__ active,
+                                                          float4* __restrict__ pos) {
+  const int t = blockIdx.x * BLOCK + threadIdx.x;
+  const int x = t / CONTACT_LANES, lane = t % CONTACT_LANES;
+  const bool valid = x < fp.ns && active[x];
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  float4 px = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (valid) {
+    px = pos[x];
+    Bonds bonds;
+    load_bonds(x, cc.adj_off, cc.adj, cc.alive, bonds);
+    const uint32_t k0 = cell_key(fp.cg, cell_of(px.x, px.y, px.z, fp.cg.inv_cell));
+    for (int r = lane; r < 9; r += CONTACT_LANES) {
+      int r0, r1, w1;
+      run_ranges(fp.cg, cc.lstart, row_key(fp.cg, k0, r / 3 - 1, r % 3 - 1), r0, r1, w1);
+      for (int pass = 0; pass < 2; pass++) {
+        const int qa = pass ? 0 : r0, qb = pass ? w1 : r1;
+        for (int q = qa; q < qb; q++) {
+          const int y = cc.sid[q];
+          if (y == x) continue;
+          const float4 py = cc.spos[q];
+          if (y < fp.ns) {
+            // a bonded pair does not collide; test the bond only when close
+            const float dx = px.x - py.x, dy = px.y - py.y, dz = px.z - py.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            const float rsum = fp.rs + fp.rs;
+            if (d2 >= rsum * rsum || d2 < 1e-12f) continue;
+            if (is_bonded(x, y, bonds, cc.adj_off, cc.adj, cc.alive)) continue;
+            contact_add(fp, px, fp.rs, py, fp.rs, ax, ay, az);
+          } else {
+            contact_add(fp, px, fp.rs, py, fp.rf, ax, ay, az);
+          }
+        }
+      }
     }
   }
+#pragma unroll
+  for (int off = 1; off < CONTACT_LANES; off <<= 1) {
+    ax += __shfl_xor(ax, off, CONTACT_LANES);
+    ay += __shfl_xor(ay, off, CONTACT_LANES);
+    az += __shfl_xor(az, off, CONTACT_LANES);
+  }
+  if (valid && lane == 0) pos[x] = make_float4(px.x + ax, px.y + ay, px.z + az, px.w);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_contacts(FluidParams fp, ContactCtx cc, const int* __restrict__ lstart,
-                                                    float4* __restrict__ pos, float4* __restrict__ tpos,
-                                                    uint32_t* __restrict__ tkey, uint32_t* __restrict__ tslot,
-                                                    int* __restrict__ lcount) {
-  __shared__ RowStage st;
-  __shared__ float4 buf[STAGE_CAP];
-  __shared__ int bid[STAGE_CAP];
+// Fluid in grid slot e: corrections from the solids of its 27 cells (rows in
+// order, cells of a row in order, solids of a cell by ascending index), then
+// the post-contact cell: flags, count, staging for the fluid order.
+__global__ void __launch_bounds__(BLOCK) k_contacts_fluid(FluidParams fp, ContactCtx cc, float4* __restrict__ tpos,
+                                                          uint32_t* __restrict__ tkey, uint32_t* __restrict__ tslot,
+                                                          int* __restrict__ lcount) {
   const int e = blockIdx.x * BLOCK + threadIdx.x;
-  const bool valid = e < lstart[fp.cg.M];
-  int x = 0;
-  float4 px = make_float4(0.f, 0.f, 0.f, 0.f);
-  bool solid_x = false, want = false;
-  Cell c0;
-  uint32_t k0 = 0;
-  if (valid) {
-    x = cc.sid[e];
-    px = cc.spos[e];
-    solid_x = x < fp.ns;
-    want = solid_x;
-    if (!solid_x && fp.ns > 0) {
-      const float g = fp.rs + fp.rf;
-      want = px.x > unorder(cc.sbox[0]) - g && px.x < unorder(cc.sbox[3]) + g && px.y > unorder(cc.sbox[1]) - g &&
-             px.y < unorder(cc.sbox[4]) + g && px.z > unorder(cc.sbox[2]) - g && px.z < unorder(cc.sbox[5]) + g;
-    }
-    c0 = cell_of(px.x, px.y, px.z, fp.cg.inv_cell);
-    k0 = cell_key(fp.cg, c0);
-  }
-  // stage positions and particle ids of the rows
-  stage_rows(st, buf, cc.spos, fp.cg, lstart, want, k0);
-  if (st.ok) {
-    for (int i = threadIdx.x; i < st.off[9]; i += BLOCK) {
-      int r = 0;
-#pragma unroll
-      for (int k = 1; k < 9; k++) r += (i >= st.off[k]) ? 1 : 0;
-      bid[i] = cc.sid[st.lo[r] + (i - st.off[r])];
-    }
-  }
-  __syncthreads();
-  if (!valid) {
-    if (e < fp.n) tkey[e] = NO_KEY;
-    return;
-  }
-  const float rx = solid_x ? fp.rs : fp.rf;
-  float ax = 0.f, ay = 0.f, az = 0.f;
-  if (want) {
-    if (st.ok)
-      contact_rows<true>(fp, cc, lstart, st, buf, bid, e, x, solid_x, px, rx, k0, ax, ay, az);
-    else
-      contact_rows<false>(fp, cc, lstart, st, buf, bid, e, x, solid_x, px, rx, k0, ax, ay, az);
-  }
-  const float ox = px.x + ax, oy = px.y + ay, oz = px.z + az;
-  if (solid_x) {
-    pos[x] = make_float4(ox, oy, oz, px.w);
+  if (e >= fp.n) return;
+  if (e >= cc.lstart[fp.cg.M]) {
     tkey[e] = NO_KEY;
     return;
   }
+  const int x = cc.sid[e];
+  if (x < fp.ns) {
+    tkey[e] = NO_KEY;
+    return;
+  }
+  const float4 px = cc.spos[e];
+  const Cell c0 = cell_of(px.x, px.y, px.z, fp.cg.inv_cell);
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  if (fp.ns > 0) {
+    const float g = fp.rs + fp.rf;
+    const bool near = px.x > unorder(cc.sbox[0]) - g && px.x < unorder(cc.sbox[3]) + g &&
+                      px.y > unorder(cc.sbox[1]) - g && px.y < unorder(cc.sbox[4]) + g &&
+                      px.z > unorder(cc.sbox[2]) - g && px.z < unorder(cc.sbox[5]) + g;
+    if (near) {
+      const uint32_t k0 = cell_key(fp.cg, c0);
+#pragma unroll 1
+      for (int r = 0; r < 9; r++) {
+        const uint32_t kr = row_key(fp.cg, k0, r / 3 - 1, r % 3 - 1);
+        for (uint32_t j = 0; j < 3; j++) {
+          const uint32_t k = (kr + j) & fp.cg.kmask;
+          const int qe = cc.lstart[k + 1];
+          for (int q = cc.lstart[k]; q < qe; q++) {
+            const int y = cc.sid[q];
+            if (y >= fp.ns) break;
+            contact_add(fp, px, fp.rf, cc.spos[q], fp.rs, ax, ay, az);
+          }
+        }
+      }
+    }
+  }
+  const float ox = px.x + ax, oy = px.y + ay, oz = px.z + az;
   const Cell c1 = cell_of(ox, oy, oz, fp.cg.inv_cell);
   uint32_t fl = 0;
   if (c1.x != c0.x || c1.y != c0.y || c1.z != c0.z) fl |= PF_MOVED;
@@ -334,26 +294,37 @@ __device__ __forceinline__ int pair_weight(const FluidParams& fp, const PairSour
 }
 
 // The pairs of x, in the deterministic order of the fluid order's rows:
-// f.pair(q, w, dx, dy, dz, d2).  LDS: the block's rows are staged in buf (a
-// separate instantiation, so each path keeps its own address space:
-// ds_read for LDS, global loads otherwise).
-template <bool LDS, class F>
-__device__ __forceinline__ void visit_rows(const FluidParams& fp, const PairSource& src, const PairSelf& me,
-                                           const RowStage& st, const float4* buf, F& f) {
+// f.pair(q, w, dx, dy, dz, d2).
+template <class F>
+__device__ __forceinline__ void visit_rows(const FluidParams& fp, const PairSource& src, const PairSelf& me, F& f) {
   const uint32_t kx = cell_key(fp.cg, me.c);
+  const float4* __restrict__ row = src.spos2;
 #pragma unroll 1
   for (int r = 0; r < 9; r++) {
     int r0, r1, w1;
     run_ranges(fp.cg, src.lstart, row_key(fp.cg, kx, r / 3 - 1, r % 3 - 1), r0, r1, w1);
-    const int base = LDS ? st.off[r] - st.lo[r] : 0;
-    for (int q = r0; q < r1; q++) {
+    int q = r0;
+    // four candidates per step: their loads are in flight together
+    for (; q + 4 <= r1; q += 4) {
+      const float4 c0 = row[q], c1 = row[q + 1], c2 = row[q + 2], c3 = row[q + 3];
       float dx, dy, dz, d2;
-      const int w = pair_weight(fp, src, me, q, LDS ? buf[base + q] : src.spos2[q], dx, dy, dz, d2);
+      int w = pair_weight(fp, src, me, q, c0, dx, dy, dz, d2);
+      if (w) f.pair(q, w, dx, dy, dz, d2);
+      w = pair_weight(fp, src, me, q + 1, c1, dx, dy, dz, d2);
+      if (w) f.pair(q + 1, w, dx, dy, dz, d2);
+      w = pair_weight(fp, src, me, q + 2, c2, dx, dy, dz, d2);
+      if (w) f.pair(q + 2, w, dx, dy, dz, d2);
+      w = pair_weight(fp, src, me, q + 3, c3, dx, dy, dz, d2);
+      if (w) f.pair(q + 3, w, dx, dy, dz, d2);
+    }
+    for (; q < r1; q++) {
+      float dx, dy, dz, d2;
+      const int w = pair_weight(fp, src, me, q, row[q], dx, dy, dz, d2);
       if (w) f.pair(q, w, dx, dy, dz, d2);
     }
-    for (int q = 0; q < w1; q++) {
+    for (q = 0; q < w1; q++) {
       float dx, dy, dz, d2;
-      const int w = pair_weight(fp, src, me, q, src.spos2[q], dx, dy, dz, d2);
+      const int w = pair_weight(fp, src, me, q, row[q], dx, dy, dz, d2);
       if (w) f.pair(q, w, dx, dy, dz, d2);
     }
   }
@@ -418,8 +389,7 @@ __device__ __forceinline__ void recorded_pairs(const FluidParams& fp, const Pair
   if (__builtin_expect(cnt <= pl.cap, 1)) {
     replay_pairs(src, pl, me, cnt, f);
   } else {
-    RowStage none;
-    visit_rows<false>(fp, src, me, none, nullptr, f);
+    visit_rows(fp, src, me, f);
   }
 }
 
@@ -451,21 +421,14 @@ struct DensityAcc {
 __global__ void __launch_bounds__(BLOCK) k_density(FluidParams fp, PairSource src, PairList pl,
                                                    float2* __restrict__ sP, float2* __restrict__ rho_out,
                                                    int* __restrict__ sbox, int* __restrict__ dflags) {
-  __shared__ RowStage st;
-  __shared__ float4 buf[STAGE_CAP];
   // the solids' bounding box is rebuilt by the next integrate
   if (blockIdx.x == 0 && threadIdx.x < 6) sbox[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : INT_MIN;
   PairSelf me;
-  const bool valid = pair_self(fp, src, blockIdx.x * BLOCK + threadIdx.x, me);
-  stage_rows(st, buf, src.spos2, fp.cg, src.lstart, valid, valid ? cell_key(fp.cg, me.c) : 0u);
-  if (!valid) return;
+  if (!pair_self(fp, src, blockIdx.x * BLOCK + threadIdx.x, me)) return;
   DensityAcc acc;
   acc.inv_H = fp.inv_H;
   Recorder<DensityAcc> rec(acc, pl, me.e);
-  if (st.ok)
-    visit_rows<true>(fp, src, me, st, buf, rec);
-  else
-    visit_rows<false>(fp, src, me, st, buf, rec);
+  visit_rows(fp, src, me, rec);
   if (rec.cnt > pl.cap) {
     pl.ncount[me.e] = pl.cap + 1;
     atomicAdd(&dflags[DF_PAIR_OVF], 1);
@@ -598,10 +561,13 @@ static PairList pair_list(const fp_solver* s) {
 
 int fp_launch_contacts(fp_solver* s) {
   const FluidParams fp = fp_fluid_params(s);
-  ContactCtx cc{s->spos, s->sid, s->adj_off, s->adj, s->alive[s->cur], s->sbox};
+  ContactCtx cc{s->spos, s->sid, s->lstart, s->adj_off, s->adj, s->alive[s->cur], s->sbox};
   PassScope ps(s, P_CONTACTS);
-  hipLaunchKernelGGL(k_contacts, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, cc, s->lstart, s->pos,
-                     s->tpos, s->tkey, s->tslot, s->lcount);
+  if (s->ns > 0)
+    hipLaunchKernelGGL(k_contacts_solid, dim3(fp_blocks(s->ns * CONTACT_LANES)), dim3(BLOCK), 0, s->stream, fp, cc,
+                       s->active, s->pos);
+  hipLaunchKernelGGL(k_contacts_fluid, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, cc, s->tpos, s->tkey,
+                     s->tslot, s->lcount);
   FP_TRY(s, hipGetLastError());
   return FP_OK;
 }

@@ -70,7 +70,7 @@ typedef struct fp_params {
   int32_t pair_slots;          /* pairs recorded per fluid particle by the density pass for the
                                   displacement and XSPH passes (the reference's pair list,
                                   :284-327); a particle with more is gathered again from the
-                                  grid.  0 = 64 */
+                                  grid.  0 = 128 */
   int32_t reserved0;           /* 0 */
   double jacobi_omega;         /* Jacobi relaxation: each constraint's correction is scaled by
                                   jacobi_omega / max(deg_a, deg_b); default 4.0 */
