@@ -45,6 +45,15 @@
 #include "fp_internal.h"
 #include "fp_kernels.h"
 
+#ifndef FLUID_BLOCK
+#define FLUID_BLOCK 256  // threads per block of the density, displacement and XSPH passes
+#endif
+#ifndef FLUID_XCD
+#define FLUID_XCD 0      // 1: give each XCD a contiguous range of slots
+#endif
+#ifndef NB_ROWMAJOR
+#define NB_ROWMAJOR 0    // 1: a particle's list entries are contiguous
+#endif
 #define CONTACT_LANES 4  // lanes per solid in the contact pass
 #define MAXB 32          // bonds per solid held in registers
 
@@ -62,6 +71,19 @@ __device__ __forceinline__ void run_ranges(const CellGrid& g, const int* __restr
     a0 = lstart[k0];
     a1 = lstart[g.M];
     b1 = lstart[k0 + 3u - g.M];
+  }
+}
+
+// The same for a run of len (1..3) keys starting at key k (k <= kmask).
+__device__ __forceinline__ void run_ranges_n(const CellGrid& g, const int* __restrict__ lstart, uint32_t k,
+                                             uint32_t len, int& a0, int& a1, int& b1) {
+  a0 = lstart[k];
+  if (k + len <= g.M) {
+    a1 = lstart[k + len];
+    b1 = 0;
+  } else {
+    a1 = lstart[g.M];
+    b1 = lstart[k + len - g.M];
   }
 }
 
@@ -242,6 +264,17 @@ __global__ void __launch_bounds__(BLOCK) k_contacts_fluid(FluidParams fp, Contac
   tpos[e] = make_float4(ox, oy, oz, __uint_as_float((uint32_t)x | (fl << PF_SHIFT)));
 }
 
+// Fluid slot of this thread.  Workgroup b runs on XCD b % 8; with FLUID_XCD
+// the blocks are renumbered so that each XCD (and its L2) works through one
+// contiguous eighth of the cell-sorted slots.  The grid is a multiple of 8.
+__device__ __forceinline__ int fluid_slot() {
+  int b = blockIdx.x;
+  if (FLUID_XCD) b = (b & 7) * (int)(gridDim.x >> 3) + (b >> 3);
+  return b * FLUID_BLOCK + (int)threadIdx.x;
+}
+
+static inline int fluid_blocks(int n) { return ((n + FLUID_BLOCK - 1) / FLUID_BLOCK + 7) & ~7; }
+
 // ---- the reference's fluid pairs of the particle in slot e ------------------
 
 struct PairSource {
@@ -258,20 +291,39 @@ struct PairSelf {
   float4 p;
 };
 
-// The reference-hash multiplicity (rare path: moved particles, repeated
-// buckets), kept out of line so the candidate loop stays small.
-__device__ __attribute__((noinline)) int multiplicity(const Cell& c, uint32_t mask, uint32_t target) {
+// The reference-hash multiplicities, out of line (scalar arguments only,
+// so the call passes registers): probes of cell c against bucket target,
+// and probes of the cell of (x, y, z) against bucket target.
+__device__ __attribute__((noinline)) int mult_cell(double cx, double cy, double cz, uint32_t mask, uint32_t target) {
+  Cell c;
+  c.x = cx;
+  c.y = cy;
+  c.z = cz;
   return probe_multiplicity(c, mask, target);
 }
 
-__device__ __attribute__((noinline)) int multiplicity_of(float x, float y, float z, double inv, uint32_t mask,
-                                                         uint32_t target) {
+__device__ __attribute__((noinline)) int mult_pos(float x, float y, float z, double inv, uint32_t mask,
+                                                  uint32_t target) {
   return probe_multiplicity(cell_of(x, y, z, inv), mask, target);
 }
 
-// Weight of the candidate (0: not a pair of x), written to keep the common
-// case branch-free: one combined predicate, the exact double test and the
-// multiplicity count only where they can matter.
+// Weight of a candidate that needs more than the float test: d^2 within
+// 1e-5 of H^2 (decided in double), or a multiplicity that is not 1.
+__device__ __forceinline__ int slow_weight(const FluidParams& fp, const PairSource& src, const PairSelf& me, int q,
+                                           const float4& py, float d2) {
+  if (d2 >= fp.H2_lo && !within_h(fp, me.p, py, d2)) return 0;
+  const uint32_t by = fbits(py.w);
+  const int y = (int)(by & SID_MASK);
+  const uint32_t fy = by >> PF_SHIFT;
+  // y > x: x's probes against y's grid bucket; y < x: y's probes against x's
+  const bool special = (y > me.x) ? (me.dup | ((fy & PF_MOVED) != 0u)) : (me.moved | ((fy & PF_DUP) != 0u));
+  if (!special) return 1;
+  return y > me.x ? mult_cell(me.c.x, me.c.y, me.c.z, fp.cg.hmask, src.sphash[q])
+                  : mult_pos(py.x, py.y, py.z, fp.cg.inv_cell, fp.cg.hmask, me.gh);
+}
+
+// Weight of the candidate in slot q (0: not a pair of x).  The common case
+// is decided by float compares; one rarely taken branch covers the rest.
 __device__ __forceinline__ int pair_weight(const FluidParams& fp, const PairSource& src, const PairSelf& me, int q,
                                            const float4& py, float& dx, float& dy, float& dz, float& d2) {
   const uint32_t by = fbits(py.w);
@@ -281,53 +333,31 @@ __device__ __forceinline__ int pair_weight(const FluidParams& fp, const PairSour
   dy = me.p.y - py.y;
   dz = me.p.z - py.z;
   d2 = dx * dx + dy * dy + dz * dz;
-  bool ok = (q != me.e) & (d2 < fp.H2_hi);
-  if (__builtin_expect(ok && d2 >= fp.H2_lo, 0)) ok = within_h(fp, me.p, py, d2);
-  if (!ok) return 0;
-  // y > x: x's probes against y's grid bucket; y < x: y's probes against x's
+  const bool cand = q != me.e;
+  const bool inside = cand & (d2 < fp.H2_lo);
+  const bool band = cand & (d2 >= fp.H2_lo) & (d2 < fp.H2_hi);
   const bool special = (y > me.x) ? (me.dup | ((fy & PF_MOVED) != 0u)) : (me.moved | ((fy & PF_DUP) != 0u));
-  if (__builtin_expect(special, 0)) {
-    return y > me.x ? multiplicity(me.c, fp.cg.hmask, src.sphash[q])
-                    : multiplicity_of(py.x, py.y, py.z, fp.cg.inv_cell, fp.cg.hmask, me.gh);
-  }
-  return 1;
+  int w = inside ? 1 : 0;
+  if (__builtin_expect(band | (inside & special), 0)) w = slow_weight(fp, src, me, q, py, d2);
+  return w;
 }
 
-// The pairs of x, in the deterministic order of the fluid order's rows:
-// f.pair(q, w, dx, dy, dz, d2).
-template <class F>
-__device__ __forceinline__ void visit_rows(const FluidParams& fp, const PairSource& src, const PairSelf& me, F& f) {
-  const uint32_t kx = cell_key(fp.cg, me.c);
-  const float4* __restrict__ row = src.spos2;
-#pragma unroll 1
-  for (int r = 0; r < 9; r++) {
-    int r0, r1, w1;
-    run_ranges(fp.cg, src.lstart, row_key(fp.cg, kx, r / 3 - 1, r % 3 - 1), r0, r1, w1);
-    int q = r0;
-    // four candidates per step: their loads are in flight together
-    for (; q + 4 <= r1; q += 4) {
-      const float4 c0 = row[q], c1 = row[q + 1], c2 = row[q + 2], c3 = row[q + 3];
-      float dx, dy, dz, d2;
-      int w = pair_weight(fp, src, me, q, c0, dx, dy, dz, d2);
-      if (w) f.pair(q, w, dx, dy, dz, d2);
-      w = pair_weight(fp, src, me, q + 1, c1, dx, dy, dz, d2);
-      if (w) f.pair(q + 1, w, dx, dy, dz, d2);
-      w = pair_weight(fp, src, me, q + 2, c2, dx, dy, dz, d2);
-      if (w) f.pair(q + 2, w, dx, dy, dz, d2);
-      w = pair_weight(fp, src, me, q + 3, c3, dx, dy, dz, d2);
-      if (w) f.pair(q + 3, w, dx, dy, dz, d2);
-    }
-    for (; q < r1; q++) {
-      float dx, dy, dz, d2;
-      const int w = pair_weight(fp, src, me, q, row[q], dx, dy, dz, d2);
-      if (w) f.pair(q, w, dx, dy, dz, d2);
-    }
-    for (q = 0; q < w1; q++) {
-      float dx, dy, dz, d2;
-      const int w = pair_weight(fp, src, me, q, row[q], dx, dy, dz, d2);
-      if (w) f.pair(q, w, dx, dy, dz, d2);
-    }
-  }
+struct RowRange {
+  int r0, r1, w1;  // [r0, r1) and, past the table's end, [0, w1)
+};
+
+// Slot range(s) of row r of x (empty when culled).
+__device__ __forceinline__ RowRange row_range(const FluidParams& fp, const int* __restrict__ lstart, uint32_t kx, int r,
+                                              float fx0, float fx1, float fy0, float fy1, float fz0, float fz1) {
+  RowRange rr{0, 0, 0};
+  const int a = r / 3 - 1, b = r % 3 - 1;
+  const float ex = a < 0 ? fx0 : fx1, ey = b < 0 ? fy0 : fy1;
+  const float dxy2 = (a != 0 ? ex * ex : 0.f) + (b != 0 ? ey * ey : 0.f);
+  if (!(dxy2 < fp.cull_r2) && dxy2 == dxy2) return rr;  // NaN positions visit every cell
+  const uint32_t lo = (dxy2 + fz0 * fz0 < fp.cull_r2) ? 0u : 1u;
+  const uint32_t hi = (dxy2 + fz1 * fz1 < fp.cull_r2) ? 3u : 2u;
+  run_ranges_n(fp.cg, lstart, (row_key(fp.cg, kx, a, b) + lo) & fp.cg.kmask, hi - lo, rr.r0, rr.r1, rr.w1);
+  return rr;
 }
 
 // ---- the pair list ----------------------------------------------------------------
@@ -339,7 +369,8 @@ __device__ __forceinline__ void visit_rows(const FluidParams& fp, const PairSour
 // contiguous run.  An entry packs the neighbour's fluid slot (27 bits) and
 // the weight - 1 (5 bits).  A particle with more than pair_slots pairs gets
 // the count pair_slots + 1 and is gathered again from the grid by the later
-// passes (same pairs, same order).
+// passes (same pairs, same order); its extra entries go to a sink row
+// (row pair_slots) so the recording loop needs no bounds branch.
 
 #define NB_QBITS 27
 #define NB_QMASK ((1u << NB_QBITS) - 1u)
@@ -350,35 +381,180 @@ struct PairList {
   int stride, cap;
 };
 
-// Replays the recorded pairs of me into f, in the order they were found.
-template <class F>
-__device__ __forceinline__ void replay_pairs(const PairSource& src, const PairList& pl, const PairSelf& me, int cnt,
-                                             F& f) {
-  const uint32_t* __restrict__ row = pl.nbr + me.e;
-  for (int k = 0; k < cnt; k++) {
-    const uint32_t v = row[(size_t)k * pl.stride];
-    const int q = (int)(v & NB_QMASK);
-    const int w = (int)(v >> NB_QBITS) + 1;
-    const float4 py = src.spos2[q];
-    const float dx = me.p.x - py.x, dy = me.p.y - py.y, dz = me.p.z - py.z;
-    f.pair(q, w, dx, dy, dz, dx * dx + dy * dy + dz * dz);
+// v_sqrt_f32 (within 1 ulp); the reference's own square root is a double
+// one, so no float32 rounding of it is more faithful than another.
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// Density sums and the pair list of x: every candidate of the 9 culled rows
+// of the fluid order, in slot order.  A z-cell of a row, or a whole row,
+// whose box lies farther than H from x is skipped: it cannot hold a pair
+// (cull_r2 carries a margin over the float and double tests).
+//
+// Memory latency and branch overhead bound this loop, so: the next row's
+// bounds load while the current row runs; a row is read in batches of four
+// candidates (the last batch predicated, no serial tail), each batch's
+// loads issued before the previous batch is evaluated; the common case is
+// decided with selects and a single, rarely taken branch per batch covers
+// the d ~ H band and multiplicities other than 1.
+struct DensityGather {
+  float rho = 0.f, rhon = 0.f;
+  int cnt = 0;
+  uint32_t* wp;  // where entry cnt goes while cnt < cap
+};
+
+__device__ __forceinline__ void density_take(const FluidParams& fp, const PairList& pl, int q, int w, float d2,
+                                             DensityGather& g) {
+  const float u = fmaf(-fast_sqrt(d2), fp.inv_H, 1.f);
+  const float u2 = (float)w * (u * u);
+  g.rho += w ? u2 : 0.f;
+  g.rhon += w ? u2 * u : 0.f;
+  if (w) {
+    // entries past pl.cap all land in the sink row pl.cap, which nothing reads
+    *g.wp = (uint32_t)q | ((uint32_t)(w - 1) << NB_QBITS);
+    g.wp += g.cnt < pl.cap ? pl.stride : 0;
+    g.cnt++;
   }
 }
 
-// Records every pair it is handed, then forwards it.
-template <class F>
-struct Recorder {
-  F& f;
-  const PairList& pl;
-  int e;
-  int cnt = 0;
-  __device__ Recorder(F& f_, const PairList& pl_, int e_) : f(f_), pl(pl_), e(e_) {}
-  __device__ __forceinline__ void pair(int q, int w, float dx, float dy, float dz, float d2) {
-    if (cnt < pl.cap) pl.nbr[(size_t)cnt * pl.stride + e] = (uint32_t)q | ((uint32_t)(w - 1) << NB_QBITS);
-    cnt++;
-    f.pair(q, w, dx, dy, dz, d2);
+// One batch of four candidates in slots q .. q+3 (those past last are not
+// candidates).  Weight 1 for d^2 < H^2 (1 - 1e-5) unless a flag is set on
+// either particle; the band and flagged cases go to the one rare branch.
+__device__ __forceinline__ void density_batch(const FluidParams& fp, const PairSource& src, const PairList& pl,
+                                              const PairSelf& me, bool me_flagged, int q, int last, const float4* c,
+                                              DensityGather& g) {
+  float d2[4];
+  int w[4];
+  bool rare = false;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const float dx = me.p.x - c[i].x, dy = me.p.y - c[i].y, dz = me.p.z - c[i].z;
+    d2[i] = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const bool cand = (q + i <= last) & (q + i != me.e);
+    const bool inside = cand & (d2[i] < fp.H2_lo);
+    const bool band = cand & (d2[i] >= fp.H2_lo) & (d2[i] < fp.H2_hi);
+    const bool flagged = me_flagged | (fbits(c[i].w) >= (1u << PF_SHIFT));
+    w[i] = inside ? 1 : 0;
+    rare |= band | (inside & flagged);
   }
-};
+  if (__builtin_expect(rare, 0)) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      if ((q + i <= last) & (q + i != me.e) & (d2[i] < fp.H2_hi)) w[i] = slow_weight(fp, src, me, q + i, c[i], d2[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) density_take(fp, pl, q + i, w[i], d2[i], g);
+}
+
+__device__ __forceinline__ void load4(const float4* __restrict__ row, int q, int last, float4* b) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) b[i] = row[min(q + i, last)];
+}
+
+__device__ __forceinline__ void density_gather(const FluidParams& fp, const PairSource& src, const PairList& pl,
+                                               const PairSelf& me, DensityGather& g) {
+  const uint32_t kx = cell_key(fp.cg, me.c);
+  const float4* __restrict__ row = src.spos2;
+  const bool me_flagged = me.moved | me.dup;
+  // distances from x to the lower and upper faces of its cell
+  const float fx0 = (float)((double)me.p.x - me.c.x * fp.cs), fx1 = (float)((me.c.x + 1.0) * fp.cs - (double)me.p.x);
+  const float fy0 = (float)((double)me.p.y - me.c.y * fp.cs), fy1 = (float)((me.c.y + 1.0) * fp.cs - (double)me.p.y);
+  const float fz0 = (float)((double)me.p.z - me.c.z * fp.cs), fz1 = (float)((me.c.z + 1.0) * fp.cs - (double)me.p.z);
+  RowRange nx = row_range(fp, src.lstart, kx, 0, fx0, fx1, fy0, fy1, fz0, fz1);
+#pragma unroll 1
+  for (int r = 0; r < 9; r++) {
+    const RowRange rr = nx;
+    if (r < 8) nx = row_range(fp, src.lstart, kx, r + 1, fx0, fx1, fy0, fy1, fz0, fz1);
+    if (rr.r1 > rr.r0) {
+      const int last = rr.r1 - 1;
+      // two buffers in turn: the next batch's loads are in flight while the
+      // current batch is evaluated
+      float4 ba[4], bb[4];
+      load4(row, rr.r0, last, ba);
+      for (int q = rr.r0; q <= last; q += 8) {
+        if (q + 4 <= last) load4(row, q + 4, last, bb);
+        density_batch(fp, src, pl, me, me_flagged, q, last, ba, g);
+        if (q + 4 > last) break;
+        if (q + 8 <= last) load4(row, q + 8, last, ba);
+        density_batch(fp, src, pl, me, me_flagged, q + 4, last, bb, g);
+      }
+    }
+    for (int q = 0; q < rr.w1; q++) {
+      float dx, dy, dz, d2;
+      const int w = pair_weight(fp, src, me, q, row[q], dx, dy, dz, d2);
+      density_take(fp, pl, q, w, d2, g);
+    }
+  }
+}
+
+// The pairs of x from the grid, one candidate at a time (the rare regather
+// of a particle whose recorded list overflowed; same order as density_gather).
+template <class F>
+__device__ __forceinline__ void visit_rows_simple(const FluidParams& fp, const PairSource& src, const PairSelf& me,
+                                                  F& f) {
+  const uint32_t kx = cell_key(fp.cg, me.c);
+  const float fx0 = (float)((double)me.p.x - me.c.x * fp.cs), fx1 = (float)((me.c.x + 1.0) * fp.cs - (double)me.p.x);
+  const float fy0 = (float)((double)me.p.y - me.c.y * fp.cs), fy1 = (float)((me.c.y + 1.0) * fp.cs - (double)me.p.y);
+  const float fz0 = (float)((double)me.p.z - me.c.z * fp.cs), fz1 = (float)((me.c.z + 1.0) * fp.cs - (double)me.p.z);
+#pragma unroll 1
+  for (int r = 0; r < 9; r++) {
+    const RowRange rr = row_range(fp, src.lstart, kx, r, fx0, fx1, fy0, fy1, fz0, fz1);
+    for (int pass = 0; pass < 2; pass++) {
+      const int qa = pass ? 0 : rr.r0, qb = pass ? rr.w1 : rr.r1;
+      for (int q = qa; q < qb; q++) {
+        float dx, dy, dz, d2;
+        const int w = pair_weight(fp, src, me, q, src.spos2[q], dx, dy, dz, d2);
+        if (w) f.pair(q, w, dx, dy, dz, d2);
+      }
+    }
+  }
+}
+
+// Replays the recorded pairs of me into f, in the order they were found.
+// Entries are read one batch of four ahead; a batch's gathers (the
+// neighbour's position when F::kNeedPos, and f.fetch(q)) are issued
+// together before any of them is used.
+template <class F>
+__device__ __forceinline__ void replay_pairs(const PairSource& src, const PairList& pl, const PairSelf& me, int cnt,
+                                             F& f) {
+  const uint32_t* __restrict__ row = pl.nbr + (NB_ROWMAJOR ? (size_t)me.e * (pl.cap + 1) : (size_t)me.e);
+  const size_t st = NB_ROWMAJOR ? 1 : (size_t)pl.stride;
+  const int last = cnt - 1;
+  uint32_t v[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) v[i] = cnt > 0 ? row[(size_t)min(i, last) * st] : 0u;
+  for (int k = 0; k < cnt; k += 4) {
+    uint32_t u[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) u[i] = v[i];
+    if (k + 4 < cnt) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) v[i] = row[(size_t)min(k + 4 + i, last) * st];
+    }
+    float4 py[4];
+    typename F::Extra ex[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int q = (int)(u[i] & NB_QMASK);
+      if (F::kNeedPos) py[i] = src.spos2[q];
+      ex[i] = f.fetch(q);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      if (k + i <= last) {
+        const int q = (int)(u[i] & NB_QMASK);
+        const int w = (int)(u[i] >> NB_QBITS) + 1;
+        float dx = 0.f, dy = 0.f, dz = 0.f, d2 = 0.f;
+        if (F::kNeedPos) {
+          dx = me.p.x - py[i].x;
+          dy = me.p.y - py[i].y;
+          dz = me.p.z - py[i].z;
+          d2 = dx * dx + dy * dy + dz * dz;
+        }
+        f.use(q, w, dx, dy, dz, d2, ex[i]);
+      }
+    }
+  }
+}
 
 // The pairs of me for the passes after density: the recorded list, or the
 // grid again (global reads) for a particle whose list overflowed.
@@ -389,7 +565,7 @@ __device__ __forceinline__ void recorded_pairs(const FluidParams& fp, const Pair
   if (__builtin_expect(cnt <= pl.cap, 1)) {
     replay_pairs(src, pl, me, cnt, f);
   } else {
-    visit_rows(fp, src, me, f);
+    visit_rows_simple(fp, src, me, f);
   }
 }
 
@@ -407,48 +583,38 @@ __device__ __forceinline__ bool pair_self(const FluidParams& fp, const PairSourc
   return true;
 }
 
-struct DensityAcc {
-  float inv_H;
-  float rho = 0.f, rhon = 0.f;
-  __device__ __forceinline__ void pair(int, int w, float, float, float, float d2) {
-    const float q = 1.f - sqrtf(d2) * inv_H;
-    const float q2 = q * q;
-    rho += (float)w * q2;
-    rhon += (float)w * q2 * q;
-  }
-};
-
-__global__ void __launch_bounds__(BLOCK) k_density(FluidParams fp, PairSource src, PairList pl,
+__global__ void __launch_bounds__(FLUID_BLOCK) k_density(FluidParams fp, PairSource src, PairList pl,
                                                    float2* __restrict__ sP, float2* __restrict__ rho_out,
                                                    int* __restrict__ sbox, int* __restrict__ dflags) {
   // the solids' bounding box is rebuilt by the next integrate
   if (blockIdx.x == 0 && threadIdx.x < 6) sbox[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : INT_MIN;
   PairSelf me;
-  if (!pair_self(fp, src, blockIdx.x * BLOCK + threadIdx.x, me)) return;
-  DensityAcc acc;
-  acc.inv_H = fp.inv_H;
-  Recorder<DensityAcc> rec(acc, pl, me.e);
-  visit_rows(fp, src, me, rec);
-  if (rec.cnt > pl.cap) {
+  if (!pair_self(fp, src, fluid_slot(), me)) return;
+  DensityGather g;
+  g.wp = pl.nbr + (NB_ROWMAJOR ? (size_t)me.e * (pl.cap + 1) : (size_t)me.e);
+  density_gather(fp, src, pl, me, g);
+  if (g.cnt > pl.cap) {
     pl.ncount[me.e] = pl.cap + 1;
     atomicAdd(&dflags[DF_PAIR_OVF], 1);
   } else {
-    pl.ncount[me.e] = rec.cnt;
+    pl.ncount[me.e] = g.cnt;
   }
-  rho_out[me.x] = make_float2(acc.rho, acc.rhon);
-  sP[me.e] = make_float2(fp.stiff * (acc.rho - fp.rest_density), fp.near_stiff * acc.rhon);
+  rho_out[me.x] = make_float2(g.rho, g.rhon);
+  sP[me.e] = make_float2(fp.stiff * (g.rho - fp.rest_density), fp.near_stiff * g.rhon);
 }
 
 struct DisplaceAcc {
+  static constexpr bool kNeedPos = true;
+  typedef float2 Extra;
   const float2* __restrict__ sP;
   float2 Px;
   float inv_H, max_pair;
   float sx = 0.f, sy = 0.f, sz = 0.f;
-  __device__ __forceinline__ void pair(int q, int w, float dx, float dy, float dz, float d2) {
-    const float d = sqrtf(d2);
+  __device__ __forceinline__ float2 fetch(int q) const { return sP[q]; }
+  __device__ __forceinline__ void use(int, int w, float dx, float dy, float dz, float d2, float2 Py) {
+    const float d = fast_sqrt(d2);
     if (d < 1e-9f) return;
     const float u = 1.f - d * inv_H;
-    const float2 Py = sP[q];
     float D = 0.25f * ((Px.x + Py.x) * u + (Px.y + Py.y) * u * u);
     D = fminf(fmaxf(D, -max_pair), max_pair);
     const float s = (float)w * D / d;
@@ -456,14 +622,17 @@ struct DisplaceAcc {
     sy += dy * s;
     sz += dz * s;
   }
+  __device__ __forceinline__ void pair(int q, int w, float dx, float dy, float dz, float d2) {
+    use(q, w, dx, dy, dz, d2, fetch(q));
+  }
 };
 
 // spos2 + the clamped displacement -> pos for the fluid; the inverse mass
 // is kept from pos.w.
-__global__ void __launch_bounds__(BLOCK) k_displace(FluidParams fp, PairSource src, PairList pl,
+__global__ void __launch_bounds__(FLUID_BLOCK) k_displace(FluidParams fp, PairSource src, PairList pl,
                                                     const float2* __restrict__ sP, float4* __restrict__ pos) {
   PairSelf me;
-  if (!pair_self(fp, src, blockIdx.x * BLOCK + threadIdx.x, me)) return;
+  if (!pair_self(fp, src, fluid_slot(), me)) return;
   DisplaceAcc acc;
   acc.sP = sP;
   acc.Px = sP[me.e];
@@ -483,26 +652,31 @@ __global__ void __launch_bounds__(BLOCK) k_displace(FluidParams fp, PairSource s
 }
 
 struct XsphAcc {
+  static constexpr bool kNeedPos = false;
+  typedef float4 Extra;
   const float4* __restrict__ svtmp;
   float sx = 0.f, sy = 0.f, sz = 0.f;
   int m = 0;
-  __device__ __forceinline__ void pair(int q, int w, float, float, float, float) {
-    const float4 v = svtmp[q];
+  __device__ __forceinline__ float4 fetch(int q) const { return svtmp[q]; }
+  __device__ __forceinline__ void use(int, int w, float, float, float, float, float4 v) {
     const float fw = (float)w;
     sx += fw * v.x;
     sy += fw * v.y;
     sz += fw * v.z;
     m += w;
   }
+  __device__ __forceinline__ void pair(int q, int w, float dx, float dy, float dz, float d2) {
+    use(q, w, dx, dy, dz, d2, fetch(q));
+  }
 };
 
 // XSPH: svtmp -> vel for the fluid that is still active (the reference skips
 // particles culled since the density pass, whose velocity stays as it was).
-__global__ void __launch_bounds__(BLOCK) k_xsph(FluidParams fp, PairSource src, PairList pl,
+__global__ void __launch_bounds__(FLUID_BLOCK) k_xsph(FluidParams fp, PairSource src, PairList pl,
                                                 const float4* __restrict__ svtmp, float4* __restrict__ vel,
                                                 const uint8_t* __restrict__ active) {
   PairSelf me;
-  if (!pair_self(fp, src, blockIdx.x * BLOCK + threadIdx.x, me)) return;
+  if (!pair_self(fp, src, fluid_slot(), me)) return;
   if (!active[me.x]) return;
   const float4 v = svtmp[me.e];
   XsphAcc acc;
@@ -532,6 +706,8 @@ FluidParams fp_fluid_params(const fp_solver* s) {
   fp.H2d = P.fluid_h * P.fluid_h;
   fp.H2_lo = (float)(fp.H2d * (1.0 - 1e-5));
   fp.H2_hi = (float)(fp.H2d * (1.0 + 1e-5));
+  fp.cs = P.cell_size;
+  fp.cull_r2 = (float)(fp.H2d * (1.0 + 1e-4));
   fp.max_pair = (float)(0.05 * P.fluid_h);
   fp.max_total = (float)(0.35 * P.fluid_h);
   fp.stiff = (float)P.stiffness;
@@ -575,7 +751,7 @@ int fp_launch_contacts(fp_solver* s) {
 int fp_launch_density(fp_solver* s) {
   const FluidParams fp = fp_fluid_params(s);
   PassScope ps(s, P_DENSITY);
-  hipLaunchKernelGGL(k_density, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, pair_source(s), pair_list(s),
+  hipLaunchKernelGGL(k_density, dim3(fluid_blocks(s->n)), dim3(FLUID_BLOCK), 0, s->stream, fp, pair_source(s), pair_list(s),
                      s->sP, s->rho, s->sbox, s->dflags);
   FP_TRY(s, hipGetLastError());
   return FP_OK;
@@ -584,7 +760,7 @@ int fp_launch_density(fp_solver* s) {
 int fp_launch_displace(fp_solver* s) {
   const FluidParams fp = fp_fluid_params(s);
   PassScope ps(s, P_DISPLACE);
-  hipLaunchKernelGGL(k_displace, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, pair_source(s),
+  hipLaunchKernelGGL(k_displace, dim3(fluid_blocks(s->n)), dim3(FLUID_BLOCK), 0, s->stream, fp, pair_source(s),
                      pair_list(s), s->sP, s->pos);
   FP_TRY(s, hipGetLastError());
   return FP_OK;
@@ -594,7 +770,7 @@ int fp_launch_xsph(fp_solver* s) {
   if (s->nf == 0) return FP_OK;
   const FluidParams fp = fp_fluid_params(s);
   PassScope ps(s, P_XSPH);
-  hipLaunchKernelGGL(k_xsph, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, pair_source(s), pair_list(s),
+  hipLaunchKernelGGL(k_xsph, dim3(fluid_blocks(s->n)), dim3(FLUID_BLOCK), 0, s->stream, fp, pair_source(s), pair_list(s),
                      s->svtmp, s->vel, s->active);
   FP_TRY(s, hipGetLastError());
   return FP_OK;

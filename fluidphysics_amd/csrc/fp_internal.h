@@ -18,7 +18,7 @@
 //   density, displacement, XSPH): fsid, fslot i32[N] (inverse), spos2
 //     float4 (= tpos), sphash2 u32, sP float2 (P, Pnear), svtmp float4
 //     (velocity before XSPH)
-//   nbr u32[K*N] (entry k of fluid slot e at k*N + e), ncount i32[N]: the
+//   nbr u32[(K+1)*N] (entry k of fluid slot e at k*N + e; row K is a sink), ncount i32[N]: the
 //     pairs the density pass found, replayed by displacement and XSPH
 //   count/cell_start/entries_tmp/cell_entries: the reference's hash grid,
 //     rebuilt from phash only when read
@@ -103,7 +103,7 @@ struct fp_solver {
   float4 *spos2 = nullptr, *svtmp = nullptr;
   uint32_t* sphash2 = nullptr;
   float2* sP = nullptr;
-  uint32_t* nbr = nullptr;  // pair list, pair_slots x capacity (slot-major)
+  uint32_t* nbr = nullptr;  // pair list, (pair_slots + 1) x capacity, entry-major; the last row is a sink
   int* ncount = nullptr;
   int pair_slots = 0;
   int* sbox = nullptr;  // solids' bounding box at grid time (6 ordered ints)
@@ -157,6 +157,8 @@ struct FluidParams {
   float rs, rf;
   float inv_H, H2_lo, H2_hi, max_pair, max_total;
   double H2d;
+  double cs;       // cell size
+  float cull_r2;   // a cell whose box lies at least this far (squared) holds no pair
   float stiff, near_stiff, rest_density;
   float contact_scale, visc;
 };
