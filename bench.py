@@ -32,6 +32,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled by a plain copy kernel of known size); see the file for commands.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "large_hbm_traffic.json")
 
 
 def model_bytes_per_substep(n, nf, ns, c, t, iters):
@@ -59,6 +62,20 @@ def pass_bytes(name, n, nf, ns, c, t):
         "scan_reduce": 4 * t,
         "scan_apply": 8 * t,
     }.get(name)
+
+
+def measured_traffic(workload, pass_name):
+    """HBM bytes per launch of the pass's kernel on this workload, from the
+    committed rocprofv3 counter summary (None when absent)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload:
+        return None
+    k = t.get("kernels", {}).get("k_" + pass_name)
+    return int(k["hbm_bytes"]) if k else None
 
 
 def run_frames(solver, scene, f0, frames, rest_pos, grab_ids, sync_each=False):
@@ -186,9 +203,12 @@ def main():
     avg_s = total_ms / launches * 1e-3
     bytes_dom = pass_bytes(dom, n_mean, nf, ns, c, t)
     achieved = bytes_dom / avg_s / 1e9
+    traffic = measured_traffic(scene.name, dom)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "bytes_per_launch": int(bytes_dom), "avg_launch_us": round(avg_s * 1e6, 3)}
+    if traffic:
+        roofline["traffic_gbs"] = round(traffic / avg_s / 1e9, 1)
     inf_end = solver.info()
     b_sub = model_bytes_per_substep(n_mean, nf, ns, c, t, iters)
     model_frac = b_sub * (value / n_mean) / (HBM_PEAK_GBS * 1e9)
