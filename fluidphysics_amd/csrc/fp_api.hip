@@ -419,8 +419,8 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
       (rc = dalloc(s, &s->svtmp, N)) || (rc = dalloc(s, &s->sphash, N)) || (rc = dalloc(s, &s->sP, N)) ||
       (rc = dalloc(s, &s->tpos, N)) || (rc = dalloc(s, &s->tkey, N)) || (rc = dalloc(s, &s->tslot, N)) ||
       (rc = dalloc(s, &s->fsid, N)) || (rc = dalloc(s, &s->fslot, N)) || (rc = dalloc(s, &s->sphash2, N)) ||
-      (rc = dalloc(s, &s->sbox, 6)) || (rc = dalloc(s, &s->nbr, (size_t)(s->pair_slots + 1) * N)) ||
-      (rc = dalloc(s, &s->ncount, N)) ||
+      (rc = dalloc(s, &s->sbox, 6)) || (rc = dalloc(s, &s->nbr, (size_t)(s->pair_slots + FLUID_LANES) * N)) ||
+      (rc = dalloc(s, &s->ncount, (size_t)N * FLUID_LANES)) ||
       (rc = dalloc(s, &s->pslot, N)) || (rc = dalloc(s, &s->count, T)) ||
       (rc = dalloc(s, &s->cell_start, (size_t)T + 1)) || (rc = dalloc(s, &s->entries_tmp, N)) ||
       (rc = dalloc(s, &s->cell_entries, N)) || (rc = dalloc(s, &s->block_sums, s->scan_blocks)) ||

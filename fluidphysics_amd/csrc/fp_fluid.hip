@@ -40,6 +40,7 @@
 // double, from the float32 positions, whenever the float32 d^2 lies within
 // 1e-5 of H^2 -- the reference's own test (XSPH counts every pair, so the
 // test's outcome matters even where q ~ 0).
+#include <algorithm>
 #include <climits>
 
 #include "fp_internal.h"
@@ -51,8 +52,8 @@
 #ifndef FLUID_XCD
 #define FLUID_XCD 0      // 1: give each XCD a contiguous range of slots
 #endif
-#ifndef NB_ROWMAJOR
-#define NB_ROWMAJOR 0    // 1: a particle's list entries are contiguous
+#ifndef DENSITY_BATCH
+#define DENSITY_BATCH 2  // candidates per step of the density loop
 #endif
 #define CONTACT_LANES 4  // lanes per solid in the contact pass
 #define MAXB 32          // bonds per solid held in registers
@@ -264,16 +265,32 @@ __global__ void __launch_bounds__(BLOCK) k_contacts_fluid(FluidParams fp, Contac
   tpos[e] = make_float4(ox, oy, oz, __uint_as_float((uint32_t)x | (fl << PF_SHIFT)));
 }
 
-// Fluid slot of this thread.  Workgroup b runs on XCD b % 8; with FLUID_XCD
-// the blocks are renumbered so that each XCD (and its L2) works through one
-// contiguous eighth of the cell-sorted slots.  The grid is a multiple of 8.
-__device__ __forceinline__ int fluid_slot() {
+// Lane index of this thread over the fluid order: FLUID_LANES consecutive
+// lanes share one fluid slot (slot = lane / FLUID_LANES).  Workgroup b runs
+// on XCD b % 8; with FLUID_XCD the blocks are renumbered so that each XCD
+// (and its L2) works through one contiguous eighth of the cell-sorted slots.
+// The grid is a multiple of 8.
+__device__ __forceinline__ int fluid_lane() {
   int b = blockIdx.x;
   if (FLUID_XCD) b = (b & 7) * (int)(gridDim.x >> 3) + (b >> 3);
   return b * FLUID_BLOCK + (int)threadIdx.x;
 }
 
-static inline int fluid_blocks(int n) { return ((n + FLUID_BLOCK - 1) / FLUID_BLOCK + 7) & ~7; }
+static inline int fluid_blocks(int n) {
+  return ((int)(((int64_t)n * FLUID_LANES + FLUID_BLOCK - 1) / FLUID_BLOCK) + 7) & ~7;
+}
+
+// Sum over the FLUID_LANES lanes of one slot (pairwise, the same in every lane).
+__device__ __forceinline__ float lanes_sum(float v) {
+#pragma unroll
+  for (int off = 1; off < FLUID_LANES; off <<= 1) v += __shfl_xor(v, off, FLUID_LANES);
+  return v;
+}
+__device__ __forceinline__ int lanes_sum(int v) {
+#pragma unroll
+  for (int off = 1; off < FLUID_LANES; off <<= 1) v += __shfl_xor(v, off, FLUID_LANES);
+  return v;
+}
 
 // ---- the reference's fluid pairs of the particle in slot e ------------------
 
@@ -285,6 +302,7 @@ struct PairSource {
 
 struct PairSelf {
   int e, x;
+  int v, h;  // lane (over the fluid order) and its index among the slot's lanes
   bool moved, dup;
   Cell c;
   uint32_t gh;
@@ -422,11 +440,11 @@ __device__ __forceinline__ void density_take(const FluidParams& fp, const PairLi
 __device__ __forceinline__ void density_batch(const FluidParams& fp, const PairSource& src, const PairList& pl,
                                               const PairSelf& me, bool me_flagged, int q, int last, const float4* c,
                                               DensityGather& g) {
-  float d2[4];
-  int w[4];
+  float d2[DENSITY_BATCH];
+  int w[DENSITY_BATCH];
   bool rare = false;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
+  for (int i = 0; i < DENSITY_BATCH; i++) {
     const float dx = me.p.x - c[i].x, dy = me.p.y - c[i].y, dz = me.p.z - c[i].z;
     d2[i] = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const bool cand = (q + i <= last) & (q + i != me.e);
@@ -438,16 +456,16 @@ __device__ __forceinline__ void density_batch(const FluidParams& fp, const PairS
   }
   if (__builtin_expect(rare, 0)) {
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < DENSITY_BATCH; i++)
       if ((q + i <= last) & (q + i != me.e) & (d2[i] < fp.H2_hi)) w[i] = slow_weight(fp, src, me, q + i, c[i], d2[i]);
   }
 #pragma unroll
-  for (int i = 0; i < 4; i++) density_take(fp, pl, q + i, w[i], d2[i], g);
+  for (int i = 0; i < DENSITY_BATCH; i++) density_take(fp, pl, q + i, w[i], d2[i], g);
 }
 
-__device__ __forceinline__ void load4(const float4* __restrict__ row, int q, int last, float4* b) {
+__device__ __forceinline__ void load_batch(const float4* __restrict__ row, int q, int last, float4* b) {
 #pragma unroll
-  for (int i = 0; i < 4; i++) b[i] = row[min(q + i, last)];
+  for (int i = 0; i < DENSITY_BATCH; i++) b[i] = row[min(q + i, last)];
 }
 
 __device__ __forceinline__ void density_gather(const FluidParams& fp, const PairSource& src, const PairList& pl,
@@ -466,19 +484,21 @@ __device__ __forceinline__ void density_gather(const FluidParams& fp, const Pair
     if (r < 8) nx = row_range(fp, src.lstart, kx, r + 1, fx0, fx1, fy0, fy1, fz0, fz1);
     if (rr.r1 > rr.r0) {
       const int last = rr.r1 - 1;
-      // two buffers in turn: the next batch's loads are in flight while the
-      // current batch is evaluated
-      float4 ba[4], bb[4];
-      load4(row, rr.r0, last, ba);
-      for (int q = rr.r0; q <= last; q += 8) {
-        if (q + 4 <= last) load4(row, q + 4, last, bb);
-        density_batch(fp, src, pl, me, me_flagged, q, last, ba, g);
-        if (q + 4 > last) break;
-        if (q + 8 <= last) load4(row, q + 8, last, ba);
-        density_batch(fp, src, pl, me, me_flagged, q + 4, last, bb, g);
+      // this lane's batches of the row: h, h + LANES, ...; the next batch's
+      // loads are in flight while the current one is evaluated
+      constexpr int step = DENSITY_BATCH * FLUID_LANES;
+      int q = rr.r0 + DENSITY_BATCH * me.h;
+      float4 b[DENSITY_BATCH];
+      if (q <= last) load_batch(row, q, last, b);
+      for (; q <= last; q += step) {
+        float4 c[DENSITY_BATCH];
+#pragma unroll
+        for (int i = 0; i < DENSITY_BATCH; i++) c[i] = b[i];
+        if (q + step <= last) load_batch(row, q + step, last, b);
+        density_batch(fp, src, pl, me, me_flagged, q, last, c, g);
       }
     }
-    for (int q = 0; q < rr.w1; q++) {
+    for (int q = 0; q < (me.h == 0 ? rr.w1 : 0); q++) {
       float dx, dy, dz, d2;
       const int w = pair_weight(fp, src, me, q, row[q], dx, dy, dz, d2);
       density_take(fp, pl, q, w, d2, g);
@@ -498,13 +518,18 @@ __device__ __forceinline__ void visit_rows_simple(const FluidParams& fp, const P
 #pragma unroll 1
   for (int r = 0; r < 9; r++) {
     const RowRange rr = row_range(fp, src.lstart, kx, r, fx0, fx1, fy0, fy1, fz0, fz1);
-    for (int pass = 0; pass < 2; pass++) {
-      const int qa = pass ? 0 : rr.r0, qb = pass ? rr.w1 : rr.r1;
-      for (int q = qa; q < qb; q++) {
+    // the same batches as density_gather gives this lane
+    for (int q0 = rr.r0 + DENSITY_BATCH * me.h; q0 < rr.r1; q0 += DENSITY_BATCH * FLUID_LANES) {
+      for (int q = q0; q < min(q0 + DENSITY_BATCH, rr.r1); q++) {
         float dx, dy, dz, d2;
         const int w = pair_weight(fp, src, me, q, src.spos2[q], dx, dy, dz, d2);
         if (w) f.pair(q, w, dx, dy, dz, d2);
       }
+    }
+    for (int q = 0; q < (me.h == 0 ? rr.w1 : 0); q++) {
+      float dx, dy, dz, d2;
+      const int w = pair_weight(fp, src, me, q, src.spos2[q], dx, dy, dz, d2);
+      if (w) f.pair(q, w, dx, dy, dz, d2);
     }
   }
 }
@@ -516,8 +541,8 @@ __device__ __forceinline__ void visit_rows_simple(const FluidParams& fp, const P
 template <class F>
 __device__ __forceinline__ void replay_pairs(const PairSource& src, const PairList& pl, const PairSelf& me, int cnt,
                                              F& f) {
-  const uint32_t* __restrict__ row = pl.nbr + (NB_ROWMAJOR ? (size_t)me.e * (pl.cap + 1) : (size_t)me.e);
-  const size_t st = NB_ROWMAJOR ? 1 : (size_t)pl.stride;
+  const uint32_t* __restrict__ row = pl.nbr + me.v;
+  const size_t st = (size_t)pl.stride;
   const int last = cnt - 1;
   uint32_t v[4];
 #pragma unroll
@@ -561,26 +586,42 @@ __device__ __forceinline__ void replay_pairs(const PairSource& src, const PairLi
 template <class F>
 __device__ __forceinline__ void recorded_pairs(const FluidParams& fp, const PairSource& src, const PairList& pl,
                                                const PairSelf& me, F& f) {
-  const int cnt = pl.ncount[me.e];
-  if (__builtin_expect(cnt <= pl.cap, 1)) {
+  const int cnt = pl.ncount[me.v];
+  // a slot whose lanes did not all fit is gathered again by all its lanes
+  if (__builtin_expect(lanes_sum(cnt > pl.cap ? 1 : 0) == 0, 1)) {
     replay_pairs(src, pl, me, cnt, f);
   } else {
     visit_rows_simple(fp, src, me, f);
   }
 }
 
-// Who is in fluid slot e (false past the last one).
-__device__ __forceinline__ bool pair_self(const FluidParams& fp, const PairSource& src, int e, PairSelf& me) {
-  me.e = e;
-  if (e >= src.lstart[fp.cg.M]) return false;
-  me.p = src.spos2[e];
+// Who is in the fluid slot of lane v (false past the last slot).
+__device__ __forceinline__ bool pair_self(const FluidParams& fp, const PairSource& src, int v, PairSelf& me) {
+  me.v = v;
+  me.h = v % FLUID_LANES;
+  me.e = v / FLUID_LANES;
+  if (me.e >= src.lstart[fp.cg.M]) return false;
+  me.p = src.spos2[me.e];
   const uint32_t b = fbits(me.p.w);
   me.x = (int)(b & SID_MASK);
   me.moved = (b >> PF_SHIFT) & PF_MOVED;
   me.dup = (b >> PF_SHIFT) & PF_DUP;
   me.c = cell_of(me.p.x, me.p.y, me.p.z, fp.cg.inv_cell);
-  me.gh = src.sphash[e];
+  me.gh = src.sphash[me.e];
   return true;
+}
+
+// The outputs of a gathered slot (every lane of the slot calls this).
+__device__ __forceinline__ void density_store(const FluidParams& fp, const PairList& pl, const PairSelf& me,
+                                              const DensityGather& g, float2* __restrict__ sP,
+                                              float2* __restrict__ rho_out, int* __restrict__ dflags) {
+  const bool over = lanes_sum(g.cnt > pl.cap ? 1 : 0) != 0;
+  pl.ncount[me.v] = g.cnt > pl.cap ? pl.cap + 1 : g.cnt;
+  const float rho = lanes_sum(g.rho), rhon = lanes_sum(g.rhon);
+  if (me.h != 0) return;
+  if (over) atomicAdd(&dflags[DF_PAIR_OVF], 1);
+  rho_out[me.x] = make_float2(rho, rhon);
+  sP[me.e] = make_float2(fp.stiff * (rho - fp.rest_density), fp.near_stiff * rhon);
 }
 
 __global__ void __launch_bounds__(FLUID_BLOCK) k_density(FluidParams fp, PairSource src, PairList pl,
@@ -589,18 +630,11 @@ __global__ void __launch_bounds__(FLUID_BLOCK) k_density(FluidParams fp, PairSou
   // the solids' bounding box is rebuilt by the next integrate
   if (blockIdx.x == 0 && threadIdx.x < 6) sbox[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : INT_MIN;
   PairSelf me;
-  if (!pair_self(fp, src, fluid_slot(), me)) return;
+  if (!pair_self(fp, src, fluid_lane(), me)) return;
   DensityGather g;
-  g.wp = pl.nbr + (NB_ROWMAJOR ? (size_t)me.e * (pl.cap + 1) : (size_t)me.e);
+  g.wp = pl.nbr + me.v;
   density_gather(fp, src, pl, me, g);
-  if (g.cnt > pl.cap) {
-    pl.ncount[me.e] = pl.cap + 1;
-    atomicAdd(&dflags[DF_PAIR_OVF], 1);
-  } else {
-    pl.ncount[me.e] = g.cnt;
-  }
-  rho_out[me.x] = make_float2(g.rho, g.rhon);
-  sP[me.e] = make_float2(fp.stiff * (g.rho - fp.rest_density), fp.near_stiff * g.rhon);
+  density_store(fp, pl, me, g, sP, rho_out, dflags);
 }
 
 struct DisplaceAcc {
@@ -632,14 +666,15 @@ struct DisplaceAcc {
 __global__ void __launch_bounds__(FLUID_BLOCK) k_displace(FluidParams fp, PairSource src, PairList pl,
                                                     const float2* __restrict__ sP, float4* __restrict__ pos) {
   PairSelf me;
-  if (!pair_self(fp, src, fluid_slot(), me)) return;
+  if (!pair_self(fp, src, fluid_lane(), me)) return;
   DisplaceAcc acc;
   acc.sP = sP;
   acc.Px = sP[me.e];
   acc.inv_H = fp.inv_H;
   acc.max_pair = fp.max_pair;
   recorded_pairs(fp, src, pl, me, acc);
-  float sx = acc.sx, sy = acc.sy, sz = acc.sz;
+  float sx = lanes_sum(acc.sx), sy = lanes_sum(acc.sy), sz = lanes_sum(acc.sz);
+  if (me.h != 0) return;
   const float s2 = sx * sx + sy * sy + sz * sz;
   if (s2 > fp.max_total * fp.max_total) {
     const float g = fp.max_total / sqrtf(s2);
@@ -676,19 +711,22 @@ __global__ void __launch_bounds__(FLUID_BLOCK) k_xsph(FluidParams fp, PairSource
                                                 const float4* __restrict__ svtmp, float4* __restrict__ vel,
                                                 const uint8_t* __restrict__ active) {
   PairSelf me;
-  if (!pair_self(fp, src, fluid_slot(), me)) return;
+  if (!pair_self(fp, src, fluid_lane(), me)) return;
   if (!active[me.x]) return;
-  const float4 v = svtmp[me.e];
   XsphAcc acc;
   acc.svtmp = svtmp;
   recorded_pairs(fp, src, pl, me, acc);
-  if (acc.m == 0) {
+  const int m = lanes_sum(acc.m);
+  const float sx = lanes_sum(acc.sx), sy = lanes_sum(acc.sy), sz = lanes_sum(acc.sz);
+  if (me.h != 0) return;
+  const float4 v = svtmp[me.e];
+  if (m == 0) {
     vel[me.x] = v;
     return;
   }
-  const float inv_m = 1.f / (float)acc.m;
-  vel[me.x] = make_float4(v.x + fp.visc * (acc.sx * inv_m - v.x), v.y + fp.visc * (acc.sy * inv_m - v.y),
-                          v.z + fp.visc * (acc.sz * inv_m - v.z), 0.f);
+  const float inv_m = 1.f / (float)m;
+  vel[me.x] = make_float4(v.x + fp.visc * (sx * inv_m - v.x), v.y + fp.visc * (sy * inv_m - v.y),
+                          v.z + fp.visc * (sz * inv_m - v.z), 0.f);
 }
 
 // ---- launchers ------------------------------------------------------------------
@@ -730,8 +768,8 @@ static PairList pair_list(const fp_solver* s) {
   PairList pl;
   pl.nbr = s->nbr;
   pl.ncount = s->ncount;
-  pl.stride = s->n;
-  pl.cap = s->pair_slots;
+  pl.stride = s->n * FLUID_LANES;
+  pl.cap = s->pair_slots / FLUID_LANES;
   return pl;
 }
 
@@ -751,8 +789,8 @@ int fp_launch_contacts(fp_solver* s) {
 int fp_launch_density(fp_solver* s) {
   const FluidParams fp = fp_fluid_params(s);
   PassScope ps(s, P_DENSITY);
-  hipLaunchKernelGGL(k_density, dim3(fluid_blocks(s->n)), dim3(FLUID_BLOCK), 0, s->stream, fp, pair_source(s), pair_list(s),
-                     s->sP, s->rho, s->sbox, s->dflags);
+  hipLaunchKernelGGL(k_density, dim3(fluid_blocks(s->n)), dim3(FLUID_BLOCK), 0, s->stream, fp, pair_source(s),
+                     pair_list(s), s->sP, s->rho, s->sbox, s->dflags);
   FP_TRY(s, hipGetLastError());
   return FP_OK;
 }

@@ -58,9 +58,12 @@ __global__ void __launch_bounds__(BLOCK) k_scan_reduce(int t, const int* __restr
 
 // Exclusive scan of one tile plus the carry of the tiles before it; writes
 // start[k] and zeroes count[k]; the tile holding the last entry writes
-// start[t] = total.
+// start[t] = total.  zero_word (may be null) is cleared for the passes that
+// follow (a counter, cleared by a kernel so the frame can be graph-captured).
 __global__ void __launch_bounds__(BLOCK) k_scan_apply(int t, int* __restrict__ count,
-                                                      const int* __restrict__ block_sums, int* __restrict__ start) {
+                                                      const int* __restrict__ block_sums, int* __restrict__ start,
+                                                      int* __restrict__ zero_word) {
+  if (zero_word && blockIdx.x == 0 && threadIdx.x == 0) *zero_word = 0;
   __shared__ int wsum[BLOCK / 64];
   __shared__ int carry_s;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -204,7 +207,7 @@ int fp_launch_cell_grid(fp_solver* s, bool count_done) {
   {
     PassScope ps(s, P_SCAN_APPLY);
     hipLaunchKernelGGL(k_scan_apply, dim3(s->lscan_blocks), dim3(BLOCK), 0, s->stream, M, s->lcount,
-                       s->lblock_sums, s->lstart);
+                       s->lblock_sums, s->lstart, nullptr);
   }
   {
     PassScope ps(s, P_SCATTER);
@@ -268,7 +271,7 @@ int fp_launch_resort(fp_solver* s) {
   PassScope ps(s, P_RESORT);
   hipLaunchKernelGGL(k_scan_reduce, dim3(s->lscan_blocks), dim3(BLOCK), 0, s->stream, M, s->lcount, s->lblock_sums);
   hipLaunchKernelGGL(k_scan_apply, dim3(s->lscan_blocks), dim3(BLOCK), 0, s->stream, M, s->lcount, s->lblock_sums,
-                     s->lstart);
+                     s->lstart, nullptr);
   hipLaunchKernelGGL(k_fluid_scatter, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->sid, s->tkey, s->tslot,
                      s->lstart, s->ltmp);
   hipLaunchKernelGGL(k_fluid_rank, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->sid, s->tkey, s->lstart,
@@ -335,7 +338,7 @@ int fp_launch_ref_grid(fp_solver* s, bool rehash) {
   hipLaunchKernelGGL(k_ref_count, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->phash, s->count, s->pslot);
   hipLaunchKernelGGL(k_scan_reduce, dim3(s->scan_blocks), dim3(BLOCK), 0, s->stream, s->T, s->count, s->block_sums);
   hipLaunchKernelGGL(k_scan_apply, dim3(s->scan_blocks), dim3(BLOCK), 0, s->stream, s->T, s->count, s->block_sums,
-                     s->cell_start);
+                     s->cell_start, nullptr);
   hipLaunchKernelGGL(k_ref_scatter, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->phash, s->pslot,
                      s->cell_start, s->entries_tmp);
   hipLaunchKernelGGL(k_ref_rank, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->phash, s->cell_start,

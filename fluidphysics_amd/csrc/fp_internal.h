@@ -18,7 +18,8 @@
 //   density, displacement, XSPH): fsid, fslot i32[N] (inverse), spos2
 //     float4 (= tpos), sphash2 u32, sP float2 (P, Pnear), svtmp float4
 //     (velocity before XSPH)
-//   nbr u32[(K+1)*N] (entry k of fluid slot e at k*N + e; row K is a sink), ncount i32[N]: the
+//   nbr u32[(K+L)*N] (L = FLUID_LANES lanes per fluid slot, each with K/L
+//     entries: entry k of lane v at k*N*L + v; row K/L is a sink), ncount i32[N*L]: the
 //     pairs the density pass found, replayed by displacement and XSPH
 //   count/cell_start/entries_tmp/cell_entries: the reference's hash grid,
 //     rebuilt from phash only when read
@@ -44,6 +45,9 @@
 #define SID_MASK 0x3fffffffu
 #define PF_SHIFT 30
 #define NO_KEY 0xffffffffu
+#ifndef FLUID_LANES
+#define FLUID_LANES 2  // lanes per fluid particle in density, displacement and XSPH
+#endif
 
 // device flag words (dflags)
 #define DF_TEAR 0        // a constraint tore this step
