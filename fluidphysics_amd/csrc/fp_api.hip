@@ -69,6 +69,7 @@ void free_all(fp_solver* s) {
   dfree(s->sid), dfree(s->spos), dfree(s->spos2), dfree(s->svtmp), dfree(s->sphash), dfree(s->sP);
   dfree(s->tpos), dfree(s->tkey), dfree(s->tslot), dfree(s->fsid), dfree(s->fslot), dfree(s->sphash2);
   dfree(s->sbox);
+  dfree(s->lnsol);
   dfree(s->nbr), dfree(s->ncount);
   dfree(s->pslot), dfree(s->count), dfree(s->cell_start), dfree(s->entries_tmp), dfree(s->cell_entries);
   dfree(s->block_sums);
@@ -412,7 +413,7 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
   const size_t M = s->cg.M;
   if ((rc = dalloc(s, &s->pos, N)) || (rc = dalloc(s, &s->prev, N)) || (rc = dalloc(s, &s->vel, N)) ||
       (rc = dalloc(s, &s->active, N)) || (rc = dalloc(s, &s->rho, N)) || (rc = dalloc(s, &s->phash, N)) ||
-      (rc = dalloc(s, &s->lcount, M)) || (rc = dalloc(s, &s->lstart, M + 1)) ||
+      (rc = dalloc(s, &s->lcount, M)) || (rc = dalloc(s, &s->lstart, M + 1)) || (rc = dalloc(s, &s->lnsol, M)) ||
       (rc = dalloc(s, &s->lblock_sums, s->lscan_blocks)) || (rc = dalloc(s, &s->ltmp, N)) ||
       (rc = dalloc(s, &s->lkey, N)) || (rc = dalloc(s, &s->lslot, N)) || (rc = dalloc(s, &s->sid, N)) ||
       (rc = dalloc(s, &s->spos, N)) || (rc = dalloc(s, &s->spos2, N)) ||

@@ -96,6 +96,37 @@ __device__ __forceinline__ bool within_h(const FluidParams& fp, const float4& a,
   return dx * dx + dy * dy + dz * dz < fp.H2d;
 }
 
+// Distances from p to the lower and upper faces of its cell c (cell size cs):
+// the floor of the distance to any particle of a neighbouring cell.
+struct FaceDist {
+  float x0, x1, y0, y1, z0, z1;
+};
+
+__device__ __forceinline__ FaceDist face_dist(float px, float py, float pz, const Cell& c, double cs) {
+  FaceDist f;
+  f.x0 = (float)((double)px - c.x * cs);
+  f.x1 = (float)((c.x + 1.0) * cs - (double)px);
+  f.y0 = (float)((double)py - c.y * cs);
+  f.y1 = (float)((c.y + 1.0) * cs - (double)py);
+  f.z0 = (float)((double)pz - c.z * cs);
+  f.z1 = (float)((c.z + 1.0) * cs - (double)pz);
+  return f;
+}
+
+// Cells of row r (a, b) = (r / 3 - 1, r % 3 - 1) that can hold a particle
+// closer than sqrt(r2): z offsets lo .. hi-1 of {-1, 0, +1} as key offsets
+// 0..3 from the row's first key; false when the whole row is farther.  NaN
+// positions keep every cell.
+__device__ __forceinline__ bool row_cells(const FaceDist& f, int r, float r2, uint32_t& lo, uint32_t& hi) {
+  const int a = r / 3 - 1, b = r % 3 - 1;
+  const float ex = a < 0 ? f.x0 : f.x1, ey = b < 0 ? f.y0 : f.y1;
+  const float dxy2 = (a != 0 ? ex * ex : 0.f) + (b != 0 ? ey * ey : 0.f);
+  if (!(dxy2 < r2) && dxy2 == dxy2) return false;
+  lo = (dxy2 + f.z0 * f.z0 < r2) ? 0u : 1u;
+  hi = (dxy2 + f.z1 * f.z1 < r2) ? 3u : 2u;
+  return true;
+}
+
 // ---- contacts -----------------------------------------------------------------
 
 // The alive bonds of solid x, in registers: the other endpoints of x's
@@ -142,6 +173,7 @@ struct ContactCtx {
   const float4* __restrict__ spos;  // grid order: xyz + inverse mass
   const int* __restrict__ sid;
   const int* __restrict__ lstart;   // grid-order cell starts
+  const int* __restrict__ lnsol;    // solids of each non-empty cell (they come first)
   const int* __restrict__ adj_off;
   const int4* __restrict__ adj;
   const uint8_t* __restrict__ alive;
@@ -167,6 +199,27 @@ __device__ __forceinline__ void contact_add(const FluidParams& fp, const float4&
 // Solid x: CONTACT_LANES lanes split its 9 rows (row r to lane r % lanes),
 // each sums its candidates in slot order, then the lanes' sums are added
 // pairwise.  Writes pos[x] = grid-time position + correction.
+__device__ __forceinline__ void solid_candidate(const FluidParams& fp, const ContactCtx& cc, int x, const Bonds& bonds,
+                                                const float4& px, int y, const float4& py, float& ax, float& ay,
+                                                float& az) {
+  if (y == x) return;
+  if (y < fp.ns) {
+    // a bonded pair does not collide; test the bond only when close
+    const float dx = px.x - py.x, dy = px.y - py.y, dz = px.z - py.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float rsum = fp.rs + fp.rs;
+    if (d2 >= rsum * rsum || d2 < 1e-12f) return;
+    if (is_bonded(x, y, bonds, cc.adj_off, cc.adj, cc.alive)) return;
+    contact_add(fp, px, fp.rs, py, fp.rs, ax, ay, az);
+  } else {
+    contact_add(fp, px, fp.rs, py, fp.rf, ax, ay, az);
+  }
+}
+
+// Solid x: CONTACT_LANES lanes split its 9 rows (row r to lane r % lanes),
+// each sums its candidates in slot order (cells beyond the contact range
+// skipped, loads four at a time), then the lanes' sums are added pairwise.
+// Writes pos[x] = grid-time position + correction.
 __global__ void __launch_bounds__(BLOCK) k_contacts_solid(FluidParams fp, ContactCtx cc, const uint8_t* __restrict__ active,
                                                           float4* __restrict__ pos) {
   const int t = blockIdx.x * BLOCK + threadIdx.x;
@@ -178,29 +231,26 @@ __global__ void __launch_bounds__(BLOCK) k_contacts_solid(FluidParams fp, Contac
     px = pos[x];
     Bonds bonds;
     load_bonds(x, cc.adj_off, cc.adj, cc.alive, bonds);
-    const uint32_t k0 = cell_key(fp.cg, cell_of(px.x, px.y, px.z, fp.cg.inv_cell));
+    const Cell c0 = cell_of(px.x, px.y, px.z, fp.cg.inv_cell);
+    const uint32_t k0 = cell_key(fp.cg, c0);
+    const FaceDist fd = face_dist(px.x, px.y, px.z, c0, fp.cs);
     for (int r = lane; r < 9; r += CONTACT_LANES) {
+      uint32_t lo, hi;
+      if (!row_cells(fd, r, fp.contact_r2, lo, hi)) continue;
       int r0, r1, w1;
-      run_ranges(fp.cg, cc.lstart, row_key(fp.cg, k0, r / 3 - 1, r % 3 - 1), r0, r1, w1);
-      for (int pass = 0; pass < 2; pass++) {
-        const int qa = pass ? 0 : r0, qb = pass ? w1 : r1;
-        for (int q = qa; q < qb; q++) {
-          const int y = cc.sid[q];
-          if (y == x) continue;
-          const float4 py = cc.spos[q];
-          if (y < fp.ns) {
-            // a bonded pair does not collide; test the bond only when close
-            const float dx = px.x - py.x, dy = px.y - py.y, dz = px.z - py.z;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            const float rsum = fp.rs + fp.rs;
-            if (d2 >= rsum * rsum || d2 < 1e-12f) continue;
-            if (is_bonded(x, y, bonds, cc.adj_off, cc.adj, cc.alive)) continue;
-            contact_add(fp, px, fp.rs, py, fp.rs, ax, ay, az);
-          } else {
-            contact_add(fp, px, fp.rs, py, fp.rf, ax, ay, az);
-          }
-        }
+      run_ranges_n(fp.cg, cc.lstart, (row_key(fp.cg, k0, r / 3 - 1, r % 3 - 1) + lo) & fp.cg.kmask, hi - lo, r0, r1,
+                   w1);
+      int q = r0;
+      for (; q + 4 <= r1; q += 4) {
+        const int y0 = cc.sid[q], y1 = cc.sid[q + 1], y2 = cc.sid[q + 2], y3 = cc.sid[q + 3];
+        const float4 p0 = cc.spos[q], p1 = cc.spos[q + 1], p2 = cc.spos[q + 2], p3 = cc.spos[q + 3];
+        solid_candidate(fp, cc, x, bonds, px, y0, p0, ax, ay, az);
+        solid_candidate(fp, cc, x, bonds, px, y1, p1, ax, ay, az);
+        solid_candidate(fp, cc, x, bonds, px, y2, p2, ax, ay, az);
+        solid_candidate(fp, cc, x, bonds, px, y3, p3, ax, ay, az);
       }
+      for (; q < r1; q++) solid_candidate(fp, cc, x, bonds, px, cc.sid[q], cc.spos[q], ax, ay, az);
+      for (q = 0; q < w1; q++) solid_candidate(fp, cc, x, bonds, px, cc.sid[q], cc.spos[q], ax, ay, az);
     }
   }
 #pragma unroll
@@ -219,50 +269,65 @@ __global__ void __launch_bounds__(BLOCK) k_contacts_fluid(FluidParams fp, Contac
                                                           uint32_t* __restrict__ tkey, uint32_t* __restrict__ tslot,
                                                           int* __restrict__ lcount) {
   const int e = blockIdx.x * BLOCK + threadIdx.x;
-  if (e >= fp.n) return;
-  if (e >= cc.lstart[fp.cg.M]) {
-    tkey[e] = NO_KEY;
-    return;
-  }
-  const int x = cc.sid[e];
-  if (x < fp.ns) {
-    tkey[e] = NO_KEY;
-    return;
-  }
-  const float4 px = cc.spos[e];
-  const Cell c0 = cell_of(px.x, px.y, px.z, fp.cg.inv_cell);
-  float ax = 0.f, ay = 0.f, az = 0.f;
-  if (fp.ns > 0) {
-    const float g = fp.rs + fp.rf;
-    const bool near = px.x > unorder(cc.sbox[0]) - g && px.x < unorder(cc.sbox[3]) + g &&
-                      px.y > unorder(cc.sbox[1]) - g && px.y < unorder(cc.sbox[4]) + g &&
-                      px.z > unorder(cc.sbox[2]) - g && px.z < unorder(cc.sbox[5]) + g;
-    if (near) {
-      const uint32_t k0 = cell_key(fp.cg, c0);
+  // every lane stays to the bucket vote at the end
+  bool has = false;
+  uint32_t k1 = 0;
+  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (e < fp.n) {
+    const int x = e < cc.lstart[fp.cg.M] ? cc.sid[e] : -1;
+    if (x < fp.ns) {
+      tkey[e] = NO_KEY;
+    } else {
+      const float4 px = cc.spos[e];
+      const Cell c0 = cell_of(px.x, px.y, px.z, fp.cg.inv_cell);
+      float ax = 0.f, ay = 0.f, az = 0.f;
+      if (fp.ns > 0) {
+        const float g = fp.rs + fp.rf;
+        const bool near = px.x > unorder(cc.sbox[0]) - g && px.x < unorder(cc.sbox[3]) + g &&
+                          px.y > unorder(cc.sbox[1]) - g && px.y < unorder(cc.sbox[4]) + g &&
+                          px.z > unorder(cc.sbox[2]) - g && px.z < unorder(cc.sbox[5]) + g;
+        if (near) {
+          const uint32_t k0 = cell_key(fp.cg, c0);
+          const FaceDist fd = face_dist(px.x, px.y, px.z, c0, fp.cs);
 #pragma unroll 1
-      for (int r = 0; r < 9; r++) {
-        const uint32_t kr = row_key(fp.cg, k0, r / 3 - 1, r % 3 - 1);
-        for (uint32_t j = 0; j < 3; j++) {
-          const uint32_t k = (kr + j) & fp.cg.kmask;
-          const int qe = cc.lstart[k + 1];
-          for (int q = cc.lstart[k]; q < qe; q++) {
-            const int y = cc.sid[q];
-            if (y >= fp.ns) break;
-            contact_add(fp, px, fp.rf, cc.spos[q], fp.rs, ax, ay, az);
+          for (int r = 0; r < 9; r++) {
+            uint32_t lo, hi;
+            if (!row_cells(fd, r, fp.contact_r2, lo, hi)) continue;
+            const uint32_t kr = row_key(fp.cg, k0, r / 3 - 1, r % 3 - 1);
+            // the row's cell bounds and solid counts, all loads in flight together
+            int a[3], b[3], m[3];
+#pragma unroll
+            for (uint32_t j = 0; j < 3; j++) {
+              const uint32_t k = (kr + j) & fp.cg.kmask;
+              a[j] = cc.lstart[k];
+              b[j] = cc.lstart[k + 1];
+              m[j] = cc.lnsol[k];
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < 3; j++) {
+              if (j < lo || j >= hi || a[j] == b[j]) continue;
+              for (int q = a[j]; q < a[j] + m[j]; q++) contact_add(fp, px, fp.rf, cc.spos[q], fp.rs, ax, ay, az);
+            }
           }
         }
       }
+      const float ox = px.x + ax, oy = px.y + ay, oz = px.z + az;
+      const Cell c1 = cell_of(ox, oy, oz, fp.cg.inv_cell);
+      uint32_t fl = 0;
+      if (c1.x != c0.x || c1.y != c0.y || c1.z != c0.z) fl |= PF_MOVED;
+      if (probe_any_dup(c1, fp.cg.hmask)) fl |= PF_DUP;
+      k1 = cell_key(fp.cg, c1);
+      out = make_float4(ox, oy, oz, __uint_as_float((uint32_t)x | (fl << PF_SHIFT)));
+      has = true;
     }
   }
-  const float ox = px.x + ax, oy = px.y + ay, oz = px.z + az;
-  const Cell c1 = cell_of(ox, oy, oz, fp.cg.inv_cell);
-  uint32_t fl = 0;
-  if (c1.x != c0.x || c1.y != c0.y || c1.z != c0.z) fl |= PF_MOVED;
-  if (probe_any_dup(c1, fp.cg.hmask)) fl |= PF_DUP;
-  const uint32_t k1 = cell_key(fp.cg, c1);
-  tkey[e] = k1;
-  tslot[e] = (uint32_t)atomicAdd(&lcount[k1], 1);
-  tpos[e] = make_float4(ox, oy, oz, __uint_as_float((uint32_t)x | (fl << PF_SHIFT)));
+  // consecutive grid slots mostly share a cell: one atomic per run
+  const uint32_t slot = wave_bucket_slot(lcount, k1, has);
+  if (has) {
+    tkey[e] = k1;
+    tslot[e] = slot;
+    tpos[e] = out;
+  }
 }
 
 // Lane index of this thread over the fluid order: FLUID_LANES consecutive
@@ -746,6 +811,10 @@ FluidParams fp_fluid_params(const fp_solver* s) {
   fp.H2_hi = (float)(fp.H2d * (1.0 + 1e-5));
   fp.cs = P.cell_size;
   fp.cull_r2 = (float)(fp.H2d * (1.0 + 1e-4));
+  {
+    const double rc = P.solid_radius + std::max(P.solid_radius, P.fluid_radius);
+    fp.contact_r2 = (float)(rc * rc * (1.0 + 1e-4));
+  }
   fp.max_pair = (float)(0.05 * P.fluid_h);
   fp.max_total = (float)(0.35 * P.fluid_h);
   fp.stiff = (float)P.stiffness;
@@ -775,7 +844,7 @@ static PairList pair_list(const fp_solver* s) {
 
 int fp_launch_contacts(fp_solver* s) {
   const FluidParams fp = fp_fluid_params(s);
-  ContactCtx cc{s->spos, s->sid, s->lstart, s->adj_off, s->adj, s->alive[s->cur], s->sbox};
+  ContactCtx cc{s->spos, s->sid, s->lstart, s->lnsol, s->adj_off, s->adj, s->alive[s->cur], s->sbox};
   PassScope ps(s, P_CONTACTS);
   if (s->ns > 0)
     hipLaunchKernelGGL(k_contacts_solid, dim3(fp_blocks(s->ns * CONTACT_LANES)), dim3(BLOCK), 0, s->stream, fp, cc,

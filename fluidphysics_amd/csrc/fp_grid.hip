@@ -149,16 +149,18 @@ __global__ void __launch_bounds__(BLOCK) k_cell_count(CellGrid cg, int n, int ns
   const bool in = i < n;
   const bool act = in && active[i];
   float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t k = 0;
   if (act) {
     p = pos[i];
     const Cell c = cell_of(p.x, p.y, p.z, cg.inv_cell);
     phash[i] = cell_hash(c, cg.hmask);
-    const uint32_t k = cell_key(cg, c);
+    k = cell_key(cg, c);
     lkey[i] = k;
-    lslot[i] = (uint32_t)atomicAdd(&lcount[k], 1);
   } else if (in) {
     phash[i] = 0xffffffffu;
   }
+  const uint32_t sl = wave_bucket_slot(lcount, k, act);
+  if (act) lslot[i] = sl;
   box_update(act && i < ns, p.x, p.y, p.z, sbox);
 }
 
@@ -171,24 +173,32 @@ __global__ void __launch_bounds__(BLOCK) k_cell_scatter(int n, const uint32_t* _
   tmp[lstart[lkey[i]] + (int)lslot[i]] = i;
 }
 
-// Stable order inside a cell (ascending particle index) and the sorted copies.
-__global__ void __launch_bounds__(BLOCK) k_cell_rank(int n, const uint32_t* __restrict__ phash,
+// Stable order inside a cell (ascending particle index) and the sorted
+// copies.  The cell's first particle also records how many of its particles
+// are solids (they come first): lnsol is read only for non-empty cells.
+__global__ void __launch_bounds__(BLOCK) k_cell_rank(int n, int ns, const uint32_t* __restrict__ phash,
                                                      const uint32_t* __restrict__ lkey,
                                                      const int* __restrict__ lstart, const int* __restrict__ tmp,
                                                      const float4* __restrict__ pos, int* __restrict__ sid,
-                                                     float4* __restrict__ spos, uint32_t* __restrict__ sphash) {
+                                                     float4* __restrict__ spos, uint32_t* __restrict__ sphash,
+                                                     int* __restrict__ lnsol) {
   const int i = blockIdx.x * BLOCK + threadIdx.x;
   if (i >= n) return;
   const uint32_t h = phash[i];
   if (h == 0xffffffffu) return;
   const uint32_t k = lkey[i];
   const int s = lstart[k], e = lstart[k + 1];
-  int r = 0;
-  for (int q = s; q < e; q++) r += tmp[q] < i ? 1 : 0;
+  int r = 0, nsol = 0;
+  for (int q = s; q < e; q++) {
+    const int j = tmp[q];
+    r += j < i ? 1 : 0;
+    nsol += j < ns ? 1 : 0;
+  }
   const int d = s + r;
   sid[d] = i;
   spos[d] = pos[i];
   sphash[d] = h;
+  if (r == 0) lnsol[k] = nsol;
 }
 
 int fp_launch_cell_grid(fp_solver* s, bool count_done) {
@@ -216,8 +226,8 @@ int fp_launch_cell_grid(fp_solver* s, bool count_done) {
   }
   {
     PassScope ps(s, P_RANK);
-    hipLaunchKernelGGL(k_cell_rank, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->phash, s->lkey, s->lstart,
-                       s->ltmp, s->pos, s->sid, s->spos, s->sphash);
+    hipLaunchKernelGGL(k_cell_rank, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->ns, s->phash, s->lkey,
+                       s->lstart, s->ltmp, s->pos, s->sid, s->spos, s->sphash, s->lnsol);
   }
   FP_TRY(s, hipGetLastError());
   s->ref_grid_valid = false;

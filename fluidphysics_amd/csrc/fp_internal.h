@@ -7,7 +7,8 @@
 // table size):
 //   pos, prev, vel  float4[N] particle-index order (pos.w = inverse mass)
 //   active u8[N]; rho float2[N]; phash u32[N] reference bucket at grid time
-//   lcount i32[M] (zero between builds), lstart i32[M+1], ltmp/lkey/lslot [N]
+//   lcount i32[M] (zero between builds), lstart i32[M+1], ltmp/lkey/lslot [N],
+//   lnsol i32[M] (solids of each non-empty cell of the grid order)
 //   grid order (every active particle, sorted by its cell after integrate;
 //   read by contacts): sid i32 (particle of the slot), spos float4 (xyz +
 //     inverse mass), sphash u32 (bucket)
@@ -97,6 +98,7 @@ struct fp_solver {
   CellGrid cg;
   int lscan_blocks = 0;
   int *lcount = nullptr, *lstart = nullptr, *lblock_sums = nullptr, *ltmp = nullptr;
+  int* lnsol = nullptr;  // solids per non-empty cell of the grid order
   uint32_t *lkey = nullptr, *lslot = nullptr;
   int* sid = nullptr;
   float4* spos = nullptr;
@@ -163,6 +165,7 @@ struct FluidParams {
   double H2d;
   double cs;       // cell size
   float cull_r2;   // a cell whose box lies at least this far (squared) holds no pair
+  float contact_r2;  // the same for contacts (rs + max(rs, rf), squared, with margin)
   float stiff, near_stiff, rest_density;
   float contact_scale, visc;
 };

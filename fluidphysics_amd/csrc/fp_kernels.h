@@ -209,3 +209,25 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 }
 
 __device__ __forceinline__ int wave_sum(int v) { return __builtin_amdgcn_readlane(wave_incl_scan(v), 63); }
+
+// A slot in bucket `key` of a counting sort: atomicAdd(&count[key], 1), but
+// one atomic per run of equal keys in consecutive lanes (lanes with
+// has == false take part in the vote and get nothing).  Every lane of the
+// wave must call it.  Slots inside a bucket come out in no fixed order.
+__device__ __forceinline__ uint32_t wave_bucket_slot(int* __restrict__ count, uint32_t key, bool has) {
+  const int lane = (int)(threadIdx.x & 63);
+  const uint32_t kprev = (uint32_t)__shfl_up((int)key, 1);
+  const bool hprev = __shfl_up(has ? 1 : 0, 1) != 0;
+  const bool head = has && (lane == 0 || !hprev || kprev != key);
+  // a run ends before the next head or the next lane without a key
+  const uint64_t heads = __ballot(head);
+  const uint64_t stops = heads | ~__ballot(has);
+  const uint64_t above = lane == 63 ? 0ull : (stops & (~0ull << (lane + 1)));
+  const int next = above ? (int)__builtin_ctzll(above) : 64;
+  int base = 0;
+  if (head) base = atomicAdd(&count[key], next - lane);
+  const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+  const int hl = upto ? 63 - (int)__builtin_clzll(upto) : 0;
+  const int b = __shfl(base, hl);
+  return (uint32_t)(b + (lane - hl));
+}

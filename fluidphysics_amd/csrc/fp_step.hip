@@ -33,6 +33,7 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(CellGrid cg, int n, int ns,
   const bool in = i < n;
   const bool act = in && active[i];
   float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t k = 0;
   if (act) {
     float4 v = vel[i];
     p = pos[i];
@@ -46,15 +47,19 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(CellGrid cg, int n, int ns,
     if (COUNT) {
       const Cell c = cell_of(p.x, p.y, p.z, cg.inv_cell);
       phash[i] = cell_hash(c, cg.hmask);
-      const uint32_t k = cell_key(cg, c);
+      k = cell_key(cg, c);
       lkey[i] = k;
-      lslot[i] = (uint32_t)atomicAdd(&lcount[k], 1);
     }
   } else if (in) {
     if (COUNT) phash[i] = 0xffffffffu;
     rho[i] = make_float2(0.f, 0.f);
   }
-  if (COUNT) box_update(act && i < ns, p.x, p.y, p.z, sbox);
+  if (COUNT) {
+    // neighbouring indices are often in one cell: one atomic per run
+    const uint32_t sl = wave_bucket_slot(lcount, k, act);
+    if (act) lslot[i] = sl;
+    box_update(act && i < ns, p.x, p.y, p.z, sbox);
+  }
 }
 
 // One round of grab pulls; indices inside a round are distinct.
