@@ -55,6 +55,8 @@ _FIELDS = {
     "grabStiffness": "grab_stiffness",
     "cellSize": "cell_size",
     "constraintIters": "constraint_iters",
+    "gravityX": "gravity_x",
+    "gravityZ": "gravity_z",
 }
 
 
@@ -356,6 +358,75 @@ class Solver:
         return {buf[k].name.decode(): (buf[k].launches, buf[k].total_ms) for k in range(n.value)}
 
 
+class Batch:
+    """Many independent scenes stepped together (fp_batch_*): one kernel
+    launch per pass over every scene, a frame captured once into a HIP graph
+    and replayed.  Scenes keep their own Solver API between frames; all must
+    use the same constraintIters."""
+
+    def __init__(self, device=-1):
+        self._lib = _abi.load_library()
+        h = ctypes.c_void_p()
+        self._h = None
+        self._check(self._lib.fp_batch_create(int(device), ctypes.byref(h)))
+        self._h = h
+        self.scenes = []
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self._lib.fp_batch_last_error(self._h)
+            raise FpError("fluidphysics batch error %d: %s" % (rc, msg.decode() if msg else ""))
+
+    def add(self, solver):
+        solver._flush()
+        self._check(self._lib.fp_batch_add(self._h, solver._h))
+        self.scenes.append(solver)
+
+    def remove(self, solver):
+        self._check(self._lib.fp_batch_remove(self._h, solver._h))
+        self.scenes.remove(solver)
+
+    def step(self, dt, substeps=5, wait=True):
+        """One frame of every scene (src/solver.js:187-242 each)."""
+        for s in self.scenes:
+            s._flush()
+            s._push_grabs()
+        fn = self._lib.fp_batch_step if wait else self._lib.fp_batch_step_async
+        self._check(fn(self._h, float(dt), int(substeps)))
+        for s in self.scenes:
+            s._invalidate()
+
+    def sync(self):
+        self._check(self._lib.fp_batch_sync(self._h))
+
+    def graph(self, enable=True):
+        self._check(self._lib.fp_batch_set_graph(self._h, 1 if enable else 0))
+
+    def info(self):
+        inf = _abi.FpBatchInfo()
+        self._check(self._lib.fp_batch_get_info(self._h, ctypes.byref(inf)))
+        return inf
+
+    def profile(self, enable=True):
+        self._check(self._lib.fp_batch_profile_enable(self._h, 1 if enable else 0))
+
+    def profile_read(self):
+        buf = (FpPassStat * _abi.FP_MAX_PASSES)()
+        n = ctypes.c_int32()
+        self._check(self._lib.fp_batch_profile_read(self._h, buf, _abi.FP_MAX_PASSES, ctypes.byref(n)))
+        return {buf[k].name.decode(): (buf[k].launches, buf[k].total_ms) for k in range(n.value)}
+
+    def close(self):
+        h = self.__dict__.get("_h")
+        if h:
+            self._lib.fp_batch_destroy(h)
+            self._h = None
+        self.scenes = []
+
+    def __del__(self):
+        self.close()
+
+
 def from_scene(scene, device=-1, **param_overrides):
     """A solver set up with a scene's cube and initial fluid (fluidphysics_amd.scenes)."""
     s = Solver(scene.max_fluid, params=scene.params_with(**param_overrides), device=device)
@@ -365,4 +436,4 @@ def from_scene(scene, device=-1, **param_overrides):
     return s
 
 
-__all__ = ["Solver", "FLUID", "SOLID", "FpError", "FpParams", "make_params", "from_scene"]
+__all__ = ["Solver", "Batch", "FLUID", "SOLID", "FpError", "FpParams", "make_params", "from_scene"]

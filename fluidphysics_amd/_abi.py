@@ -51,6 +51,8 @@ class FpParams(ctypes.Structure):
         ("reserved0", ctypes.c_int32),
         ("jacobi_omega", ctypes.c_double),
         ("contact_scale", ctypes.c_double),
+        ("gravity_x", ctypes.c_double),
+        ("gravity_z", ctypes.c_double),
     ]
 
     def to_dict(self):
@@ -74,7 +76,8 @@ REFERENCE_DEFAULTS = dict(
     floor_friction_fluid=0.04, wall_x=3.5, wall_z=3.5, wall_height=0.92,
     wall_half_thick=0.07, cull_radius=7.0, max_speed=14.0, grab_stiffness=0.3,
     cell_size=0.22, constraint_iters=3, table_size=0,
-    pair_slots=0, reserved0=0, jacobi_omega=4.0, contact_scale=1.0,
+    pair_slots=0, reserved0=0, jacobi_omega=4.0, contact_scale=1.0, gravity_x=0.0,
+    gravity_z=0.0,
 )
 
 
@@ -98,6 +101,17 @@ class FpInfo(ctypes.Structure):
         ("substeps_done", ctypes.c_int64),
         ("device_bytes", ctypes.c_int64),
         ("pair_overflow", ctypes.c_int64),
+    ]
+
+
+class FpBatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("scenes", ctypes.c_int32),
+        ("graph", ctypes.c_int32),
+        ("capacity", ctypes.c_int64),
+        ("device_bytes", ctypes.c_int64),
+        ("frames", ctypes.c_int64),
+        ("graph_launches", ctypes.c_int64),
     ]
 
 
@@ -149,6 +163,18 @@ SIGNATURES = {
     "fp_read_density": (_I, [_P, _FP, _FP]),
     "fp_profile_enable": (_I, [_P, _I]),
     "fp_profile_read": (_I, [_P, ctypes.POINTER(FpPassStat), _I, _IP]),
+    "fp_batch_create": (_I, [_I, _PP]),
+    "fp_batch_destroy": (_I, [_P]),
+    "fp_batch_last_error": (ctypes.c_char_p, [_P]),
+    "fp_batch_add": (_I, [_P, _P]),
+    "fp_batch_remove": (_I, [_P, _P]),
+    "fp_batch_step": (_I, [_P, _D, _I]),
+    "fp_batch_step_async": (_I, [_P, _D, _I]),
+    "fp_batch_sync": (_I, [_P]),
+    "fp_batch_set_graph": (_I, [_P, _I]),
+    "fp_batch_get_info": (_I, [_P, ctypes.POINTER(FpBatchInfo)]),
+    "fp_batch_profile_enable": (_I, [_P, _I]),
+    "fp_batch_profile_read": (_I, [_P, ctypes.POINTER(FpPassStat), _I, _IP]),
 }
 
 _lib = None

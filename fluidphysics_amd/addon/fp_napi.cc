@@ -145,6 +145,8 @@ const Field kFields[] = {
     FI("pairSlots", pair_slots),
     FD("jacobiOmega", jacobi_omega),
     FD("contactScale", contact_scale),
+    FD("gravityX", gravity_x),
+    FD("gravityZ", gravity_z),
 };
 
 void params_from_object(napi_env env, napi_value obj, fp_params* p) {

@@ -33,17 +33,27 @@ hipError_t h2d(fp_solver* s, void* dst, const void* src, size_t bytes) {
   return e == hipSuccess ? hipStreamSynchronize(s->stream) : e;
 }
 
-void profile_collect(fp_solver* s) {
-  for (auto& e : s->ev_live) {
+void profile_collect(Prof* p) {
+  for (auto& e : p->live) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
-      s->prof_ms[e.pass] += ms;
-      s->prof_n[e.pass] += 1;
+      p->ms[e.pass] += ms;
+      p->n[e.pass] += 1;
     }
-    s->ev_pool.push_back(e.a);
-    s->ev_pool.push_back(e.b);
+    p->pool.push_back(e.a);
+    p->pool.push_back(e.b);
   }
-  s->ev_live.clear();
+  p->live.clear();
+}
+
+void prof_free(Prof* p) {
+  for (auto& e : p->live) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  for (auto e : p->pool) (void)hipEventDestroy(e);
+  p->live.clear();
+  p->pool.clear();
 }
 
 template <typename T>
@@ -78,11 +88,6 @@ void free_all(fp_solver* s) {
   dfree(s->dpick_i), dfree(s->grab_idx), dfree(s->grab_tgt), dfree(s->emit_idx), dfree(s->emit_p), dfree(s->emit_v);
 }
 
-int log2_pow2(uint32_t v) {
-  int k = 0;
-  while ((1u << k) < v) k++;
-  return k;
-}
 
 uint32_t next_pow2(double v, uint32_t lo) {
   uint32_t t = lo;
@@ -90,10 +95,14 @@ uint32_t next_pow2(double v, uint32_t lo) {
   return t;
 }
 
-// Cell-grid dimensions: NZ and NY cover the basin (walls + 1 m) and 4 m of
-// height without aliasing; the table holds at least 4 z-y planes and the
+// Cell-grid dimensions.  Direct layout: NZ and NY cover the basin (walls +
+// 1 m) and 4 m of height, the table holds at least 4 z-y planes and the
 // reference table size, at most 2^25 keys (far cells beyond that alias and
-// are rejected by the distance tests).
+// are rejected by the distance tests).  A scene whose particles would fill
+// under a quarter of that table gets a compact one instead, 4 keys per
+// particle slot (at least 4096), with hashed column offsets: scans of the
+// counting sort cost 12 B per key, so small scenes (batches of them) keep
+// their tables small.
 void setup_cell_grid(fp_solver* s) {
   const fp_params& P = s->P;
   const double cs = P.cell_size;
@@ -104,14 +113,21 @@ void setup_cell_grid(fp_solver* s) {
   uint64_t M = std::max<uint64_t>(plane * NZ, (uint64_t)s->T);
   M = std::max<uint64_t>(M, 4 * plane);
   M = std::min<uint64_t>(M, 1ull << 25);
-  uint32_t m32 = next_pow2((double)M, 4);
+  const uint32_t direct = next_pow2((double)M, 4);
+  const uint32_t compact = next_pow2(4.0 * s->n, 4096);
   s->cg.inv_cell = 1.0 / cs;
   s->cg.hmask = (uint32_t)(s->T - 1);
-  s->cg.M = m32;
-  s->cg.kmask = m32 - 1;
-  s->cg.nz_shift = (uint32_t)log2_pow2(NZ);
-  s->cg.nyz_shift = (uint32_t)log2_pow2((uint32_t)plane);
-  s->lscan_blocks = (int)((m32 + SCAN_TILE - 1) / SCAN_TILE);
+  if ((uint64_t)compact * 4 <= direct) {
+    s->cg.M = compact;
+    s->cg.mx = 0x9E3779B1u;  // odd: every x step moves the column
+    s->cg.my = 0x85EBCA6Bu;
+  } else {
+    s->cg.M = direct;
+    s->cg.mx = (uint32_t)plane;
+    s->cg.my = NZ;
+  }
+  s->cg.kmask = s->cg.M - 1;
+  s->lscan_blocks = (int)((s->cg.M + SCAN_TILE - 1) / SCAN_TILE);
 }
 
 int set_device(fp_solver* s) {
@@ -145,7 +161,7 @@ void recompute_groups_host(fp_solver* s, const std::vector<uint8_t>& alive) {
 // relabel the components when a tear or a cut happened (solver.js:238-241).
 int settle(fp_solver* s) {
   FP_TRY(s, hipStreamSynchronize(s->stream));
-  if (s->profile) profile_collect(s);
+  if (s->prof.on) profile_collect(&s->prof);
   if (s->steps_pending && s->C > 0) {
     int tear = 0;
     FP_TRY(s, d2h(s, &tear, s->dflags + DF_TEAR, sizeof(int)));
@@ -159,7 +175,7 @@ int settle(fp_solver* s) {
   s->steps_pending = false;
   if (s->groups_dirty) {
     std::vector<uint8_t> al(s->C);
-    if (s->C) FP_TRY(s, d2h(s, al.data(), s->alive[s->cur], s->C));
+    if (s->C) FP_TRY(s, d2h(s, al.data(), s->alive[0], s->C));
     recompute_groups_host(s, al);
     s->groups_dirty = false;
   }
@@ -188,10 +204,10 @@ int fp_fail(fp_solver* s, int code, const std::string& msg) {
   return code;
 }
 
-hipEvent_t fp_event_get(fp_solver* s) {
-  if (!s->ev_pool.empty()) {
-    hipEvent_t e = s->ev_pool.back();
-    s->ev_pool.pop_back();
+hipEvent_t fp_event_get(Prof* p) {
+  if (!p->pool.empty()) {
+    hipEvent_t e = p->pool.back();
+    p->pool.pop_back();
     return e;
   }
   hipEvent_t e;
@@ -260,6 +276,12 @@ int fp_create(const fp_params* params, int32_t max_fluid, int32_t device, fp_sol
     delete s;
     return FP_EDEVICE;
   }
+  s->own_stream = s->stream;
+  s->group = new fp_group();
+  s->group->device = device;
+  s->group->stream = s->stream;
+  s->group->prof = &s->prof;
+  s->group->scenes.push_back(s);
   *out = s;
   return FP_OK;
 }
@@ -267,14 +289,15 @@ int fp_create(const fp_params* params, int32_t max_fluid, int32_t device, fp_sol
 int fp_destroy(fp_solver* s) {
   if (!s) return FP_OK;
   (void)hipSetDevice(s->device);
+  if (s->batch) (void)fp_batch_remove(s->batch, s);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  free_all(s);
-  for (auto& e : s->ev_live) {
-    (void)hipEventDestroy(e.a);
-    (void)hipEventDestroy(e.b);
+  if (s->group) {
+    fp_group_free(s->group);
+    delete s->group;
   }
-  for (auto e : s->ev_pool) (void)hipEventDestroy(e);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+  free_all(s);
+  prof_free(&s->prof);
+  if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
   delete s;
   return FP_OK;
 }
@@ -288,6 +311,7 @@ int fp_set_params(fp_solver* s, const fp_params* p) {
   if (s->has_cube && (p->table_size != s->P.table_size || p->pair_slots != s->P.pair_slots))
     return fp_fail(s, FP_ESTATE, "table_size / pair_slots are fixed once the scene is allocated");
   s->P = *p;
+  s->version++;
   return FP_OK;
 }
 
@@ -451,7 +475,6 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
   }
   FP_TRY(s, h2d(s, s->adj_off, hoff.data(), (ns + 1) * sizeof(int)));
   // prev stays zero until the first integrate (solver.js:43)
-  s->cur = 0;
   s->has_cube = true;
   return FP_OK;
 }
@@ -517,8 +540,8 @@ int fp_reset_solid(fp_solver* s) {
   FP_TRY(s, h2d(s, s->vel, v.data(), ns * sizeof(float4)));
   if (s->C) {
     std::vector<uint8_t> ones(s->C, 1);
-    FP_TRY(s, h2d(s, s->rest[s->cur], s->h_snap_rest.data(), s->C * sizeof(float)));
-    FP_TRY(s, h2d(s, s->alive[s->cur], ones.data(), s->C));
+    FP_TRY(s, h2d(s, s->rest[0], s->h_snap_rest.data(), s->C * sizeof(float)));
+    FP_TRY(s, h2d(s, s->alive[0], ones.data(), s->C));
   }
   s->h_group.assign(ns, 0);
   s->groups_dirty = false;
@@ -547,8 +570,8 @@ static int step_impl(fp_solver* s, double dt, int32_t substeps, bool wait) {
     s->cut_dirty_at_step = true;
     s->cut_dirty = false;
   }
-  for (int k = 0; k < substeps; k++)
-    if ((rc = fp_launch_substep(s, h))) return rc;
+  (void)h;
+  if ((rc = fp_group_frame(s->group, dt, substeps))) return fp_fail(s, rc, s->group->err);
   s->steps_pending = true;
   return wait ? settle(s) : FP_OK;
 }
@@ -625,7 +648,7 @@ int fp_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[
   if ((rc = fp_launch_cut(s, o, d1, d2, nx, ny, nz))) return rc;
   int cnt = 0;
   FP_TRY(s, d2h(s, &cnt, s->dflags + DF_CUT, sizeof(int)));
-  if (s->profile) profile_collect(s);
+  if (s->prof.on) profile_collect(&s->prof);
   if (ncut) *ncut = cnt;
   // the reference marks components dirty; they are relabelled at the end
   // of the next step (solver.js:238-241, :386)
@@ -738,8 +761,8 @@ int fp_read_constraints(fp_solver* s, int32_t* a, int32_t* b, float* rest, uint8
   if ((rc = settle(s))) return rc;
   if (a) std::memcpy(a, s->h_cons_a.data(), s->C * sizeof(int32_t));
   if (b) std::memcpy(b, s->h_cons_b.data(), s->C * sizeof(int32_t));
-  if (rest && s->C) FP_TRY(s, d2h(s, rest, s->rest[s->cur], s->C * sizeof(float)));
-  if (alive && s->C) FP_TRY(s, d2h(s, alive, s->alive[s->cur], s->C));
+  if (rest && s->C) FP_TRY(s, d2h(s, rest, s->rest[0], s->C * sizeof(float)));
+  if (alive && s->C) FP_TRY(s, d2h(s, alive, s->alive[0], s->C));
   return FP_OK;
 }
 
@@ -749,8 +772,8 @@ int fp_write_constraints(fp_solver* s, const float* rest, const uint8_t* alive) 
   int rc = set_device(s);
   if (rc) return rc;
   if ((rc = settle(s))) return rc;
-  if (rest && s->C) FP_TRY(s, h2d(s, s->rest[s->cur], rest, s->C * sizeof(float)));
-  if (alive && s->C) FP_TRY(s, h2d(s, s->alive[s->cur], alive, s->C));
+  if (rest && s->C) FP_TRY(s, h2d(s, s->rest[0], rest, s->C * sizeof(float)));
+  if (alive && s->C) FP_TRY(s, h2d(s, s->alive[0], alive, s->C));
   return FP_OK;
 }
 
@@ -795,10 +818,10 @@ int fp_profile_enable(fp_solver* s, int32_t enable) {
   int rc = set_device(s);
   if (rc) return rc;
   if ((rc = settle(s))) return rc;
-  s->profile = enable != 0;
+  s->prof.on = enable != 0;
   for (int k = 0; k < P_NUM; k++) {
-    s->prof_ms[k] = 0;
-    s->prof_n[k] = 0;
+    s->prof.ms[k] = 0;
+    s->prof.n[k] = 0;
   }
   return FP_OK;
 }
@@ -810,11 +833,178 @@ int fp_profile_read(fp_solver* s, fp_pass_stat* out, int32_t max, int32_t* n) {
   if ((rc = settle(s))) return rc;
   int m = 0;
   for (int k = 0; k < P_NUM && m < max; k++) {
-    if (!s->prof_n[k]) continue;
+    if (!s->prof.n[k]) continue;
     std::memset(out[m].name, 0, sizeof(out[m].name));
     std::strncpy(out[m].name, kPassNames[k], sizeof(out[m].name) - 1);
-    out[m].launches = s->prof_n[k];
-    out[m].total_ms = s->prof_ms[k];
+    out[m].launches = s->prof.n[k];
+    out[m].total_ms = s->prof.ms[k];
+    m++;
+  }
+  *n = m;
+  return FP_OK;
+}
+
+
+// ---- batches: many scenes, one launch per pass -------------------------------
+
+struct fp_batch {
+  fp_group g;
+  Prof prof;
+  std::string err;
+  int64_t frames = 0;
+};
+
+static int batch_fail(fp_batch* b, int code, const std::string& msg) {
+  if (b)
+    b->err = msg;
+  else
+    g_last_error = msg;
+  return code;
+}
+
+int fp_batch_create(int32_t device, fp_batch** out) {
+  if (!out) return batch_fail(nullptr, FP_EINVAL, "fp_batch_create: out is NULL");
+  *out = nullptr;
+  if (device < 0 && hipGetDevice(&device) != hipSuccess) device = 0;
+  fp_batch* b = new (std::nothrow) fp_batch();
+  if (!b) return batch_fail(nullptr, FP_ENOMEM, "fp_batch_create: out of host memory");
+  b->g.device = device;
+  b->g.prof = &b->prof;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->g.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_last_error = std::string("fp_batch_create: ") + hipGetErrorString(e);
+    delete b;
+    return FP_EDEVICE;
+  }
+  *out = b;
+  return FP_OK;
+}
+
+const char* fp_batch_last_error(const fp_batch* b) { return b ? b->err.c_str() : g_last_error.c_str(); }
+
+// The scene leaves the batch: back to its own stream.
+static void batch_detach(fp_batch* b, fp_solver* s) {
+  (void)hipStreamSynchronize(b->g.stream);
+  s->stream = s->own_stream;
+  s->group->stream = s->own_stream;
+  s->batch = nullptr;
+}
+
+int fp_batch_destroy(fp_batch* b) {
+  if (!b) return FP_OK;
+  (void)hipSetDevice(b->g.device);
+  for (fp_solver* s : b->g.scenes) batch_detach(b, s);
+  b->g.scenes.clear();
+  fp_group_free(&b->g);
+  prof_free(&b->prof);
+  if (b->g.stream) (void)hipStreamDestroy(b->g.stream);
+  delete b;
+  return FP_OK;
+}
+
+int fp_batch_add(fp_batch* b, fp_solver* s) {
+  if (!b || !s) return batch_fail(b, FP_EINVAL, "fp_batch_add: NULL argument");
+  if (!s->has_cube) return batch_fail(b, FP_ESTATE, "fp_batch_add: the scene has no cube yet");
+  if (s->batch) return batch_fail(b, FP_ESTATE, "fp_batch_add: the scene is in a batch already");
+  if (s->device != b->g.device) return batch_fail(b, FP_EINVAL, "fp_batch_add: the scene is on another device");
+  (void)hipSetDevice(b->g.device);
+  // work queued on the scene's own stream finishes before the batch's
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return batch_fail(b, FP_EDEVICE, "fp_batch_add: stream sync");
+  s->stream = b->g.stream;
+  s->group->stream = b->g.stream;
+  s->batch = b;
+  b->g.scenes.push_back(s);
+  b->g.dirty = true;
+  return FP_OK;
+}
+
+int fp_batch_remove(fp_batch* b, fp_solver* s) {
+  if (!b || !s) return batch_fail(b, FP_EINVAL, "fp_batch_remove: NULL argument");
+  auto it = std::find(b->g.scenes.begin(), b->g.scenes.end(), s);
+  if (it == b->g.scenes.end()) return batch_fail(b, FP_EINVAL, "fp_batch_remove: the scene is not in this batch");
+  (void)hipSetDevice(b->g.device);
+  batch_detach(b, s);
+  b->g.scenes.erase(it);
+  b->g.dirty = true;
+  return FP_OK;
+}
+
+static int batch_step_impl(fp_batch* b, double dt, int32_t substeps, bool wait) {
+  if (!b) return batch_fail(nullptr, FP_EINVAL, "fp_batch_step: NULL batch");
+  if (substeps < 1) return batch_fail(b, FP_EINVAL, "fp_batch_step: substeps < 1");
+  FP_GTRY(&b->g, hipSetDevice(b->g.device));
+  for (fp_solver* s : b->g.scenes)
+    if (s->cut_dirty) {
+      s->cut_dirty_at_step = true;
+      s->cut_dirty = false;
+    }
+  int rc = fp_group_frame(&b->g, dt, substeps);
+  if (rc) return batch_fail(b, rc, b->g.err);
+  b->frames++;
+  for (fp_solver* s : b->g.scenes) s->steps_pending = true;
+  if (!wait) return FP_OK;
+  return fp_batch_sync(b);
+}
+
+int fp_batch_step(fp_batch* b, double dt, int32_t substeps) { return batch_step_impl(b, dt, substeps, true); }
+int fp_batch_step_async(fp_batch* b, double dt, int32_t substeps) { return batch_step_impl(b, dt, substeps, false); }
+
+int fp_batch_sync(fp_batch* b) {
+  if (!b) return batch_fail(nullptr, FP_EINVAL, "fp_batch_sync: NULL batch");
+  FP_GTRY(&b->g, hipSetDevice(b->g.device));
+  FP_GTRY(&b->g, hipStreamSynchronize(b->g.stream));
+  if (b->prof.on) profile_collect(&b->prof);
+  for (fp_solver* s : b->g.scenes) {
+    const int rc = settle(s);
+    if (rc) return batch_fail(b, rc, s->err);
+  }
+  return FP_OK;
+}
+
+int fp_batch_set_graph(fp_batch* b, int32_t enable) {
+  if (!b) return batch_fail(nullptr, FP_EINVAL, "fp_batch_set_graph: NULL batch");
+  b->g.use_graph = enable != 0;
+  return FP_OK;
+}
+
+int fp_batch_get_info(const fp_batch* b, fp_batch_info* out) {
+  if (!b || !out) return FP_EINVAL;
+  std::memset(out, 0, sizeof(*out));
+  out->scenes = (int32_t)b->g.scenes.size();
+  out->graph = b->g.gexec != nullptr ? 1 : 0;
+  for (const fp_solver* s : b->g.scenes) {
+    out->capacity += s->n;
+    out->device_bytes += s->device_bytes;
+  }
+  out->frames = b->frames;
+  out->graph_launches = b->g.graph_launches;
+  return FP_OK;
+}
+
+int fp_batch_profile_enable(fp_batch* b, int32_t enable) {
+  if (!b) return batch_fail(nullptr, FP_EINVAL, "fp_batch_profile_enable: NULL batch");
+  const int rc = fp_batch_sync(b);
+  if (rc) return rc;
+  b->prof.on = enable != 0;
+  for (int k = 0; k < P_NUM; k++) {
+    b->prof.ms[k] = 0;
+    b->prof.n[k] = 0;
+  }
+  return FP_OK;
+}
+
+int fp_batch_profile_read(fp_batch* b, fp_pass_stat* out, int32_t max, int32_t* n) {
+  if (!b || !out || !n) return batch_fail(b, FP_EINVAL, "fp_batch_profile_read: NULL argument");
+  const int rc = fp_batch_sync(b);
+  if (rc) return rc;
+  int m = 0;
+  for (int k = 0; k < P_NUM && m < max; k++) {
+    if (!b->prof.n[k]) continue;
+    std::memset(out[m].name, 0, sizeof(out[m].name));
+    std::strncpy(out[m].name, kPassNames[k], sizeof(out[m].name) - 1);
+    out[m].launches = b->prof.n[k];
+    out[m].total_ms = b->prof.ms[k];
     m++;
   }
   *n = m;

@@ -49,9 +49,6 @@
 #ifndef FLUID_BLOCK
 #define FLUID_BLOCK 256  // threads per block of the density, displacement and XSPH passes
 #endif
-#ifndef FLUID_XCD
-#define FLUID_XCD 0      // 1: give each XCD a contiguous range of slots
-#endif
 #ifndef DENSITY_BATCH
 #define DENSITY_BATCH 2  // candidates per step of the density loop
 #endif
@@ -169,16 +166,6 @@ __device__ __forceinline__ bool is_bonded(int x, int y, const Bonds& b, const in
   return c >= 0 && alive[c] != 0;
 }
 
-struct ContactCtx {
-  const float4* __restrict__ spos;  // grid order: xyz + inverse mass
-  const int* __restrict__ sid;
-  const int* __restrict__ lstart;   // grid-order cell starts
-  const int* __restrict__ lnsol;    // solids of each non-empty cell (they come first)
-  const int* __restrict__ adj_off;
-  const int4* __restrict__ adj;
-  const uint8_t* __restrict__ alive;
-  const int* __restrict__ sbox;  // solids' bounding box at grid time (ordered-int bits)
-};
 
 __device__ __forceinline__ float unorder(int v) { return __int_as_float(v >= 0 ? v : v ^ 0x7fffffff); }
 
@@ -220,9 +207,12 @@ __device__ __forceinline__ void solid_candidate(const FluidParams& fp, const Con
 // each sums its candidates in slot order (cells beyond the contact range
 // skipped, loads four at a time), then the lanes' sums are added pairwise.
 // Writes pos[x] = grid-time position + correction.
-__global__ void __launch_bounds__(BLOCK) k_contacts_solid(FluidParams fp, ContactCtx cc, const uint8_t* __restrict__ active,
-                                                          float4* __restrict__ pos) {
-  const int t = blockIdx.x * BLOCK + threadIdx.x;
+__device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb) {
+  const FluidParams& fp = A.fp;
+  const ContactCtx& cc = A.cc;
+  const uint8_t* __restrict__ active = A.active;
+  float4* __restrict__ pos = A.pos;
+  const int t = lb * BLOCK + threadIdx.x;
   const int x = t / CONTACT_LANES, lane = t % CONTACT_LANES;
   const bool valid = x < fp.ns && active[x];
   float ax = 0.f, ay = 0.f, az = 0.f;
@@ -262,13 +252,22 @@ __global__ void __launch_bounds__(BLOCK) k_contacts_solid(FluidParams fp, Contac
   if (valid && lane == 0) pos[x] = make_float4(px.x + ax, px.y + ay, px.z + az, px.w);
 }
 
+__global__ void __launch_bounds__(BLOCK) kg_contacts_solid(const SceneArgs* __restrict__ S,
+                                                           const int2* __restrict__ W, int nw) {
+  FOR_ITEMS(S, W, nw) contacts_solid_body(A, lb);
+}
+
 // Fluid in grid slot e: corrections from the solids of its 27 cells (rows in
 // order, cells of a row in order, solids of a cell by ascending index), then
 // the post-contact cell: flags, count, staging for the fluid order.
-__global__ void __launch_bounds__(BLOCK) k_contacts_fluid(FluidParams fp, ContactCtx cc, float4* __restrict__ tpos,
-                                                          uint32_t* __restrict__ tkey, uint32_t* __restrict__ tslot,
-                                                          int* __restrict__ lcount) {
-  const int e = blockIdx.x * BLOCK + threadIdx.x;
+__device__ __forceinline__ void contacts_fluid_body(const SceneArgs& A, int lb) {
+  const FluidParams& fp = A.fp;
+  const ContactCtx& cc = A.cc;
+  float4* __restrict__ tpos = A.tpos;
+  uint32_t* __restrict__ tkey = A.tkey;
+  uint32_t* __restrict__ tslot = A.tslot;
+  int* __restrict__ lcount = A.lcount;
+  const int e = lb * BLOCK + threadIdx.x;
   // every lane stays to the bucket vote at the end
   bool has = false;
   uint32_t k1 = 0;
@@ -330,20 +329,14 @@ __global__ void __launch_bounds__(BLOCK) k_contacts_fluid(FluidParams fp, Contac
   }
 }
 
-// Lane index of this thread over the fluid order: FLUID_LANES consecutive
-// lanes share one fluid slot (slot = lane / FLUID_LANES).  Workgroup b runs
-// on XCD b % 8; with FLUID_XCD the blocks are renumbered so that each XCD
-// (and its L2) works through one contiguous eighth of the cell-sorted slots.
-// The grid is a multiple of 8.
-__device__ __forceinline__ int fluid_lane() {
-  int b = blockIdx.x;
-  if (FLUID_XCD) b = (b & 7) * (int)(gridDim.x >> 3) + (b >> 3);
-  return b * FLUID_BLOCK + (int)threadIdx.x;
+__global__ void __launch_bounds__(BLOCK) kg_contacts_fluid(const SceneArgs* __restrict__ S,
+                                                           const int2* __restrict__ W, int nw) {
+  FOR_ITEMS(S, W, nw) contacts_fluid_body(A, lb);
 }
 
-static inline int fluid_blocks(int n) {
-  return ((int)(((int64_t)n * FLUID_LANES + FLUID_BLOCK - 1) / FLUID_BLOCK) + 7) & ~7;
-}
+// Lane index of this thread over the fluid order: FLUID_LANES consecutive
+// lanes share one fluid slot (slot = lane / FLUID_LANES).
+__device__ __forceinline__ int fluid_lane(int lb) { return lb * FLUID_BLOCK + (int)threadIdx.x; }
 
 // Sum over the FLUID_LANES lanes of one slot (pairwise, the same in every lane).
 __device__ __forceinline__ float lanes_sum(float v) {
@@ -359,11 +352,6 @@ __device__ __forceinline__ int lanes_sum(int v) {
 
 // ---- the reference's fluid pairs of the particle in slot e ------------------
 
-struct PairSource {
-  const float4* __restrict__ spos2;     // fluid order: post-contact xyz, w = index | flags
-  const uint32_t* __restrict__ sphash;  // fluid order: bucket at grid time
-  const int* __restrict__ lstart;       // fluid-order cell starts
-};
 
 struct PairSelf {
   int e, x;
@@ -458,11 +446,6 @@ __device__ __forceinline__ RowRange row_range(const FluidParams& fp, const int* 
 #define NB_QBITS 27
 #define NB_QMASK ((1u << NB_QBITS) - 1u)
 
-struct PairList {
-  uint32_t* __restrict__ nbr;
-  int* __restrict__ ncount;
-  int stride, cap;
-};
 
 // v_sqrt_f32 (within 1 ulp); the reference's own square root is a double
 // one, so no float32 rounding of it is more faithful than another.
@@ -689,17 +672,23 @@ __device__ __forceinline__ void density_store(const FluidParams& fp, const PairL
   sP[me.e] = make_float2(fp.stiff * (rho - fp.rest_density), fp.near_stiff * rhon);
 }
 
-__global__ void __launch_bounds__(FLUID_BLOCK) k_density(FluidParams fp, PairSource src, PairList pl,
-                                                   float2* __restrict__ sP, float2* __restrict__ rho_out,
-                                                   int* __restrict__ sbox, int* __restrict__ dflags) {
+__device__ __forceinline__ void density_body(const SceneArgs& A, int lb) {
+  const FluidParams& fp = A.fp;
+  const PairSource& src = A.src;
+  const PairList& pl = A.pl;
   // the solids' bounding box is rebuilt by the next integrate
-  if (blockIdx.x == 0 && threadIdx.x < 6) sbox[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : INT_MIN;
+  if (lb == 0 && threadIdx.x < 6) A.sbox[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : INT_MIN;
   PairSelf me;
-  if (!pair_self(fp, src, fluid_lane(), me)) return;
+  if (!pair_self(fp, src, fluid_lane(lb), me)) return;
   DensityGather g;
   g.wp = pl.nbr + me.v;
   density_gather(fp, src, pl, me, g);
-  density_store(fp, pl, me, g, sP, rho_out, dflags);
+  density_store(fp, pl, me, g, A.sP, A.rho, A.dflags);
+}
+
+__global__ void __launch_bounds__(FLUID_BLOCK) kg_density(const SceneArgs* __restrict__ S,
+                                                          const int2* __restrict__ W, int nw) {
+  FOR_ITEMS(S, W, nw) density_body(A, lb);
 }
 
 struct DisplaceAcc {
@@ -728,10 +717,14 @@ struct DisplaceAcc {
 
 // spos2 + the clamped displacement -> pos for the fluid; the inverse mass
 // is kept from pos.w.
-__global__ void __launch_bounds__(FLUID_BLOCK) k_displace(FluidParams fp, PairSource src, PairList pl,
-                                                    const float2* __restrict__ sP, float4* __restrict__ pos) {
+__device__ __forceinline__ void displace_body(const SceneArgs& A, int lb) {
+  const FluidParams& fp = A.fp;
+  const PairSource& src = A.src;
+  const PairList& pl = A.pl;
+  const float2* __restrict__ sP = A.sP;
+  float4* __restrict__ pos = A.pos;
   PairSelf me;
-  if (!pair_self(fp, src, fluid_lane(), me)) return;
+  if (!pair_self(fp, src, fluid_lane(lb), me)) return;
   DisplaceAcc acc;
   acc.sP = sP;
   acc.Px = sP[me.e];
@@ -749,6 +742,11 @@ __global__ void __launch_bounds__(FLUID_BLOCK) k_displace(FluidParams fp, PairSo
   }
   const float w = pos[me.x].w;
   pos[me.x] = make_float4(me.p.x + sx, me.p.y + sy, me.p.z + sz, w);
+}
+
+__global__ void __launch_bounds__(FLUID_BLOCK) kg_displace(const SceneArgs* __restrict__ S,
+                                                           const int2* __restrict__ W, int nw) {
+  FOR_ITEMS(S, W, nw) displace_body(A, lb);
 }
 
 struct XsphAcc {
@@ -772,11 +770,15 @@ struct XsphAcc {
 
 // XSPH: svtmp -> vel for the fluid that is still active (the reference skips
 // particles culled since the density pass, whose velocity stays as it was).
-__global__ void __launch_bounds__(FLUID_BLOCK) k_xsph(FluidParams fp, PairSource src, PairList pl,
-                                                const float4* __restrict__ svtmp, float4* __restrict__ vel,
-                                                const uint8_t* __restrict__ active) {
+__device__ __forceinline__ void xsph_body(const SceneArgs& A, int lb) {
+  const FluidParams& fp = A.fp;
+  const PairSource& src = A.src;
+  const PairList& pl = A.pl;
+  const float4* __restrict__ svtmp = A.svtmp;
+  float4* __restrict__ vel = A.vel;
+  const uint8_t* __restrict__ active = A.active;
   PairSelf me;
-  if (!pair_self(fp, src, fluid_lane(), me)) return;
+  if (!pair_self(fp, src, fluid_lane(lb), me)) return;
   if (!active[me.x]) return;
   XsphAcc acc;
   acc.svtmp = svtmp;
@@ -792,6 +794,11 @@ __global__ void __launch_bounds__(FLUID_BLOCK) k_xsph(FluidParams fp, PairSource
   const float inv_m = 1.f / (float)m;
   vel[me.x] = make_float4(v.x + fp.visc * (sx * inv_m - v.x), v.y + fp.visc * (sy * inv_m - v.y),
                           v.z + fp.visc * (sz * inv_m - v.z), 0.f);
+}
+
+__global__ void __launch_bounds__(FLUID_BLOCK) kg_xsph(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
+                                                       int nw) {
+  FOR_ITEMS(S, W, nw) xsph_body(A, lb);
 }
 
 // ---- launchers ------------------------------------------------------------------
@@ -825,60 +832,50 @@ FluidParams fp_fluid_params(const fp_solver* s) {
   return fp;
 }
 
-static PairSource pair_source(const fp_solver* s) {
-  PairSource src;
-  src.spos2 = s->spos2;
-  src.sphash = s->sphash2;
-  src.lstart = s->lstart;
-  return src;
+// Blocks of one scene in a pass of the given kind (fp_internal.h WorkKind).
+int fp_scene_blocks(const fp_solver* s, int kind) {
+  switch (kind) {
+    case W_PARTICLE:
+      return fp_blocks(s->n);
+    case W_SCAN:
+      return s->lscan_blocks;
+    case W_SOLID:
+      return fp_blocks(s->ns * CONTACT_LANES);
+    case W_FLUID:
+      return (int)(((int64_t)s->n * FLUID_LANES + FLUID_BLOCK - 1) / FLUID_BLOCK);
+    case W_CONS:
+      return (int)(((int64_t)s->ns * 32 + BLOCK - 1) / BLOCK);
+    case W_CONSTR:
+      return fp_blocks(s->C);
+  }
+  return 0;
 }
 
-static PairList pair_list(const fp_solver* s) {
-  PairList pl;
-  pl.nbr = s->nbr;
-  pl.ncount = s->ncount;
-  pl.stride = s->n * FLUID_LANES;
-  pl.cap = s->pair_slots / FLUID_LANES;
-  return pl;
-}
-
-int fp_launch_contacts(fp_solver* s) {
-  const FluidParams fp = fp_fluid_params(s);
-  ContactCtx cc{s->spos, s->sid, s->lstart, s->lnsol, s->adj_off, s->adj, s->alive[s->cur], s->sbox};
-  PassScope ps(s, P_CONTACTS);
-  if (s->ns > 0)
-    hipLaunchKernelGGL(k_contacts_solid, dim3(fp_blocks(s->ns * CONTACT_LANES)), dim3(BLOCK), 0, s->stream, fp, cc,
-                       s->active, s->pos);
-  hipLaunchKernelGGL(k_contacts_fluid, dim3(fp_blocks(s->n)), dim3(BLOCK), 0, s->stream, fp, cc, s->tpos, s->tkey,
-                     s->tslot, s->lcount);
-  FP_TRY(s, hipGetLastError());
+int fp_group_contacts(fp_group* g) {
+  PassScope ps(g->prof, g->stream, P_CONTACTS);
+  GROUP_LAUNCH(g, kg_contacts_solid, W_SOLID, BLOCK);
+  GROUP_LAUNCH(g, kg_contacts_fluid, W_PARTICLE, BLOCK);
+  FP_GTRY(g, hipGetLastError());
   return FP_OK;
 }
 
-int fp_launch_density(fp_solver* s) {
-  const FluidParams fp = fp_fluid_params(s);
-  PassScope ps(s, P_DENSITY);
-  hipLaunchKernelGGL(k_density, dim3(fluid_blocks(s->n)), dim3(FLUID_BLOCK), 0, s->stream, fp, pair_source(s),
-                     pair_list(s), s->sP, s->rho, s->sbox, s->dflags);
-  FP_TRY(s, hipGetLastError());
+int fp_group_density(fp_group* g) {
+  PassScope ps(g->prof, g->stream, P_DENSITY);
+  GROUP_LAUNCH(g, kg_density, W_FLUID, FLUID_BLOCK);
+  FP_GTRY(g, hipGetLastError());
   return FP_OK;
 }
 
-int fp_launch_displace(fp_solver* s) {
-  const FluidParams fp = fp_fluid_params(s);
-  PassScope ps(s, P_DISPLACE);
-  hipLaunchKernelGGL(k_displace, dim3(fluid_blocks(s->n)), dim3(FLUID_BLOCK), 0, s->stream, fp, pair_source(s),
-                     pair_list(s), s->sP, s->pos);
-  FP_TRY(s, hipGetLastError());
+int fp_group_displace(fp_group* g) {
+  PassScope ps(g->prof, g->stream, P_DISPLACE);
+  GROUP_LAUNCH(g, kg_displace, W_FLUID, FLUID_BLOCK);
+  FP_GTRY(g, hipGetLastError());
   return FP_OK;
 }
 
-int fp_launch_xsph(fp_solver* s) {
-  if (s->nf == 0) return FP_OK;
-  const FluidParams fp = fp_fluid_params(s);
-  PassScope ps(s, P_XSPH);
-  hipLaunchKernelGGL(k_xsph, dim3(fluid_blocks(s->n)), dim3(FLUID_BLOCK), 0, s->stream, fp, pair_source(s), pair_list(s),
-                     s->svtmp, s->vel, s->active);
-  FP_TRY(s, hipGetLastError());
+int fp_group_xsph(fp_group* g) {
+  PassScope ps(g->prof, g->stream, P_XSPH);
+  GROUP_LAUNCH(g, kg_xsph, W_FLUID, FLUID_BLOCK);
+  FP_GTRY(g, hipGetLastError());
   return FP_OK;
 }

@@ -1,7 +1,7 @@
 // fp_grid.hip -- the two grids.
 //
 // 1. The cell grid of the hot path (every substep).  Key of a cell c:
-//      L(c) = (cx * NY*NZ + cy * NZ + cz) mod M      (32-bit wrap, M, NY, NZ powers of 2)
+//      L(c) = (cx * mx + cy * my + cz) mod M      (32-bit wrap, M a power of 2)
 //    so the three z-neighbours of a cell are consecutive keys: the 27
 //    neighbour cells of a particle are 9 runs of 3 keys, i.e. 9 contiguous
 //    ranges of the cell-sorted order.  Far cells that alias to the same key
@@ -12,7 +12,7 @@
 //    reads contiguously (sid = particle of each sorted slot, spos = position
 //    + inverse mass, sphash = the particle's bucket in the reference's hash
 //    table).  After contacts the fluid is sorted again by its post-contact
-//    cell (fp_launch_resort) for the fluid passes.
+//    cell (fp_group_resort) for the fluid passes.
 //
 // 2. The reference's hash grid (_buildGrid, src/solver.js:174-184), built on
 //    demand for fp_build_grid / fp_read_grid from the per-particle buckets
@@ -21,18 +21,23 @@
 //    filling each bucket from its end for ascending i does, :182-183).
 //    Bit-exact cellStart / cellEntries.
 //
-// Scan: per-tile sums (k_scan_reduce), then each tile scans its counts with a
-// wave64 DPP scan plus the carry of the tiles before it (k_scan_apply), and
+// Scan: per-tile sums (scan_reduce), then each tile scans its counts with a
+// wave64 DPP scan plus the carry of the tiles before it (scan_apply), and
 // zeroes the counts for the next build.  Bytes: 12 per table entry.
+//
+// The hot-path kernels (kg_*) are group kernels: they stride over a work list
+// of (scene, block of that scene) items and read the scene's arrays from its
+// SceneArgs entry, so one launch builds the grids of every scene of a batch.
 #include "fp_internal.h"
 #include "fp_kernels.h"
 
 // ---- scan (shared by both grids) ---------------------------------------------
 
-__global__ void __launch_bounds__(BLOCK) k_scan_reduce(int t, const int* __restrict__ count,
-                                                       int* __restrict__ block_sums) {
+// Sum of scan tile `tile` of count[0, t) into block_sums[tile].
+__device__ __forceinline__ void scan_reduce_body(int t, const int* __restrict__ count, int* __restrict__ block_sums,
+                                                 int tile) {
   __shared__ int wsum[BLOCK / 64];
-  const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  const int base = tile * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   int s = 0;
   if (base + SCAN_ITEMS <= t) {
     const int4* c4 = reinterpret_cast<const int4*>(count + base);
@@ -52,23 +57,21 @@ __global__ void __launch_bounds__(BLOCK) k_scan_reduce(int t, const int* __restr
     int tot = 0;
 #pragma unroll
     for (int w = 0; w < BLOCK / 64; w++) tot += wsum[w];
-    block_sums[blockIdx.x] = tot;
+    block_sums[tile] = tot;
   }
+  __syncthreads();
 }
 
-// Exclusive scan of one tile plus the carry of the tiles before it; writes
+// Exclusive scan of tile `tile` plus the carry of the tiles before it; writes
 // start[k] and zeroes count[k]; the tile holding the last entry writes
-// start[t] = total.  zero_word (may be null) is cleared for the passes that
-// follow (a counter, cleared by a kernel so the frame can be graph-captured).
-__global__ void __launch_bounds__(BLOCK) k_scan_apply(int t, int* __restrict__ count,
-                                                      const int* __restrict__ block_sums, int* __restrict__ start,
-                                                      int* __restrict__ zero_word) {
-  if (zero_word && blockIdx.x == 0 && threadIdx.x == 0) *zero_word = 0;
+// start[t] = total.
+__device__ __forceinline__ void scan_apply_body(int t, int* __restrict__ count, const int* __restrict__ block_sums,
+                                                int* __restrict__ start, int tile) {
   __shared__ int wsum[BLOCK / 64];
   __shared__ int carry_s;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int c = 0;
-  for (int k = threadIdx.x; k < (int)blockIdx.x; k += BLOCK) c += block_sums[k];
+  for (int k = threadIdx.x; k < tile; k += BLOCK) c += block_sums[k];
   c = wave_sum(c);
   if (lane == 0) wsum[wid] = c;
   __syncthreads();
@@ -82,7 +85,7 @@ __global__ void __launch_bounds__(BLOCK) k_scan_apply(int t, int* __restrict__ c
   const int carry = carry_s;
   __syncthreads();
 
-  const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  const int base = tile * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
   int v[SCAN_ITEMS];
   const bool full = base + SCAN_ITEMS <= t;
   if (full) {
@@ -135,102 +138,121 @@ __global__ void __launch_bounds__(BLOCK) k_scan_apply(int t, int* __restrict__ c
     }
   }
   if (base <= t - 1 && t - 1 < base + SCAN_ITEMS) start[t] = run;
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(BLOCK) k_scan_reduce(int t, const int* __restrict__ count,
+                                                       int* __restrict__ block_sums) {
+  scan_reduce_body(t, count, block_sums, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_scan_apply(int t, int* __restrict__ count,
+                                                      const int* __restrict__ block_sums, int* __restrict__ start) {
+  scan_apply_body(t, count, block_sums, start, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(BLOCK) kg_scan_reduce(const SceneArgs* __restrict__ S,
+                                                        const int2* __restrict__ W, int nw) {
+  FOR_ITEMS(S, W, nw) scan_reduce_body((int)A.fp.cg.M, A.lcount, A.lblock_sums, lb);
+}
+
+__global__ void __launch_bounds__(BLOCK) kg_scan_apply(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
+                                                       int nw) {
+  FOR_ITEMS(S, W, nw) scan_apply_body((int)A.fp.cg.M, A.lcount, A.lblock_sums, A.lstart, lb);
 }
 
 // ---- cell grid -----------------------------------------------------------------
 
 // Count of the cell grid when integrate could not do it (grab rounds ran in
 // between): cell key + reference bucket of the current position.
-__global__ void __launch_bounds__(BLOCK) k_cell_count(CellGrid cg, int n, int ns, const float4* __restrict__ pos,
-                                                      const uint8_t* __restrict__ active, int* __restrict__ lcount,
-                                                      uint32_t* __restrict__ lkey, uint32_t* __restrict__ lslot,
-                                                      uint32_t* __restrict__ phash, int* __restrict__ sbox) {
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  const bool in = i < n;
-  const bool act = in && active[i];
+__device__ __forceinline__ void cell_count_body(const SceneArgs& A, int lb) {
+  const CellGrid& cg = A.fp.cg;
+  const int i = lb * BLOCK + threadIdx.x;
+  const bool in = i < A.n;
+  const bool act = in && A.active[i];
   float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t k = 0;
   if (act) {
-    p = pos[i];
+    p = A.pos[i];
     const Cell c = cell_of(p.x, p.y, p.z, cg.inv_cell);
-    phash[i] = cell_hash(c, cg.hmask);
+    A.phash[i] = cell_hash(c, cg.hmask);
     k = cell_key(cg, c);
-    lkey[i] = k;
+    A.lkey[i] = k;
   } else if (in) {
-    phash[i] = 0xffffffffu;
+    A.phash[i] = 0xffffffffu;
   }
-  const uint32_t sl = wave_bucket_slot(lcount, k, act);
-  if (act) lslot[i] = sl;
-  box_update(act && i < ns, p.x, p.y, p.z, sbox);
+  const uint32_t sl = wave_bucket_slot(A.lcount, k, act);
+  if (act) A.lslot[i] = sl;
+  box_update(act && i < A.ns, p.x, p.y, p.z, A.sbox);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_cell_scatter(int n, const uint32_t* __restrict__ phash,
-                                                        const uint32_t* __restrict__ lkey,
-                                                        const uint32_t* __restrict__ lslot,
-                                                        const int* __restrict__ lstart, int* __restrict__ tmp) {
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= n || phash[i] == 0xffffffffu) return;
-  tmp[lstart[lkey[i]] + (int)lslot[i]] = i;
+__global__ void __launch_bounds__(BLOCK) kg_cell_count(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
+                                                       int nw) {
+  FOR_ITEMS(S, W, nw) cell_count_body(A, lb);
+}
+
+__device__ __forceinline__ void cell_scatter_body(const SceneArgs& A, int lb) {
+  const int i = lb * BLOCK + threadIdx.x;
+  if (i >= A.n || A.phash[i] == 0xffffffffu) return;
+  A.ltmp[A.lstart[A.lkey[i]] + (int)A.lslot[i]] = i;
+}
+
+__global__ void __launch_bounds__(BLOCK) kg_cell_scatter(const SceneArgs* __restrict__ S,
+                                                         const int2* __restrict__ W, int nw) {
+  FOR_ITEMS(S, W, nw) cell_scatter_body(A, lb);
 }
 
 // Stable order inside a cell (ascending particle index) and the sorted
 // copies.  The cell's first particle also records how many of its particles
 // are solids (they come first): lnsol is read only for non-empty cells.
-__global__ void __launch_bounds__(BLOCK) k_cell_rank(int n, int ns, const uint32_t* __restrict__ phash,
-                                                     const uint32_t* __restrict__ lkey,
-                                                     const int* __restrict__ lstart, const int* __restrict__ tmp,
-                                                     const float4* __restrict__ pos, int* __restrict__ sid,
-                                                     float4* __restrict__ spos, uint32_t* __restrict__ sphash,
-                                                     int* __restrict__ lnsol) {
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t h = phash[i];
+__device__ __forceinline__ void cell_rank_body(const SceneArgs& A, int lb) {
+  const int i = lb * BLOCK + threadIdx.x;
+  if (i >= A.n) return;
+  const uint32_t h = A.phash[i];
   if (h == 0xffffffffu) return;
-  const uint32_t k = lkey[i];
-  const int s = lstart[k], e = lstart[k + 1];
+  const uint32_t k = A.lkey[i];
+  const int s = A.lstart[k], e = A.lstart[k + 1];
   int r = 0, nsol = 0;
   for (int q = s; q < e; q++) {
-    const int j = tmp[q];
+    const int j = A.ltmp[q];
     r += j < i ? 1 : 0;
-    nsol += j < ns ? 1 : 0;
+    nsol += j < A.ns ? 1 : 0;
   }
   const int d = s + r;
-  sid[d] = i;
-  spos[d] = pos[i];
-  sphash[d] = h;
-  if (r == 0) lnsol[k] = nsol;
+  A.sid[d] = i;
+  A.spos[d] = A.pos[i];
+  A.sphash[d] = h;
+  if (r == 0) A.lnsol[k] = nsol;
 }
 
-int fp_launch_cell_grid(fp_solver* s, bool count_done) {
-  const int n = s->n;
+__global__ void __launch_bounds__(BLOCK) kg_cell_rank(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
+                                                      int nw) {
+  FOR_ITEMS(S, W, nw) cell_rank_body(A, lb);
+}
+
+int fp_group_cell_grid(fp_group* g, bool count_done) {
   if (!count_done) {
-    PassScope ps(s, P_HASH_COUNT);
-    hipLaunchKernelGGL(k_cell_count, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, s->cg, n, s->ns, s->pos,
-                       s->active, s->lcount, s->lkey, s->lslot, s->phash, s->sbox);
-  }
-  const int M = (int)s->cg.M;
-  {
-    PassScope ps(s, P_SCAN_REDUCE);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(s->lscan_blocks), dim3(BLOCK), 0, s->stream, M, s->lcount,
-                       s->lblock_sums);
+    PassScope ps(g->prof, g->stream, P_HASH_COUNT);
+    GROUP_LAUNCH(g, kg_cell_count, W_PARTICLE, BLOCK);
   }
   {
-    PassScope ps(s, P_SCAN_APPLY);
-    hipLaunchKernelGGL(k_scan_apply, dim3(s->lscan_blocks), dim3(BLOCK), 0, s->stream, M, s->lcount,
-                       s->lblock_sums, s->lstart, nullptr);
+    PassScope ps(g->prof, g->stream, P_SCAN_REDUCE);
+    GROUP_LAUNCH(g, kg_scan_reduce, W_SCAN, BLOCK);
   }
   {
-    PassScope ps(s, P_SCATTER);
-    hipLaunchKernelGGL(k_cell_scatter, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->phash, s->lkey,
-                       s->lslot, s->lstart, s->ltmp);
+    PassScope ps(g->prof, g->stream, P_SCAN_APPLY);
+    GROUP_LAUNCH(g, kg_scan_apply, W_SCAN, BLOCK);
   }
   {
-    PassScope ps(s, P_RANK);
-    hipLaunchKernelGGL(k_cell_rank, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->ns, s->phash, s->lkey,
-                       s->lstart, s->ltmp, s->pos, s->sid, s->spos, s->sphash, s->lnsol);
+    PassScope ps(g->prof, g->stream, P_SCATTER);
+    GROUP_LAUNCH(g, kg_cell_scatter, W_PARTICLE, BLOCK);
   }
-  FP_TRY(s, hipGetLastError());
-  s->ref_grid_valid = false;
+  {
+    PassScope ps(g->prof, g->stream, P_RANK);
+    GROUP_LAUNCH(g, kg_cell_rank, W_PARTICLE, BLOCK);
+  }
+  FP_GTRY(g, hipGetLastError());
+  for (fp_solver* s : g->scenes) s->ref_grid_valid = false;
   return FP_OK;
 }
 
@@ -242,51 +264,47 @@ int fp_launch_cell_grid(fp_solver* s, bool count_done) {
 // particle index inside a cell) give the fluid order the density,
 // displacement and XSPH passes gather from.
 
-__global__ void __launch_bounds__(BLOCK) k_fluid_scatter(int n, const int* __restrict__ sid,
-                                                         const uint32_t* __restrict__ tkey,
-                                                         const uint32_t* __restrict__ tslot,
-                                                         const int* __restrict__ lstart, int* __restrict__ tmp) {
-  const int e = blockIdx.x * BLOCK + threadIdx.x;
-  if (e >= n) return;
-  const uint32_t k = tkey[e];
+__device__ __forceinline__ void fluid_scatter_body(const SceneArgs& A, int lb) {
+  const int e = lb * BLOCK + threadIdx.x;
+  if (e >= A.n) return;
+  const uint32_t k = A.tkey[e];
   if (k == NO_KEY) return;
-  tmp[lstart[k] + (int)tslot[e]] = sid[e];
+  A.ltmp[A.lstart[k] + (int)A.tslot[e]] = A.sid[e];
 }
 
-__global__ void __launch_bounds__(BLOCK) k_fluid_rank(int n, const int* __restrict__ sid,
-                                                      const uint32_t* __restrict__ tkey,
-                                                      const int* __restrict__ lstart, const int* __restrict__ tmp,
-                                                      const float4* __restrict__ tpos,
-                                                      const uint32_t* __restrict__ sphash,
-                                                      int* __restrict__ fsid, int* __restrict__ fslot,
-                                                      float4* __restrict__ spos2, uint32_t* __restrict__ sphash2) {
-  const int e = blockIdx.x * BLOCK + threadIdx.x;
-  if (e >= n) return;
-  const uint32_t k = tkey[e];
+__global__ void __launch_bounds__(BLOCK) kg_fluid_scatter(const SceneArgs* __restrict__ S,
+                                                          const int2* __restrict__ W, int nw) {
+  FOR_ITEMS(S, W, nw) fluid_scatter_body(A, lb);
+}
+
+__device__ __forceinline__ void fluid_rank_body(const SceneArgs& A, int lb) {
+  const int e = lb * BLOCK + threadIdx.x;
+  if (e >= A.n) return;
+  const uint32_t k = A.tkey[e];
   if (k == NO_KEY) return;
-  const int i = sid[e];
-  const int s = lstart[k], t = lstart[k + 1];
+  const int i = A.sid[e];
+  const int s = A.lstart[k], t = A.lstart[k + 1];
   int r = 0;
-  for (int q = s; q < t; q++) r += tmp[q] < i ? 1 : 0;
+  for (int q = s; q < t; q++) r += A.ltmp[q] < i ? 1 : 0;
   const int d = s + r;
-  fsid[d] = i;
-  fslot[i] = d;
-  spos2[d] = tpos[e];
-  sphash2[d] = sphash[e];
+  A.fsid[d] = i;
+  A.fslot[i] = d;
+  A.spos2[d] = A.tpos[e];
+  A.sphash2[d] = A.sphash[e];
 }
 
-int fp_launch_resort(fp_solver* s) {
-  const int n = s->n;
-  const int M = (int)s->cg.M;
-  PassScope ps(s, P_RESORT);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(s->lscan_blocks), dim3(BLOCK), 0, s->stream, M, s->lcount, s->lblock_sums);
-  hipLaunchKernelGGL(k_scan_apply, dim3(s->lscan_blocks), dim3(BLOCK), 0, s->stream, M, s->lcount, s->lblock_sums,
-                     s->lstart, nullptr);
-  hipLaunchKernelGGL(k_fluid_scatter, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->sid, s->tkey, s->tslot,
-                     s->lstart, s->ltmp);
-  hipLaunchKernelGGL(k_fluid_rank, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->sid, s->tkey, s->lstart,
-                     s->ltmp, s->tpos, s->sphash, s->fsid, s->fslot, s->spos2, s->sphash2);
-  FP_TRY(s, hipGetLastError());
+__global__ void __launch_bounds__(BLOCK) kg_fluid_rank(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
+                                                       int nw) {
+  FOR_ITEMS(S, W, nw) fluid_rank_body(A, lb);
+}
+
+int fp_group_resort(fp_group* g) {
+  PassScope ps(g->prof, g->stream, P_RESORT);
+  GROUP_LAUNCH(g, kg_scan_reduce, W_SCAN, BLOCK);
+  GROUP_LAUNCH(g, kg_scan_apply, W_SCAN, BLOCK);
+  GROUP_LAUNCH(g, kg_fluid_scatter, W_PARTICLE, BLOCK);
+  GROUP_LAUNCH(g, kg_fluid_rank, W_PARTICLE, BLOCK);
+  FP_GTRY(g, hipGetLastError());
   return FP_OK;
 }
 
@@ -348,7 +366,7 @@ int fp_launch_ref_grid(fp_solver* s, bool rehash) {
   hipLaunchKernelGGL(k_ref_count, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->phash, s->count, s->pslot);
   hipLaunchKernelGGL(k_scan_reduce, dim3(s->scan_blocks), dim3(BLOCK), 0, s->stream, s->T, s->count, s->block_sums);
   hipLaunchKernelGGL(k_scan_apply, dim3(s->scan_blocks), dim3(BLOCK), 0, s->stream, s->T, s->count, s->block_sums,
-                     s->cell_start, nullptr);
+                     s->cell_start);
   hipLaunchKernelGGL(k_ref_scatter, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->phash, s->pslot,
                      s->cell_start, s->entries_tmp);
   hipLaunchKernelGGL(k_ref_rank, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, n, s->phash, s->cell_start,

@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -79,6 +80,21 @@ enum PassId {
   P_NUM
 };
 
+// Per-pass timing with HIP events (fp_profile_enable / fp_profile_read).
+struct Prof {
+  bool on = false;
+  struct Ev {
+    int pass;
+    hipEvent_t a, b;
+  };
+  std::vector<Ev> live;
+  std::vector<hipEvent_t> pool;
+  double ms[P_NUM] = {0};
+  int64_t n[P_NUM] = {0};
+};
+
+struct fp_group;
+
 struct fp_solver {
   fp_params P;
   int device = 0;
@@ -121,9 +137,8 @@ struct fp_solver {
   int* block_sums = nullptr;
   // constraints
   int2* cons_ab = nullptr;
-  float* rest[2] = {nullptr, nullptr};
+  float* rest[2] = {nullptr, nullptr};     // [0] is the state between substeps
   uint8_t* alive[2] = {nullptr, nullptr};
-  int cur = 0;
   int* adj_off = nullptr;
   int4* adj = nullptr;
   float4* spos_c[2] = {nullptr, nullptr};
@@ -144,15 +159,11 @@ struct fp_solver {
   bool steps_pending = false;
   bool cut_dirty = false;
   bool cut_dirty_at_step = false;
-  bool profile = false;
-  struct Ev {
-    int pass;
-    hipEvent_t a, b;
-  };
-  std::vector<Ev> ev_live;
-  std::vector<hipEvent_t> ev_pool;
-  double prof_ms[P_NUM] = {0};
-  int64_t prof_n[P_NUM] = {0};
+  Prof prof;
+  hipStream_t own_stream = nullptr;  // the stream fp_create made (stream may be a batch's)
+  fp_group* group = nullptr;         // this scene alone, for fp_step
+  fp_batch* batch = nullptr;         // the batch it belongs to, if any
+  uint64_t version = 1;              // bumped when its parameters change
 };
 
 // ---- per-pass parameter blocks ---------------------------------------------
@@ -179,14 +190,117 @@ struct BoundsParams {
 };
 
 struct ConsParams {
-  int ns;
+  int ns, iters;
   float alpha, omega, break_strain, plastic_thr, plastic_rate;
 };
 
-// ---- host helpers (fp_api.cpp) ---------------------------------------------
+// The fluid order as the fluid passes read it.
+struct PairSource {
+  const float4* __restrict__ spos2;     // fluid order: post-contact xyz, w = index | flags
+  const uint32_t* __restrict__ sphash;  // fluid order: bucket at grid time
+  const int* __restrict__ lstart;       // fluid-order cell starts
+};
+
+// The pair list the density pass records (see fp_fluid.hip).
+struct PairList {
+  uint32_t* __restrict__ nbr;
+  int* __restrict__ ncount;
+  int stride, cap;
+};
+
+// The grid order as the contact pass reads it.
+struct ContactCtx {
+  const float4* __restrict__ spos;  // grid order: xyz + inverse mass
+  const int* __restrict__ sid;
+  const int* __restrict__ lstart;   // grid-order cell starts
+  const int* __restrict__ lnsol;    // solids of each non-empty cell (they come first)
+  const int* __restrict__ adj_off;
+  const int4* __restrict__ adj;
+  const uint8_t* __restrict__ alive;
+  const int* __restrict__ sbox;  // solids' bounding box at grid time (ordered-int bits)
+};
+
+// Everything a substep's kernels need about one scene, in device memory: the
+// kernels of a launch group read their scene's entry through a work list of
+// (scene, block of that scene) items, so one launch serves every scene.
+struct SceneArgs {
+  FluidParams fp;
+  PairSource src;
+  PairList pl;
+  ContactCtx cc;
+  BoundsParams bp;
+  ConsParams cp;
+  int n, ns, C;
+  float gvx, gvy, gvz, h;  // gravity * h, substep
+  float4 *pos, *prev, *vel;
+  uint8_t* active;
+  float2* rho;
+  uint32_t* phash;
+  int *lcount, *lstart, *lblock_sums, *ltmp, *lnsol;
+  uint32_t *lkey, *lslot;
+  int* sid;
+  float4* spos;
+  uint32_t* sphash;
+  float4* tpos;
+  uint32_t *tkey, *tslot;
+  int *fsid, *fslot;
+  float4 *spos2, *svtmp;
+  uint32_t* sphash2;
+  float2* sP;
+  int* sbox;
+  int* dflags;
+  float* rest[2];
+  uint8_t* alive[2];
+  int* adj_off;
+  int4* adj;
+  float4* spos_c[2];
+};
+
+// Work lists: the blocks one scene needs in a pass of each kind.
+enum WorkKind {
+  W_PARTICLE,  // one thread per particle slot (BLOCK per block)
+  W_SCAN,      // one scan tile of the cell table
+  W_SOLID,     // CONTACT_LANES threads per solid
+  W_FLUID,     // FLUID_LANES threads per particle slot (FLUID_BLOCK per block)
+  W_CONS,      // 32 threads per solid
+  W_CONSTR,    // one thread per constraint
+  W_NUM
+};
+
+// A set of scenes stepped together: one launch per pass over all of them
+// (a solver alone is a group of one).  Owns the device scene table, the work
+// lists and the captured frame graph.
+struct fp_group {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Prof* prof = nullptr;
+  std::vector<fp_solver*> scenes;
+  std::vector<uint64_t> versions;  // scene versions the table was built from
+  SceneArgs* d_args = nullptr;
+  int args_cap = 0;
+  int2* d_items[W_NUM] = {nullptr};
+  int n_items[W_NUM] = {0};
+  int items_cap[W_NUM] = {0};
+  int grid_cap[W_NUM] = {0};  // persistent grid size per kind (0 = one block per item)
+  double built_h = -1.0;
+  bool dirty = true;
+  int iters = 0;
+  bool any_grabs = false;
+  // captured frame
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  double graph_dt = -1.0;
+  int graph_sub = 0;
+  int graph_iters = -1;
+  bool use_graph = true;
+  int64_t graph_launches = 0;
+  std::string err;
+};
+
+// ---- host helpers (fp_api.hip) ---------------------------------------------
 
 int fp_fail(fp_solver* s, int code, const std::string& msg);
-hipEvent_t fp_event_get(fp_solver* s);
+hipEvent_t fp_event_get(Prof* p);
 
 #define FP_TRY(s, expr)                                                                 \
   do {                                                                                  \
@@ -194,49 +308,86 @@ hipEvent_t fp_event_get(fp_solver* s);
     if (e_ != hipSuccess) return fp_fail((s), FP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+#define FP_GTRY(g, expr)                                                                \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      (g)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                     \
+      return FP_EDEVICE;                                                                \
+    }                                                                                   \
+  } while (0)
+
 // Brackets the launches of one pass with events when profiling is on.
 struct PassScope {
-  fp_solver* s;
+  Prof* p;
+  hipStream_t stream;
   int pass;
   hipEvent_t a = nullptr, b = nullptr;
-  PassScope(fp_solver* s_, int p) : s(s_), pass(p) {
-    if (s->profile) {
-      a = fp_event_get(s);
-      b = fp_event_get(s);
-      if (a) (void)hipEventRecord(a, s->stream);
+  PassScope(Prof* p_, hipStream_t st, int ps) : p(p_), stream(st), pass(ps) {
+    if (p && p->on) {
+      a = fp_event_get(p);
+      b = fp_event_get(p);
+      if (a) (void)hipEventRecord(a, stream);
     }
   }
+  PassScope(fp_solver* s, int ps) : PassScope(&s->prof, s->stream, ps) {}
   ~PassScope() {
-    if (s->profile && a && b) {
-      (void)hipEventRecord(b, s->stream);
-      s->ev_live.push_back({pass, a, b});
+    if (p && p->on && a && b) {
+      (void)hipEventRecord(b, stream);
+      p->live.push_back({pass, a, b});
     }
   }
 };
 
 static inline int fp_blocks(int n) { return (n + BLOCK - 1) / BLOCK; }
 
-// ---- launchers -------------------------------------------------------------
+// Launch of a group kernel over work kind k: grid = the items (or the
+// persistent cap), the kernel strides over them.
+#define GROUP_LAUNCH(g, kernel, kind, block, ...)                                                   \
+  do {                                                                                               \
+    const int n_ = (g)->n_items[kind];                                                               \
+    if (n_ > 0) {                                                                                    \
+      const int grid_ = (g)->grid_cap[kind] > 0 ? std::min(n_, (g)->grid_cap[kind]) : n_;          \
+      hipLaunchKernelGGL(kernel, dim3(grid_), dim3(block), 0, (g)->stream, (g)->d_args, (g)->d_items[kind], n_, \
+                         ##__VA_ARGS__);                                                             \
+    }                                                                                                \
+  } while (0)
+
+// The (scene, local block) items of a group kernel: for (w ...) { SceneArgs& A; int lb }
+#define FOR_ITEMS(S, W, nw)                                                         \
+  for (int w_ = (int)blockIdx.x; w_ < (nw); w_ += (int)gridDim.x)                   \
+    if (const int2 it_ = (W)[w_]; true)                                             \
+      if (const SceneArgs& A = (S)[it_.x]; true)                                    \
+        if (const int lb = it_.y; true)
+
+// ---- per-scene set-up (fp_fluid.hip, fp_step.hip, fp_solid.hip) -------------
+
+FluidParams fp_fluid_params(const fp_solver* s);
+SceneArgs fp_scene_args(const fp_solver* s, double h);
+int fp_scene_blocks(const fp_solver* s, int kind);
+
+// ---- group launches (one per pass, all scenes) -----------------------------
 
 // fp_grid.hip
-int fp_launch_cell_grid(fp_solver* s, bool count_done);
-int fp_launch_resort(fp_solver* s);
-int fp_launch_ref_grid(fp_solver* s, bool rehash);
-
+int fp_group_cell_grid(fp_group* g, bool count_done);
+int fp_group_resort(fp_group* g);
 // fp_fluid.hip
-FluidParams fp_fluid_params(const fp_solver* s);
-int fp_launch_contacts(fp_solver* s);
-int fp_launch_density(fp_solver* s);
-int fp_launch_displace(fp_solver* s);
-int fp_launch_xsph(fp_solver* s);
-
+int fp_group_contacts(fp_group* g);
+int fp_group_density(fp_group* g);
+int fp_group_displace(fp_group* g);
+int fp_group_xsph(fp_group* g);
 // fp_solid.hip
-int fp_launch_constraints(fp_solver* s, double h, const float4** solid_src);
+int fp_group_constraints(fp_group* g, int iters);
+// fp_step.hip
+int fp_group_frame(fp_group* g, double dt, int substeps);
+int fp_group_prepare(fp_group* g, double h);
+void fp_group_free(fp_group* g);
+
+// ---- single-scene launches (events, queries) -------------------------------
+
+int fp_launch_ref_grid(fp_solver* s, bool rehash);
 int fp_launch_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[3], double nx, double ny,
                   double nz);
 int fp_launch_pick(fp_solver* s, const double o[3], const double d[3], double max_perp2);
-
-// fp_step.hip
-int fp_launch_substep(fp_solver* s, double h);
 int fp_launch_emit(fp_solver* s, int m);
 int fp_launch_count_active(fp_solver* s);

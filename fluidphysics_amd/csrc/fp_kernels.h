@@ -131,29 +131,31 @@ __device__ __forceinline__ int probe_multiplicity(const Cell& c, uint32_t mask, 
 
 // ---- the cell grid of the hot path -------------------------------------------
 
-// Key of cell c: (cx * NY*NZ + cy * NZ + cz) mod M with 32-bit wrap; the three
-// z-neighbours of a cell are consecutive keys.  Coordinates beyond +-2^30
-// (and NaN) share one sentinel cell, so such particles still meet each other.
+// Cell keys of the hot path's counting sort.  Coordinates beyond +-2^30 (and
+// NaN) share one sentinel cell, so such particles still meet each other.
 struct CellGrid {
   double inv_cell;
   uint32_t hmask;      // reference hash table: T - 1
   uint32_t M, kmask;   // cell table size (power of 2), M - 1
-  uint32_t nz_shift;   // log2 NZ
-  uint32_t nyz_shift;  // log2 (NY * NZ)
+  uint32_t mx, my;     // key multipliers of the x and y cell coordinates
 };
 
 __device__ __forceinline__ uint32_t cell_coord_u32(double c) {
   return fabs(c) < 1073741824.0 ? (uint32_t)(int32_t)c : 0x40000000u;
 }
 
+// key = (cx * mx + cy * my + cz) mod M (32-bit wrap): the three z-neighbours
+// of a cell are consecutive keys.  With mx = NY*NZ, my = NZ and M covering
+// the walls every cell of the basin has its own key; with odd multipliers
+// and a smaller M (small scenes) columns of cells land at scattered offsets
+// and the few that overlap are told apart by the distance tests.
 __device__ __forceinline__ uint32_t cell_key(const CellGrid& g, const Cell& c) {
-  return ((cell_coord_u32(c.x) << g.nyz_shift) + (cell_coord_u32(c.y) << g.nz_shift) + cell_coord_u32(c.z)) &
-         g.kmask;
+  return (cell_coord_u32(c.x) * g.mx + cell_coord_u32(c.y) * g.my + cell_coord_u32(c.z)) & g.kmask;
 }
 
 // First key of the z-run of neighbour row (a, b) in {-1,0,1}^2 of key k.
 __device__ __forceinline__ uint32_t row_key(const CellGrid& g, uint32_t k, int a, int b) {
-  return (k + ((uint32_t)a << g.nyz_shift) + ((uint32_t)b << g.nz_shift) - 1u) & g.kmask;
+  return (k + (uint32_t)a * g.mx + (uint32_t)b * g.my - 1u) & g.kmask;
 }
 
 // ---- solids' bounding box ------------------------------------------------------

@@ -21,15 +21,21 @@
 
 #define CLANES 32
 
-__global__ void __launch_bounds__(BLOCK) k_constraints(ConsParams cp, const float4* __restrict__ sin,
-                                                       float4* __restrict__ sout, const int* __restrict__ adj_off,
-                                                       const int4* __restrict__ adj,
-                                                       const float* __restrict__ rest_in,
-                                                       const uint8_t* __restrict__ alive_in,
-                                                       float* __restrict__ rest_out, uint8_t* __restrict__ alive_out,
-                                                       int* __restrict__ dflags) {
+// One Jacobi iteration `it` of one scene: positions from pos (it = 0) or
+// spos_c[(it - 1) & 1] into spos_c[it & 1]; rest / alive from buffer it & 1
+// into buffer (it + 1) & 1 (every substep starts from buffer 0).
+__device__ __forceinline__ void constraints_body(const SceneArgs& A, int lb, int it) {
+  const ConsParams& cp = A.cp;
+  const float4* __restrict__ sin = it == 0 ? A.pos : A.spos_c[(it - 1) & 1];
+  float4* __restrict__ sout = A.spos_c[it & 1];
+  const float* __restrict__ rest_in = A.rest[it & 1];
+  const uint8_t* __restrict__ alive_in = A.alive[it & 1];
+  float* __restrict__ rest_out = A.rest[(it + 1) & 1];
+  uint8_t* __restrict__ alive_out = A.alive[(it + 1) & 1];
+  const int* __restrict__ adj_off = A.adj_off;
+  const int4* __restrict__ adj = A.adj;
   const int lane = threadIdx.x & (CLANES - 1);
-  const int a = (blockIdx.x * BLOCK + threadIdx.x) / CLANES;
+  const int a = (lb * BLOCK + (int)threadIdx.x) / CLANES;
   const bool valid = a < cp.ns;
   float4 pa = make_float4(0.f, 0.f, 0.f, 0.f);
   int k0 = 0, k1 = 0;
@@ -84,7 +90,26 @@ __global__ void __launch_bounds__(BLOCK) k_constraints(ConsParams cp, const floa
     cz += __shfl_xor(cz, off, CLANES);
   }
   if (valid && lane == 0) sout[a] = make_float4(pa.x + cx, pa.y + cy, pa.z + cz, pa.w);
-  if (tore) dflags[DF_TEAR] = 1;
+  if (tore) A.dflags[DF_TEAR] = 1;
+}
+
+__global__ void __launch_bounds__(BLOCK) kg_constraints(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
+                                                        int nw, int it) {
+  FOR_ITEMS(S, W, nw) constraints_body(A, lb, it);
+}
+
+// After an odd number of iterations the constraints' state is in buffer 1:
+// copy it back so every substep (and a captured frame) starts from buffer 0.
+__device__ __forceinline__ void constraints_back_body(const SceneArgs& A, int lb) {
+  const int c = lb * BLOCK + threadIdx.x;
+  if (c >= A.C) return;
+  A.rest[0][c] = A.rest[1][c];
+  A.alive[0][c] = A.alive[1][c];
+}
+
+__global__ void __launch_bounds__(BLOCK) kg_constraints_back(const SceneArgs* __restrict__ S,
+                                                             const int2* __restrict__ W, int nw) {
+  FOR_ITEMS(S, W, nw) constraints_back_body(A, lb);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_cut(int nc, const float4* __restrict__ pos, const int2* __restrict__ ab,
@@ -149,31 +174,14 @@ __global__ void __launch_bounds__(BLOCK) k_pick(int ns, const float4* __restrict
   }
 }
 
-// iters Jacobi iterations; *solid_src receives the buffer holding the
-// solids' final positions (pos itself when there is nothing to solve).
-int fp_launch_constraints(fp_solver* s, double h, const float4** solid_src) {
-  *solid_src = s->pos;
-  if (s->ns == 0 || s->C == 0) return FP_OK;
-  const fp_params& P = s->P;
-  const int iters = P.constraint_iters > 0 ? P.constraint_iters : 1;
-  ConsParams cp;
-  cp.ns = s->ns;
-  cp.alpha = (float)(P.compliance / (h * h));
-  cp.omega = (float)P.jacobi_omega;
-  cp.break_strain = (float)P.break_strain;
-  cp.plastic_thr = (float)P.plastic_threshold;
-  cp.plastic_rate = (float)P.plastic_rate;
-  const int blocks = (int)(((int64_t)s->ns * CLANES + BLOCK - 1) / BLOCK);
-  PassScope ps(s, P_CONSTRAINTS);
-  for (int it = 0; it < iters; it++) {
-    const float4* in = it == 0 ? s->pos : s->spos_c[(it - 1) & 1];
-    float4* out = s->spos_c[it & 1];
-    hipLaunchKernelGGL(k_constraints, dim3(blocks), dim3(BLOCK), 0, s->stream, cp, in, out, s->adj_off, s->adj,
-                       s->rest[s->cur], s->alive[s->cur], s->rest[s->cur ^ 1], s->alive[s->cur ^ 1], s->dflags);
-    s->cur ^= 1;
-    *solid_src = out;
-  }
-  FP_TRY(s, hipGetLastError());
+// iters Jacobi iterations for every scene of the group (the solids' final
+// positions end in spos_c[(iters - 1) & 1]; see fp_group_frame).
+int fp_group_constraints(fp_group* g, int iters) {
+  if (iters < 1) return FP_OK;
+  PassScope ps(g->prof, g->stream, P_CONSTRAINTS);
+  for (int it = 0; it < iters; it++) GROUP_LAUNCH(g, kg_constraints, W_CONS, BLOCK, it);
+  if (iters & 1) GROUP_LAUNCH(g, kg_constraints_back, W_CONSTR, BLOCK);
+  FP_GTRY(g, hipGetLastError());
   return FP_OK;
 }
 
@@ -181,7 +189,7 @@ int fp_launch_cut(fp_solver* s, const double o[3], const double d1[3], const dou
                   double nz) {
   PassScope ps(s, P_CUT);
   hipLaunchKernelGGL(k_cut, dim3(fp_blocks(s->C)), dim3(BLOCK), 0, s->stream, s->C, s->pos, s->cons_ab,
-                     s->alive[s->cur], o[0], o[1], o[2], d1[0], d1[1], d1[2], d2[0], d2[1], d2[2], nx, ny, nz,
+                     s->alive[0], o[0], o[1], o[2], d1[0], d1[1], d1[2], d2[0], d2[1], d2[2], nx, ny, nz,
                      s->dflags + DF_CUT);
   FP_TRY(s, hipGetLastError());
   return FP_OK;

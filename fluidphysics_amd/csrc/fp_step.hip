@@ -1,68 +1,77 @@
-// fp_step.hip -- the per-particle passes and the order of one substep
-// (the body of the reference's step loop, src/solver.js:192-237):
+// fp_step.hip -- the per-particle passes, the order of one substep (the
+// body of the reference's step loop, src/solver.js:192-237) and the launch
+// groups that run it for one scene or a batch of scenes:
 //
-//   integrate (+ hash count)   v.y += g h; prev = p; p += v h        (:194-202)
+//   integrate (+ hash count)   v += g h; prev = p; p += v h           (:194-202)
 //   grab rounds                p += (target - p) k                    (:205-210)
 //   cell grid                  fp_grid.hip (the reference's hash grid of
 //                              :212 is rebuilt from the kept buckets only
 //                              when read)
 //   contacts                   fp_fluid.hip                            (:213)
+//   fluid order                fp_grid.hip
 //   density, displace          fp_fluid.hip                            (:214)
 //   constraints x iters        fp_solid.hip                            (:215)
 //   bounds + velocity          floor, walls, cull; v = (p - prev)/h,
 //                              damping, speed clamp                   (:216-234)
 //   xsph                       fp_fluid.hip                            (:236)
 //
-// When no grab is active the cell-grid count runs inside the integrate
-// kernel (the cell of the integrated position is known right there).
+// Every pass is one launch over all scenes of a group (fp_internal.h): the
+// kernels stride over (scene, block) items and read the scene's SceneArgs.
+// A frame without grabs is captured once into a HIP graph and replayed.
+#include <cstring>
+
 #include "fp_internal.h"
 #include "fp_kernels.h"
 
 // Integrate (solver.js:194-202); with COUNT the particle is also entered
 // into the cell grid at its integrated position (cell key, arrival slot,
 // reference bucket).  Inactive particles get rho = 0 (the reference clears
-// rho of every fluid slot each substep, :292).
+// rho of every fluid slot each substep, :292).  Gravity acts on y; the x and
+// z components (a tilted basin, batched scenes) are added only when set.
 template <bool COUNT>
-__global__ void __launch_bounds__(BLOCK) k_integrate(CellGrid cg, int n, int ns, float4* __restrict__ pos,
-                                                     float4* __restrict__ prev, float4* __restrict__ vel,
-                                                     const uint8_t* __restrict__ active, float gh, float h,
-                                                     int* __restrict__ lcount, uint32_t* __restrict__ lkey,
-                                                     uint32_t* __restrict__ lslot, uint32_t* __restrict__ phash,
-                                                     float2* __restrict__ rho, int* __restrict__ sbox) {
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
-  const bool in = i < n;
-  const bool act = in && active[i];
+__device__ __forceinline__ void integrate_body(const SceneArgs& A, int lb) {
+  const int i = lb * BLOCK + threadIdx.x;
+  const bool in = i < A.n;
+  const bool act = in && A.active[i];
   float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t k = 0;
   if (act) {
-    float4 v = vel[i];
-    p = pos[i];
-    v.y += gh;
-    prev[i] = make_float4(p.x, p.y, p.z, 0.f);
-    p.x = fmaf(v.x, h, p.x);
-    p.y = fmaf(v.y, h, p.y);
-    p.z = fmaf(v.z, h, p.z);
-    pos[i] = p;
-    vel[i] = v;
+    float4 v = A.vel[i];
+    p = A.pos[i];
+    v.y += A.gvy;
+    if (A.gvx != 0.f) v.x += A.gvx;
+    if (A.gvz != 0.f) v.z += A.gvz;
+    A.prev[i] = make_float4(p.x, p.y, p.z, 0.f);
+    p.x = fmaf(v.x, A.h, p.x);
+    p.y = fmaf(v.y, A.h, p.y);
+    p.z = fmaf(v.z, A.h, p.z);
+    A.pos[i] = p;
+    A.vel[i] = v;
     if (COUNT) {
-      const Cell c = cell_of(p.x, p.y, p.z, cg.inv_cell);
-      phash[i] = cell_hash(c, cg.hmask);
-      k = cell_key(cg, c);
-      lkey[i] = k;
+      const Cell c = cell_of(p.x, p.y, p.z, A.fp.cg.inv_cell);
+      A.phash[i] = cell_hash(c, A.fp.cg.hmask);
+      k = cell_key(A.fp.cg, c);
+      A.lkey[i] = k;
     }
   } else if (in) {
-    if (COUNT) phash[i] = 0xffffffffu;
-    rho[i] = make_float2(0.f, 0.f);
+    if (COUNT) A.phash[i] = 0xffffffffu;
+    A.rho[i] = make_float2(0.f, 0.f);
   }
   if (COUNT) {
     // neighbouring indices are often in one cell: one atomic per run
-    const uint32_t sl = wave_bucket_slot(lcount, k, act);
-    if (act) lslot[i] = sl;
-    box_update(act && i < ns, p.x, p.y, p.z, sbox);
+    const uint32_t sl = wave_bucket_slot(A.lcount, k, act);
+    if (act) A.lslot[i] = sl;
+    box_update(act && i < A.ns, p.x, p.y, p.z, A.sbox);
   }
 }
 
-// One round of grab pulls; indices inside a round are distinct.
+template <bool COUNT>
+__global__ void __launch_bounds__(BLOCK) kg_integrate(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
+                                                      int nw) {
+  FOR_ITEMS(S, W, nw) integrate_body<COUNT>(A, lb);
+}
+
+// One round of grab pulls of one scene; indices inside a round are distinct.
 __global__ void __launch_bounds__(BLOCK) k_grab(int n, const int* __restrict__ idx, const float4* __restrict__ tgt,
                                                 float4* __restrict__ pos, float k) {
   const int g = blockIdx.x * BLOCK + threadIdx.x;
@@ -80,24 +89,23 @@ __global__ void __launch_bounds__(BLOCK) k_grab(int n, const int* __restrict__ i
 __device__ __forceinline__ float sign_or_one(float q) { return q < 0.f ? -1.f : 1.f; }
 
 // Bounds (solver.js:389-427) then velocity + damping + clamp (:219-234).
-// Solids read their positions from `solid_src` (the last constraint
-// iteration) and write vel; fluid velocities go to the fluid-order slot in svtmp,
-// where XSPH reads them (a particle culled here leaves its velocity as it
-// was, in vel and in svtmp).
-__global__ void __launch_bounds__(BLOCK) k_bounds_velocity(BoundsParams bp, float4* __restrict__ pos,
-                                                           const float4* __restrict__ solid_src,
-                                                           const float4* __restrict__ prev, float4* __restrict__ vel,
-                                                           float4* __restrict__ svtmp, const int* __restrict__ slot,
-                                                           uint8_t* __restrict__ active) {
-  const int i = blockIdx.x * BLOCK + threadIdx.x;
+// Solids read their positions from the last constraint iteration and write
+// vel; fluid velocities go to the fluid-order slot in svtmp, where XSPH
+// reads them (a particle culled here leaves its velocity as it was, in vel
+// and in svtmp).
+__device__ __forceinline__ void bounds_velocity_body(const SceneArgs& A, int lb) {
+  const BoundsParams& bp = A.bp;
+  const float4* __restrict__ solid_src = A.cp.iters > 0 ? A.spos_c[(A.cp.iters - 1) & 1] : A.pos;
+  float4* __restrict__ pos = A.pos;
+  const int i = lb * BLOCK + threadIdx.x;
   if (i >= bp.n) return;
   const bool solid = i < bp.ns;
-  if (!active[i]) {
+  if (!A.active[i]) {
     if (solid) pos[i] = solid_src[i];  // the reference's constraint sweep moves inactive solids too
     return;
   }
   float4 p = solid ? solid_src[i] : pos[i];
-  const float4 pv = prev[i];
+  const float4 pv = A.prev[i];
   const float r = solid ? bp.rs : bp.rf;
   if (p.y < r) {
     p.y = r;
@@ -118,8 +126,8 @@ __global__ void __launch_bounds__(BLOCK) k_bounds_velocity(BoundsParams bp, floa
   }
   pos[i] = p;
   if (!solid && (p.y < -3.f || p.x * p.x + p.z * p.z > bp.cull_r2)) {
-    active[i] = 0;
-    svtmp[slot[i]] = vel[i];
+    A.active[i] = 0;
+    A.svtmp[A.fslot[i]] = A.vel[i];
     return;
   }
   float vx = (p.x - pv.x) * bp.inv_h;
@@ -138,9 +146,14 @@ __global__ void __launch_bounds__(BLOCK) k_bounds_velocity(BoundsParams bp, floa
   }
   const float4 v = make_float4(vx, vy, vz, 0.f);
   if (solid)
-    vel[i] = v;
+    A.vel[i] = v;
   else
-    svtmp[slot[i]] = v;
+    A.svtmp[A.fslot[i]] = v;
+}
+
+__global__ void __launch_bounds__(BLOCK) kg_bounds_velocity(const SceneArgs* __restrict__ S,
+                                                            const int2* __restrict__ W, int nw) {
+  FOR_ITEMS(S, W, nw) bounds_velocity_body(A, lb);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_count_active(int begin, int end, const uint8_t* __restrict__ active,
@@ -166,6 +179,8 @@ __global__ void __launch_bounds__(BLOCK) k_emit(int n, const int* __restrict__ i
   active[i] = 1;
 }
 
+// ---- scene arguments and launch groups ---------------------------------------
+
 static BoundsParams bounds_params(const fp_solver* s, double h) {
   const fp_params& P = s->P;
   BoundsParams bp;
@@ -187,47 +202,233 @@ static BoundsParams bounds_params(const fp_solver* s, double h) {
   return bp;
 }
 
-int fp_launch_substep(fp_solver* s, double h) {
+SceneArgs fp_scene_args(const fp_solver* s, double h) {
   const fp_params& P = s->P;
-  const int n = s->n;
-  const bool grabs = !s->grab_round_len.empty();
-  {
-    PassScope ps(s, P_INTEGRATE);
-    if (grabs)
-      hipLaunchKernelGGL(k_integrate<false>, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, s->cg, n, s->ns, s->pos,
-                         s->prev, s->vel, s->active, (float)(P.gravity * h), (float)h, s->lcount, s->lkey, s->lslot,
-                         s->phash, s->rho, s->sbox);
-    else
-      hipLaunchKernelGGL(k_integrate<true>, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, s->cg, n, s->ns, s->pos,
-                         s->prev, s->vel, s->active, (float)(P.gravity * h), (float)h, s->lcount, s->lkey, s->lslot,
-                         s->phash, s->rho, s->sbox);
+  SceneArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.fp = fp_fluid_params(s);
+  a.src.spos2 = s->spos2;
+  a.src.sphash = s->sphash2;
+  a.src.lstart = s->lstart;
+  a.pl.nbr = s->nbr;
+  a.pl.ncount = s->ncount;
+  a.pl.stride = s->n * FLUID_LANES;
+  a.pl.cap = s->pair_slots / FLUID_LANES;
+  a.cc = ContactCtx{s->spos, s->sid, s->lstart, s->lnsol, s->adj_off, s->adj, s->alive[0], s->sbox};
+  a.bp = bounds_params(s, h);
+  a.cp.ns = s->ns;
+  a.cp.iters = (s->ns > 0 && s->C > 0) ? std::max(1, (int)P.constraint_iters) : 0;
+  a.cp.alpha = (float)(P.compliance / (h * h));
+  a.cp.omega = (float)P.jacobi_omega;
+  a.cp.break_strain = (float)P.break_strain;
+  a.cp.plastic_thr = (float)P.plastic_threshold;
+  a.cp.plastic_rate = (float)P.plastic_rate;
+  a.n = s->n;
+  a.ns = s->ns;
+  a.C = s->C;
+  a.gvx = (float)(P.gravity_x * h);
+  a.gvy = (float)(P.gravity * h);
+  a.gvz = (float)(P.gravity_z * h);
+  a.h = (float)h;
+  a.pos = s->pos;
+  a.prev = s->prev;
+  a.vel = s->vel;
+  a.active = s->active;
+  a.rho = s->rho;
+  a.phash = s->phash;
+  a.lcount = s->lcount;
+  a.lstart = s->lstart;
+  a.lblock_sums = s->lblock_sums;
+  a.ltmp = s->ltmp;
+  a.lnsol = s->lnsol;
+  a.lkey = s->lkey;
+  a.lslot = s->lslot;
+  a.sid = s->sid;
+  a.spos = s->spos;
+  a.sphash = s->sphash;
+  a.tpos = s->tpos;
+  a.tkey = s->tkey;
+  a.tslot = s->tslot;
+  a.fsid = s->fsid;
+  a.fslot = s->fslot;
+  a.spos2 = s->spos2;
+  a.svtmp = s->svtmp;
+  a.sphash2 = s->sphash2;
+  a.sP = s->sP;
+  a.sbox = s->sbox;
+  a.dflags = s->dflags;
+  a.rest[0] = s->rest[0];
+  a.rest[1] = s->rest[1];
+  a.alive[0] = s->alive[0];
+  a.alive[1] = s->alive[1];
+  a.adj_off = s->adj_off;
+  a.adj = s->adj;
+  a.spos_c[0] = s->spos_c[0];
+  a.spos_c[1] = s->spos_c[1];
+  return a;
+}
+
+void fp_group_free(fp_group* g) {
+  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->gexec = nullptr;
+  g->graph = nullptr;
+  if (g->d_args) (void)hipFree(g->d_args);
+  g->d_args = nullptr;
+  g->args_cap = 0;
+  for (int k = 0; k < W_NUM; k++) {
+    if (g->d_items[k]) (void)hipFree(g->d_items[k]);
+    g->d_items[k] = nullptr;
+    g->items_cap[k] = 0;
+    g->n_items[k] = 0;
   }
-  if (grabs) {
-    PassScope ps(s, P_GRAB);
-    int off = 0;
-    for (int len : s->grab_round_len) {
-      hipLaunchKernelGGL(k_grab, dim3(fp_blocks(len)), dim3(BLOCK), 0, s->stream, len, s->grab_idx + off,
-                         s->grab_tgt + off, s->pos, (float)P.grab_stiffness);
-      off += len;
+}
+
+static void drop_graph(fp_group* g) {
+  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->gexec = nullptr;
+  g->graph = nullptr;
+  g->graph_sub = 0;
+}
+
+// Scene table and work lists for substep h: rebuilt when the members, their
+// parameters or h changed.  A changed membership also drops the graph.
+int fp_group_prepare(fp_group* g, double h) {
+  bool changed = g->dirty || g->built_h != h || g->versions.size() != g->scenes.size();
+  if (!changed)
+    for (size_t k = 0; k < g->scenes.size(); k++)
+      if (g->versions[k] != g->scenes[k]->version) changed = true;
+  g->any_grabs = false;
+  for (fp_solver* s : g->scenes) g->any_grabs |= !s->grab_round_len.empty();
+  if (!changed) return FP_OK;
+  const int ns = (int)g->scenes.size();
+  std::vector<SceneArgs> args(ns);
+  g->iters = -1;
+  for (int k = 0; k < ns; k++) {
+    args[k] = fp_scene_args(g->scenes[k], h);
+    if (g->iters < 0) g->iters = args[k].cp.iters;
+    if (args[k].cp.iters != g->iters && args[k].cp.iters > 0) {
+      if (g->iters == 0) {
+        g->iters = args[k].cp.iters;
+      } else {
+        g->err = "the scenes of a batch must use the same constraint_iters";
+        return FP_EINVAL;
+      }
     }
   }
-  int rc = fp_launch_cell_grid(s, !grabs);
-  if (rc) return rc;
-  if ((rc = fp_launch_contacts(s))) return rc;
-  if ((rc = fp_launch_resort(s))) return rc;
-  if ((rc = fp_launch_density(s))) return rc;
-  if ((rc = fp_launch_displace(s))) return rc;
-  const float4* solid_src = nullptr;
-  if ((rc = fp_launch_constraints(s, h, &solid_src))) return rc;
-  {
-    const BoundsParams bp = bounds_params(s, h);
-    PassScope ps(s, P_BOUNDS_VELOCITY);
-    hipLaunchKernelGGL(k_bounds_velocity, dim3(fp_blocks(n)), dim3(BLOCK), 0, s->stream, bp, s->pos, solid_src,
-                       s->prev, s->vel, s->svtmp, s->fslot, s->active);
+  if (g->iters < 0) g->iters = 0;
+  if (ns > g->args_cap) {
+    if (g->d_args) (void)hipFree(g->d_args);
+    g->d_args = nullptr;
+    FP_GTRY(g, hipMalloc((void**)&g->d_args, sizeof(SceneArgs) * std::max(ns, 1)));
+    g->args_cap = ns;
+    drop_graph(g);
   }
-  if ((rc = fp_launch_xsph(s))) return rc;
-  FP_TRY(s, hipGetLastError());
-  s->substeps_done++;
+  if (ns) {
+    FP_GTRY(g, hipMemcpyAsync(g->d_args, args.data(), sizeof(SceneArgs) * ns, hipMemcpyHostToDevice, g->stream));
+  }
+  if (g->dirty || g->versions.size() != g->scenes.size()) {
+    drop_graph(g);
+    for (int kind = 0; kind < W_NUM; kind++) {
+      std::vector<int2> items;
+      for (int k = 0; k < ns; k++) {
+        const int nb = fp_scene_blocks(g->scenes[k], kind);
+        for (int b = 0; b < nb; b++) items.push_back(make_int2(k, b));
+      }
+      const int n = (int)items.size();
+      if (n > g->items_cap[kind]) {
+        if (g->d_items[kind]) (void)hipFree(g->d_items[kind]);
+        g->d_items[kind] = nullptr;
+        FP_GTRY(g, hipMalloc((void**)&g->d_items[kind], sizeof(int2) * std::max(n, 1)));
+        g->items_cap[kind] = n;
+      }
+      if (n) {
+        FP_GTRY(g, hipMemcpyAsync(g->d_items[kind], items.data(), sizeof(int2) * n, hipMemcpyHostToDevice,
+                                  g->stream));
+      }
+      g->n_items[kind] = n;
+    }
+  }
+  // the host vectors are read by the copies above before they go out of scope
+  FP_GTRY(g, hipStreamSynchronize(g->stream));
+  g->versions.resize(ns);
+  for (int k = 0; k < ns; k++) g->versions[k] = g->scenes[k]->version;
+  g->built_h = h;
+  g->dirty = false;
+  return FP_OK;
+}
+
+// One substep of every scene of the group.
+static int group_substep(fp_group* g) {
+  int rc;
+  {
+    PassScope ps(g->prof, g->stream, P_INTEGRATE);
+    if (g->any_grabs)
+      GROUP_LAUNCH(g, kg_integrate<false>, W_PARTICLE, BLOCK);
+    else
+      GROUP_LAUNCH(g, kg_integrate<true>, W_PARTICLE, BLOCK);
+  }
+  if (g->any_grabs) {
+    PassScope ps(g->prof, g->stream, P_GRAB);
+    for (fp_solver* s : g->scenes) {
+      int off = 0;
+      for (int len : s->grab_round_len) {
+        hipLaunchKernelGGL(k_grab, dim3(fp_blocks(len)), dim3(BLOCK), 0, g->stream, len, s->grab_idx + off,
+                           s->grab_tgt + off, s->pos, (float)s->P.grab_stiffness);
+        off += len;
+      }
+    }
+  }
+  if ((rc = fp_group_cell_grid(g, !g->any_grabs))) return rc;
+  if ((rc = fp_group_contacts(g))) return rc;
+  if ((rc = fp_group_resort(g))) return rc;
+  if ((rc = fp_group_density(g))) return rc;
+  if ((rc = fp_group_displace(g))) return rc;
+  if ((rc = fp_group_constraints(g, g->iters))) return rc;
+  {
+    PassScope ps(g->prof, g->stream, P_BOUNDS_VELOCITY);
+    GROUP_LAUNCH(g, kg_bounds_velocity, W_PARTICLE, BLOCK);
+  }
+  if ((rc = fp_group_xsph(g))) return rc;
+  FP_GTRY(g, hipGetLastError());
+  return FP_OK;
+}
+
+// One frame (substeps of dt / substeps) of every scene of the group, queued
+// on the group's stream.  Without grabs or profiling the frame is captured
+// into a HIP graph once and replayed.
+int fp_group_frame(fp_group* g, double dt, int substeps) {
+  const double h = dt / substeps;
+  int rc = fp_group_prepare(g, h);
+  if (rc) return rc;
+  const bool profiling = g->prof && g->prof->on;
+  const bool graphable = g->use_graph && !g->any_grabs && !profiling;
+  if (graphable) {
+    if (!g->gexec || g->graph_sub != substeps || g->graph_iters != g->iters) {
+      drop_graph(g);
+      FP_GTRY(g, hipStreamBeginCapture(g->stream, hipStreamCaptureModeRelaxed));
+      int crc = FP_OK;
+      for (int k = 0; k < substeps && crc == FP_OK; k++) crc = group_substep(g);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(g->stream, &graph);
+      if (crc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return crc;
+      }
+      FP_GTRY(g, e);
+      g->graph = graph;
+      FP_GTRY(g, hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0));
+      g->graph_sub = substeps;
+      g->graph_iters = g->iters;
+    }
+    FP_GTRY(g, hipGraphLaunch(g->gexec, g->stream));
+    g->graph_launches++;
+  } else {
+    for (int k = 0; k < substeps; k++)
+      if ((rc = group_substep(g))) return rc;
+  }
+  for (fp_solver* s : g->scenes) s->substeps_done += substeps;
   return FP_OK;
 }
 

@@ -11,6 +11,8 @@ the identical inputs).
   plain_cpu_scene   7^3 = 343-particle cube + 2,600 fluid, reference constants
   single_gpu_scene  20^3 = 8,000 cube + 65,536 fluid column, GPU-scale constants
   large_scene       24^3 = 13,824 cube + 262,144 fluid slab + 120/frame hose
+  batched_scene     one of many independent scenes: cube 6^3..20^3, 1k..64k
+                    fluid, tilted gravity, seeded slice wedges
 
 GPU-scale constants follow deriveOpts(dim, maxFluid, 30 mm)
 (src/gpu/main.js:30-54) and the GPU defaults (src/gpu/sim.js:34-63), with the
@@ -209,6 +211,53 @@ def large_scene(frames=600, seed=3):
         hose=dict(per_frame=per_frame, origin=(-2.9, 1.6, -2.9), target=(0.0, 0.4, 0.0), speed=4.0,
                   cone_deg=10.0, seed=seed + 100, radius=0.12),
     )
+
+
+def batched_scene(index, seed=4, frames=600):
+    """Scene `index` of the batched workload: cube side uniform in 6..20,
+    fluid count log-uniform over [1000, 65536], gravity tilted by up to 15
+    degrees towards a uniform azimuth, one or two seeded slice wedges.  Every
+    draw comes from the splitmix64 stream of (seed, index), so a scene does
+    not depend on how many others are generated."""
+    r = 0.03
+    sp = 1.91 * r
+    fs = 0.041
+    u = uniform01(seed * 1000003 + index, 8)
+    dim = 6 + min(14, int(u[0] * 15))
+    nf = int(round(math.exp(math.log(1000.0) + u[1] * (math.log(65536.0) - math.log(1000.0)))))
+    tilt = math.radians(15.0) * u[2]
+    azim = 2.0 * math.pi * u[3]
+    g = 9.8
+    ny = max(4, min(32, int(round(nf ** (1.0 / 3.0)))))
+    nxz = int(math.ceil(math.sqrt(nf / ny)))
+    origin = (0.7, r + 0.01, -(nxz - 1) * fs * 0.5)
+    fp = _jittered_block(nxz, ny, nxz, fs, origin, 0.1 * fs, seed * 7919 + index)[:nf]
+    cy = (dim - 1) * sp / 2 + 0.5
+    center = (0.0, cy, 0.0)
+    rad = (dim - 1) * sp * 0.5
+    wseed = seed * 104729 + index
+    wedges = [_random_wedge(20 + int(u[4] * min(200, frames)), wseed, 0, center, rad)]
+    if u[5] < 0.5:
+        wedges.append(_random_wedge(wedges[0].frame + 30 + int(u[6] * 200), wseed, 1, center, rad))
+    params = gpu_scale_params(r)
+    params.update(gravity=-g * math.cos(tilt), gravity_x=g * math.sin(tilt) * math.cos(azim),
+                  gravity_z=g * math.sin(tilt) * math.sin(azim))
+    return Scene(
+        name="batched_%d_%d_%d" % (index, dim ** 3, nf),
+        params=params,
+        max_fluid=nf,
+        cube=(0.0, cy, 0.0, dim, sp, 100 * r ** 3),
+        fluid_pos=fp,
+        fluid_vel=np.zeros_like(fp),
+        fluid_mass=12 * r ** 3,
+        frames=frames,
+        wedges=[w for w in wedges if w.frame < frames],
+    )
+
+
+def batched_scenes(count=256, seed=4, frames=600):
+    """The batched workload: `count` independent scenes (see batched_scene)."""
+    return [batched_scene(k, seed, frames) for k in range(count)]
 
 
 def hose_burst(hose, frame, spacing):

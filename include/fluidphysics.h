@@ -75,6 +75,9 @@ typedef struct fp_params {
   double jacobi_omega;         /* Jacobi relaxation: each constraint's correction is scaled by
                                   jacobi_omega / max(deg_a, deg_b); default 4.0 */
   double contact_scale;        /* scale on each Jacobi contact correction, 1.0 */
+  double gravity_x;            /* gravity along x and z (a tilted basin; the reference has y only,
+                                  solver.js:15); default 0 */
+  double gravity_z;
 } fp_params;
 
 /* Opaque solver handle (one scene, one HIP stream). */
@@ -219,6 +222,42 @@ typedef struct fp_pass_stat {
 int fp_profile_enable(fp_solver* s, int32_t enable);
 /* Accumulated statistics; *n = number of passes written (<= max). */
 int fp_profile_read(fp_solver* s, fp_pass_stat* out, int32_t max, int32_t* n);
+
+/* ---- batches: many independent scenes, one launch per pass ----------- */
+
+/* No counterpart in the reference (it steps one scene).  A batch steps
+ * every scene added to it with one kernel launch per pass: the kernels walk
+ * a (scene, block) work list over a device table of the scenes' arrays and
+ * parameters, and a frame without grabs is captured once into a HIP graph
+ * and replayed.  Scenes keep their own state and API (emit, cut, grabs,
+ * read_state ...); while in a batch they run on the batch's stream.  Every
+ * scene must use the same constraint_iters. */
+typedef struct fp_batch fp_batch;
+
+typedef struct fp_batch_info {
+  int32_t scenes;
+  int32_t graph;               /* 1 while a captured frame is held */
+  int64_t capacity;            /* particle slots over all scenes */
+  int64_t device_bytes;        /* HBM held by the scenes */
+  int64_t frames;              /* fp_batch_step calls */
+  int64_t graph_launches;      /* frames run by replaying the captured graph */
+} fp_batch_info;
+
+int fp_batch_create(int32_t device, fp_batch** out);
+int fp_batch_destroy(fp_batch* b);
+const char* fp_batch_last_error(const fp_batch* b);
+int fp_batch_add(fp_batch* b, fp_solver* s);
+int fp_batch_remove(fp_batch* b, fp_solver* s);
+/* One frame (substeps of dt/substeps) of every scene; returns after the
+ * frame finished and every scene settled (tears, groups), like fp_step. */
+int fp_batch_step(fp_batch* b, double dt, int32_t substeps);
+int fp_batch_step_async(fp_batch* b, double dt, int32_t substeps);
+int fp_batch_sync(fp_batch* b);
+/* 0: launch the passes each frame instead of replaying a captured graph. */
+int fp_batch_set_graph(fp_batch* b, int32_t enable);
+int fp_batch_get_info(const fp_batch* b, fp_batch_info* out);
+int fp_batch_profile_enable(fp_batch* b, int32_t enable);
+int fp_batch_profile_read(fp_batch* b, fp_pass_stat* out, int32_t max, int32_t* n);
 
 #ifdef __cplusplus
 }

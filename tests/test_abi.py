@@ -58,8 +58,8 @@ def test_params_default_and_error_paths_without_device():
 
 
 def test_params_struct_layout_matches_header():
-    # 24 doubles, 4 int32, 2 doubles
-    assert ctypes.sizeof(_abi.FpParams) == 24 * 8 + 4 * 4 + 2 * 8
+    # 24 doubles, 4 int32, 4 doubles
+    assert ctypes.sizeof(_abi.FpParams) == 24 * 8 + 4 * 4 + 4 * 8
 
 
 def _c_layout(struct, fields, tmp_path):
@@ -76,7 +76,7 @@ def _c_layout(struct, fields, tmp_path):
 
 
 @pytest.mark.parametrize("struct,cls", [("fp_params", "FpParams"), ("fp_info", "FpInfo"),
-                                        ("fp_pass_stat", "FpPassStat")])
+                                        ("fp_pass_stat", "FpPassStat"), ("fp_batch_info", "FpBatchInfo")])
 def test_struct_offsets_match_the_c_compiler(struct, cls, tmp_path):
     """Every field of the ctypes mirror sits where gcc puts it."""
     c = getattr(_abi, cls)

@@ -1,0 +1,207 @@
+"""Batched stepping (fp_batch_*): many independent scenes in one launch per
+pass, a frame captured into a HIP graph and replayed.
+
+Bars:
+  * every scene of a batch is bit-identical, frame by frame, to the same
+    scene stepped alone (same kernels, same per-scene arithmetic; nothing
+    crosses scenes);
+  * graph replay is bit-identical to direct launches, with host work and
+    synchronisations between the replays;
+  * membership changes between frames (remove, add) leave the remaining
+    scenes bit-identical to stepping them alone.
+"""
+import numpy as np
+import pytest
+
+from tests._gpu import log
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 60.0
+SUB = 5
+KEYS = ("pos", "prev", "vel", "active")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _lib():
+    from fluidphysics_amd import _abi
+    _abi.load_library()
+
+
+def _scenes(idx, frames):
+    from fluidphysics_amd import scenes
+    out = []
+    for k in idx:
+        sc = scenes.batched_scene(k, frames=frames)
+        # bring the slice strokes into the short test horizon
+        for j, w in enumerate(sc.wedges):
+            w.frame = 3 + 5 * j + k % 4
+        out.append(sc)
+    return out
+
+
+def _cuts(s, sc, frame):
+    for w in sc.wedges:
+        if w.frame == frame:
+            s.cut(*w.o, *w.d1, *w.d2)
+    g = getattr(sc, "test_grab", None)
+    if g is not None:
+        f0, f1, ids = g
+        if frame == f0:
+            cx, cy, cz = sc.cube[:3]
+            s.grabs = [{"i": int(i), "x": cx + 0.3, "y": cy + 0.2 + 0.01 * k, "z": cz} for k, i in enumerate(ids)]
+        elif frame == f1:
+            s.grabs = []
+
+
+def _snap(s):
+    st = s.read_state()
+    c = s.read_constraints()
+    return {k: st[k].copy() for k in KEYS} | {"rest": c["rest"], "alive": c["alive"]}
+
+
+def _same(a, b, label):
+    for k in a:
+        if not np.array_equal(a[k], b[k]):
+            bad = np.flatnonzero(a[k].ravel() != b[k].ravel())
+            raise AssertionError("%s: %s differs at %d entries (first %d)" % (label, k, bad.size, bad[0]))
+
+
+def _alone(sc, frames):
+    from fluidphysics_amd import from_scene
+    s = from_scene(sc)
+    snaps = []
+    for f in range(frames):
+        _cuts(s, sc, f)
+        s.step(DT, SUB)
+        snaps.append(_snap(s))
+    s.close()
+    return snaps
+
+
+def test_batch_bit_identical_to_scenes_alone():
+    from fluidphysics_amd import Batch, from_scene
+    frames = 12
+    scs = _scenes([0, 1, 2, 3, 5], frames)
+    log("case batch vs alone: %s" % [sc.name for sc in scs])
+    ref = [_alone(sc, frames) for sc in scs]
+    b = Batch()
+    ss = [from_scene(sc) for sc in scs]
+    for s in ss:
+        b.add(s)
+    for f in range(frames):
+        for s, sc in zip(ss, scs):
+            _cuts(s, sc, f)
+        b.step(DT, SUB)
+        for k, s in enumerate(ss):
+            _same(_snap(s), ref[k][f], "%s frame %d" % (scs[k].name, f))
+    inf = b.info()
+    assert inf.scenes == len(scs) and inf.frames == frames
+    assert inf.graph_launches == frames
+    b.close()
+
+
+def test_graph_replay_equals_direct_launches():
+    from fluidphysics_amd import Batch, from_scene
+    frames = 10
+    scs = _scenes([6, 7, 8], frames)
+    log("case graph vs direct: %s" % [sc.name for sc in scs])
+    runs = []
+    for use_graph in (False, True):
+        b = Batch()
+        b.graph(use_graph)
+        ss = [from_scene(sc) for sc in scs]
+        for s in ss:
+            b.add(s)
+        snaps = []
+        for f in range(frames):
+            for s, sc in zip(ss, scs):
+                _cuts(s, sc, f)
+            b.step(DT, SUB)
+            # eager device work between replays: reads, a constraint
+            # read-back, a fluid count
+            snaps.append([_snap(s) for s in ss])
+            ss[0].fluidCount()
+        inf = b.info()
+        assert inf.graph_launches == (frames if use_graph else 0)
+        runs.append(snaps)
+        b.close()
+    for f in range(frames):
+        for k in range(len(scs)):
+            _same(runs[1][f][k], runs[0][f][k], "%s frame %d" % (scs[k].name, f))
+
+
+def test_membership_changes_and_mixed_parameters():
+    """A scene removed mid-run and another added keep every member identical
+    to stepping it alone; members differ in every parameter but
+    constraint_iters (here 2, an even count of Jacobi sweeps), and one has a
+    grab, which runs the frame without the graph."""
+    from fluidphysics_amd import Batch, from_scene, scenes
+    frames = 8
+    scs = _scenes([9, 10, 11], frames)
+    plain = scenes.plain_cpu_scene(frames=frames)
+    plain.wedges[0].frame = 4
+    for sc in scs:
+        sc.params["constraint_iters"] = 2
+    plain.params["constraint_iters"] = 2
+    scs[0].test_grab = (2, 6, list(range(8)))
+    log("case membership: %s + %s" % ([sc.name for sc in scs], plain.name))
+    ref = {sc.name: _alone(sc, frames) for sc in scs}
+    # the plain scene joins at frame 3: its alone run starts there too
+    ref_plain = _alone(plain, frames - 3)
+    b = Batch()
+    ss = {sc.name: from_scene(sc) for sc in scs}
+    for s in ss.values():
+        b.add(s)
+    gone = ss[scs[1].name]
+    p = None
+    for f in range(frames):
+        if f == 3:
+            p = from_scene(plain)
+            b.add(p)
+        if f == 5:
+            b.remove(gone)
+        members = [(sc, ss[sc.name]) for sc in scs if not (f >= 5 and ss[sc.name] is gone)]
+        for sc, s in members:
+            _cuts(s, sc, f)
+        if p is not None:
+            _cuts(p, plain, f - 3)
+        b.step(DT, SUB)
+        for sc, s in members:
+            _same(_snap(s), ref[sc.name][f], "%s frame %d" % (sc.name, f))
+        if p is not None:
+            _same(_snap(p), ref_plain[f - 3], "%s frame %d" % (plain.name, f - 3))
+    # the removed scene still steps alone, from where it left the batch
+    _cuts(gone, scs[1], 5)
+    gone.step(DT, SUB)
+    _same(_snap(gone), ref[scs[1].name][5], "%s alone after removal" % scs[1].name)
+    b.close()
+
+
+def test_batch_rejects_mismatched_constraint_iters():
+    from fluidphysics_amd import Batch, FpError, from_scene
+    scs = _scenes([12, 13], 4)
+    scs[1].params["constraint_iters"] = 4
+    b = Batch()
+    for sc in scs:
+        b.add(from_scene(sc))
+    with pytest.raises(FpError, match="constraint_iters"):
+        b.step(DT, SUB)
+    b.close()
+
+
+def test_batch_profile_reports_every_pass():
+    from fluidphysics_amd import Batch, from_scene
+    scs = _scenes([14, 15], 4)
+    b = Batch()
+    for sc in scs:
+        b.add(from_scene(sc))
+    b.profile(True)
+    for _ in range(2):
+        b.step(DT, SUB)
+    prof = b.profile_read()
+    for name in ("integrate", "scan_reduce", "scatter", "rank", "contacts", "resort", "density", "displace",
+                 "constraints", "bounds_velocity", "xsph"):
+        assert name in prof and prof[name][0] >= 2 * SUB, (name, prof.get(name))
+    assert b.info().graph_launches == 0
+    b.close()
