@@ -14,6 +14,11 @@ Workloads (fluidphysics_amd/scenes.py):
   single_gpu  20^3 = 8,000 cube + 65,536 fluid column
   plain_cpu   7^3 = 343 cube + 2,600 fluid (the reference's CPU scale)
 
+A second leg steps the batched workload (256 independent scenes of uneven
+sizes as one batch, fluidphysics_amd.scenes.batched_scene) and reports its
+aggregate throughput and frame p50/p99 beside the same scenes stepped one at
+a time (the `batched` object; --no-batched skips it).
+
 Printed: one JSON line with the metric, a `roofline` object for the dominant
 kernel (algorithmic bytes per launch / average launch time from HIP events
 on the solver's stream) and a `cpu_baseline` object (the oracle -- the
@@ -132,6 +137,95 @@ def cpu_baseline(scene, budget_s=12.0, max_frames=50):
                       % (frames, scene.substeps, scene.name, el, os.cpu_count() or 0)}
 
 
+def _pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(math.ceil(q * len(xs))) - 1)]
+
+
+def batched_leg(count, frames, warmup, solo_frames, seed=4):
+    """The batched workload: `count` independent scenes (scenes.batched_scene:
+    cube 6^3..20^3, 1k..64k fluid, tilted gravity, slice schedules) stepped
+    as one Batch -- one launch per pass over every scene, the frame replayed
+    from a HIP graph -- and then the same scenes stepped one at a time
+    through their own solvers.  Frame latency = one frame of every scene,
+    slices included, measured host-side around a completed frame."""
+    from fluidphysics_amd import Batch, from_scene, scenes
+    horizon = warmup + frames + solo_frames + 1
+    t0 = time.perf_counter()
+    scs = scenes.batched_scenes(count, seed, frames=horizon)
+    ss = [from_scene(sc) for sc in scs]
+    setup_s = time.perf_counter() - t0
+    dt, sub = scs[0].dt, scs[0].substeps
+
+    def cuts(f):
+        for s, sc in zip(ss, scs):
+            for w in sc.wedges:
+                if w.frame == f:
+                    s.cut(*w.o, *w.d1, *w.d2)
+
+    def active():
+        return sum(active_count(s) for s in ss)
+
+    b = Batch()
+    for s in ss:
+        b.add(s)
+    for f in range(warmup):
+        cuts(f)
+        b.step(dt, sub)
+    n0 = active()
+    lat = []
+    for f in range(warmup, warmup + frames):
+        t = time.perf_counter()
+        cuts(f)
+        b.step(dt, sub)
+        lat.append(time.perf_counter() - t)
+    n1 = active()
+    inf = b.info()
+    for s in ss:
+        b.remove(s)
+    b.close()
+    n_mean = 0.5 * (n0 + n1)
+    value = n_mean * sub * frames / sum(lat)
+
+    f0 = warmup + frames
+    for s in ss:  # each solver captures its own frame graph before timing
+        s.step(dt, sub, wait=False)
+    for s in ss:
+        s.sync()
+    lat1 = []
+    for f in range(f0 + 1, f0 + 1 + solo_frames):
+        t = time.perf_counter()
+        cuts(f)
+        for s in ss:
+            s.step(dt, sub, wait=False)
+        for s in ss:
+            s.sync()
+        lat1.append(time.perf_counter() - t)
+    n2 = active()
+    value1 = 0.5 * (n1 + n2) * sub * solo_frames / sum(lat1)
+    for s in ss:
+        s.close()
+    return {
+        "workload": "batched_%d_seed%d" % (count, seed),
+        "scenes": count,
+        "solids": int(sum(sc.solid_count for sc in scs)),
+        "fluid": int(sum(sc.max_fluid for sc in scs)),
+        "active_mean": int(n_mean),
+        "frames": frames,
+        "value": round(value, 1),
+        "unit": "particle-substeps/s",
+        "frame_ms_p50": round(_pct(lat, 0.50) * 1e3, 3),
+        "frame_ms_p99": round(_pct(lat, 0.99) * 1e3, 3),
+        "graph_launches": int(inf.graph_launches),
+        "device_bytes": int(inf.device_bytes),
+        "setup_s": round(setup_s, 2),
+        "one_at_a_time": {"value": round(value1, 1), "frames": solo_frames,
+                          "frame_ms_p50": round(_pct(lat1, 0.50) * 1e3, 3),
+                          "frame_ms_p99": round(_pct(lat1, 0.99) * 1e3, 3)},
+        "speedup_vs_one_at_a_time": round(value / value1, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +234,9 @@ def main():
     ap.add_argument("--workload", default="large", choices=["large", "single_gpu", "plain_cpu"])
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-scenes", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=60)
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the solver's path does not span GPUs "
@@ -218,6 +315,14 @@ def main():
         cpu = cpu_baseline(scene)
         print("bench: cpu baseline %.3e p-substeps/s" % cpu["value"], file=sys.stderr, flush=True)
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg(args.batched_scenes, args.batched_steps, max(3, args.warmup // 2),
+                              max(10, args.batched_steps // 3))
+        print("bench: batched %.3e p-substeps/s, frame p50 %.2f ms p99 %.2f ms; one at a time %.3e" %
+              (batched["value"], batched["frame_ms_p50"], batched["frame_ms_p99"],
+               batched["one_at_a_time"]["value"]), file=sys.stderr, flush=True)
+
     out = {
         "metric": "particle-substeps/s",
         "value": round(value, 1),
@@ -239,6 +344,7 @@ def main():
         "hbm_roofline_frac_model": round(model_frac, 4),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
         "passes": passes,
         "pair_slots": inf_end.pair_slots,
         "pair_overflow": inf_end.pair_overflow,

@@ -101,6 +101,7 @@ class Solver:
         object.__setattr__(self, "_pending_emit", [])
         object.__setattr__(self, "_cache", {})
         object.__setattr__(self, "_params_dirty", False)
+        object.__setattr__(self, "_grabs_pushed", False)
 
     # ---- plumbing -------------------------------------------------------
     def _check(self, rc, h="self"):
@@ -155,6 +156,9 @@ class Solver:
 
     def _push_grabs(self):
         g = self.grabs
+        if not g and not self._grabs_pushed:
+            return  # nothing held now or at the last step
+        self.__dict__["_grabs_pushed"] = bool(g)
         idx = np.array([int(t["i"]) for t in g], np.int32)
         xyz = np.array([[float(t["x"]), float(t["y"]), float(t["z"])] for t in g], np.float64).reshape(-1, 3)
         xyz = np.ascontiguousarray(xyz)

@@ -163,11 +163,10 @@ int settle(fp_solver* s) {
   FP_TRY(s, hipStreamSynchronize(s->stream));
   if (s->prof.on) profile_collect(&s->prof);
   if (s->steps_pending && s->C > 0) {
-    int tear = 0;
-    FP_TRY(s, d2h(s, &tear, s->dflags + DF_TEAR, sizeof(int)));
-    if (tear) {
+    volatile int* tear = s->tear_host;
+    if (*tear) {
       s->groups_dirty = true;
-      FP_TRY(s, hipMemsetAsync(s->dflags + DF_TEAR, 0, sizeof(int), s->stream));
+      *tear = 0;  // the stream is idle: no kernel writes it now
     }
     if (s->cut_dirty_at_step) s->groups_dirty = true;
   }
@@ -271,7 +270,15 @@ int fp_create(const fp_params* params, int32_t max_fluid, int32_t device, fp_sol
   s->max_fluid = max_fluid;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&s->tear_host, 64, hipHostMallocMapped | hipHostMallocCoherent);
+  if (e == hipSuccess) {
+    *s->tear_host = 0;
+    e = hipHostGetDevicePointer((void**)&s->tear_dev, s->tear_host, 0);
+  }
   if (e != hipSuccess) {
+    if (s->tear_host) (void)hipHostFree(s->tear_host);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     g_last_error = std::string("fp_create: ") + hipGetErrorString(e);
     delete s;
     return FP_EDEVICE;
@@ -298,6 +305,7 @@ int fp_destroy(fp_solver* s) {
   free_all(s);
   prof_free(&s->prof);
   if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
+  if (s->tear_host) (void)hipHostFree(s->tear_host);
   delete s;
   return FP_OK;
 }

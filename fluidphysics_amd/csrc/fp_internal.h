@@ -52,7 +52,7 @@
 #endif
 
 // device flag words (dflags)
-#define DF_TEAR 0        // a constraint tore this step
+#define DF_UNUSED0 0     // (the tear flag lives in host-mapped memory: fp_solver::tear_host)
 #define DF_CUT 1         // knife count
 #define DF_SCRATCH 2     // fluid count
 #define DF_PAIR_OVF 3    // particle-substeps whose pair list overflowed
@@ -161,6 +161,10 @@ struct fp_solver {
   bool cut_dirty_at_step = false;
   Prof prof;
   hipStream_t own_stream = nullptr;  // the stream fp_create made (stream may be a batch's)
+  // "a constraint tore" flag in pinned host memory, set by the constraint
+  // kernel through tear_dev and read/cleared by the host after a stream sync
+  int* tear_host = nullptr;
+  int* tear_dev = nullptr;
   fp_group* group = nullptr;         // this scene alone, for fp_step
   fp_batch* batch = nullptr;         // the batch it belongs to, if any
   uint64_t version = 1;              // bumped when its parameters change
@@ -249,6 +253,7 @@ struct SceneArgs {
   float2* sP;
   int* sbox;
   int* dflags;
+  int* tear;  // host-mapped
   float* rest[2];
   uint8_t* alive[2];
   int* adj_off;

@@ -90,7 +90,7 @@ __device__ __forceinline__ void constraints_body(const SceneArgs& A, int lb, int
     cz += __shfl_xor(cz, off, CLANES);
   }
   if (valid && lane == 0) sout[a] = make_float4(pa.x + cx, pa.y + cy, pa.z + cz, pa.w);
-  if (tore) A.dflags[DF_TEAR] = 1;
+  if (tore) *A.tear = 1;
 }
 
 __global__ void __launch_bounds__(BLOCK) kg_constraints(const SceneArgs* __restrict__ S, const int2* __restrict__ W,

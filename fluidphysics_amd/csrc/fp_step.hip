@@ -257,6 +257,7 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
   a.sP = s->sP;
   a.sbox = s->sbox;
   a.dflags = s->dflags;
+  a.tear = s->tear_dev;
   a.rest[0] = s->rest[0];
   a.rest[1] = s->rest[1];
   a.alive[0] = s->alive[0];
