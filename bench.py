@@ -17,7 +17,11 @@ Workloads (fluidphysics_amd/scenes.py):
 A second leg steps the batched workload (256 independent scenes of uneven
 sizes as one batch, fluidphysics_amd.scenes.batched_scene) and reports its
 aggregate throughput and frame p50/p99 beside the same scenes stepped one at
-a time (the `batched` object; --no-batched skips it).
+a time (the `batched` object; --no-batched skips it).  A third serves an
+open-loop Poisson stream of scene requests with continuous batching under an
+HBM budget (fluidphysics_amd.serving) and reports frame p50/p99 and
+sustained throughput beside the per-scene path on the same stream (the
+`serving` object; --no-serving skips it).
 
 Printed: one JSON line with the metric, a `roofline` object for the dominant
 kernel (algorithmic bytes per launch / average launch time from HIP events
@@ -226,6 +230,25 @@ def batched_leg(count, frames, warmup, solo_frames, seed=4):
     }
 
 
+def serving_leg(rate_hz, duration_s, budget_bytes, seed=5):
+    """The serving workload: an open-loop Poisson stream of scene requests
+    (rate_hz, seeded; sizes as in the batched workload, horizons
+    log-uniform over 1..600 frames) served for duration_s seconds by the
+    continuous-batching loop (fluidphysics_amd.serving), then the same
+    stream through the plain per-scene path.  Frame latency = dispatch of a
+    frame -> its position snapshot on the host."""
+    from fluidphysics_amd import scenes
+    from fluidphysics_amd.serving import Server
+    reqs = scenes.serving_requests(rate_hz, duration_s, seed=seed)
+    out = {"workload": "serving_poisson_%gHz_seed%d" % (rate_hz, seed), "rate_hz": rate_hz,
+           "duration_s": duration_s, "requests": len(reqs)}
+    for mode in ("batch", "per_scene"):
+        rep = Server(budget_bytes=budget_bytes, mode=mode).run(reqs, duration=duration_s)
+        out[mode] = rep.summary()
+    out["speedup_vs_per_scene"] = round(out["batch"]["value"] / max(out["per_scene"]["value"], 1.0), 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +260,10 @@ def main():
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=60)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--serving-rate", type=float, default=120.0)
+    ap.add_argument("--serving-seconds", type=float, default=10.0)
+    ap.add_argument("--serving-budget-gb", type=float, default=200.0)
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the solver's path does not span GPUs "
@@ -323,6 +350,13 @@ def main():
               (batched["value"], batched["frame_ms_p50"], batched["frame_ms_p99"],
                batched["one_at_a_time"]["value"]), file=sys.stderr, flush=True)
 
+    serving = None
+    if not args.no_serving:
+        serving = serving_leg(args.serving_rate, args.serving_seconds, args.serving_budget_gb * 1e9)
+        print("bench: serving %.3e p-substeps/s, frame p50 %.2f ms p99 %.2f ms; per-scene %.3e p99 %.2f ms" %
+              (serving["batch"]["value"], serving["batch"]["frame_ms_p50"], serving["batch"]["frame_ms_p99"],
+               serving["per_scene"]["value"], serving["per_scene"]["frame_ms_p99"]), file=sys.stderr, flush=True)
+
     out = {
         "metric": "particle-substeps/s",
         "value": round(value, 1),
@@ -345,6 +379,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "serving": serving,
         "passes": passes,
         "pair_slots": inf_end.pair_slots,
         "pair_overflow": inf_end.pair_overflow,

@@ -265,6 +265,13 @@ class Solver:
         self._cache.update(out)
         return out
 
+    def read_positions(self):
+        """pos alone (fp_read_state with the other outputs omitted), (capacity, 3)."""
+        self._flush()
+        pos = np.zeros(3 * self.capacity, np.float32)
+        self._check(self._lib.fp_read_state(self._h, _fp(pos), None, None, None, None, None))
+        return pos.reshape(-1, 3)
+
     def _arr(self, key):
         if key not in self._cache:
             self.read_state()
@@ -420,6 +427,25 @@ class Batch:
         self._check(self._lib.fp_batch_profile_read(self._h, buf, _abi.FP_MAX_PASSES, ctypes.byref(n)))
         return {buf[k].name.decode(): (buf[k].launches, buf[k].total_ms) for k in range(n.value)}
 
+    def snapshot(self):
+        """Positions of every scene after the frames queued so far
+        (fp_batch_snapshot): a float32 view of 3 x capacity floats per scene,
+        in pinned memory the batch owns and reuses two calls later -- copy
+        what must outlive that -- and the scenes' offsets into it."""
+        pos = ctypes.POINTER(ctypes.c_float)()
+        off = ctypes.POINTER(ctypes.c_int64)()
+        n = ctypes.c_int32()
+        self._check(self._lib.fp_batch_snapshot(self._h, ctypes.byref(pos), ctypes.byref(off), ctypes.byref(n)))
+        offsets = np.ctypeslib.as_array(off, shape=(n.value + 1,)).copy()
+        total = int(offsets[-1])
+        flat = np.ctypeslib.as_array(pos, shape=(total,)) if total else np.zeros(0, np.float32)
+        return flat, offsets
+
+    def positions(self):
+        """snapshot() split per scene: one (capacity, 3) view per member."""
+        flat, off = self.snapshot()
+        return [flat[off[k]:off[k + 1]].reshape(-1, 3) for k in range(len(off) - 1)]
+
     def close(self):
         h = self.__dict__.get("_h")
         if h:
@@ -431,6 +457,29 @@ class Batch:
         self.close()
 
 
+def scene_bytes(params, max_fluid, cube_n):
+    """Device bytes a solver of these parameters holds after addSolidCube(n)
+    and emits of up to max_fluid particles at once (fp_scene_bytes)."""
+    lib = _abi.load_library()
+    p = params if isinstance(params, FpParams) else make_params(**params)
+    out = ctypes.c_int64()
+    if lib.fp_scene_bytes(ctypes.byref(p), int(max_fluid), int(cube_n), ctypes.byref(out)) != 0:
+        raise FpError("fp_scene_bytes: invalid arguments")
+    return out.value
+
+
+def pool_info(device=-1):
+    """(reserved, cached) bytes of the device's memory pool."""
+    lib = _abi.load_library()
+    r, c = ctypes.c_int64(), ctypes.c_int64()
+    lib.fp_pool_info(int(device), ctypes.byref(r), ctypes.byref(c))
+    return r.value, c.value
+
+
+def pool_trim(device=-1):
+    _abi.load_library().fp_pool_trim(int(device))
+
+
 def from_scene(scene, device=-1, **param_overrides):
     """A solver set up with a scene's cube and initial fluid (fluidphysics_amd.scenes)."""
     s = Solver(scene.max_fluid, params=scene.params_with(**param_overrides), device=device)
@@ -440,4 +489,5 @@ def from_scene(scene, device=-1, **param_overrides):
     return s
 
 
-__all__ = ["Solver", "Batch", "FLUID", "SOLID", "FpError", "FpParams", "make_params", "from_scene"]
+__all__ = ["Solver", "Batch", "FLUID", "SOLID", "FpError", "FpParams", "make_params", "from_scene",
+           "scene_bytes", "pool_info", "pool_trim"]

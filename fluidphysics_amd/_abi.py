@@ -175,6 +175,10 @@ SIGNATURES = {
     "fp_batch_get_info": (_I, [_P, ctypes.POINTER(FpBatchInfo)]),
     "fp_batch_profile_enable": (_I, [_P, _I]),
     "fp_batch_profile_read": (_I, [_P, ctypes.POINTER(FpPassStat), _I, _IP]),
+    "fp_batch_snapshot": (_I, [_P, ctypes.POINTER(_FP), ctypes.POINTER(ctypes.POINTER(ctypes.c_int64)), _IP]),
+    "fp_scene_bytes": (_I, [ctypes.POINTER(FpParams), _I, _I, ctypes.POINTER(ctypes.c_int64)]),
+    "fp_pool_info": (_I, [_I, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    "fp_pool_trim": (_I, [_I]),
 }
 
 _lib = None

@@ -455,6 +455,171 @@ napi_value ReadDensity(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// ---- batches, admission estimate, pool (fp_batch_*, fp_scene_bytes, fp_pool_info) ----
+
+fp_batch* bhandle(napi_env env, napi_value v) {
+  void* p = nullptr;
+  napi_get_value_external(env, v, &p);
+  return static_cast<fp_batch*>(p);
+}
+
+napi_value throw_batch(napi_env env, fp_batch* b, int rc, const char* what) {
+  std::string msg = std::string(what) + " failed (" + std::to_string(rc) + "): " + fp_batch_last_error(b);
+  napi_throw_error(env, nullptr, msg.c_str());
+  return nullptr;
+}
+
+#define FPB_CALL(env, b, call, what)                        \
+  do {                                                      \
+    int rc_ = (call);                                       \
+    if (rc_ != 0) return throw_batch((env), (b), rc_, what); \
+  } while (0)
+
+void finalize_batch(napi_env, void* data, void*) {
+  // an explicit destroy() leaves nothing behind
+  (void)data;
+}
+
+napi_value BatchCreate(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  const int device = a.n > 0 ? (int)num(env, a.v[0], -1) : -1;
+  fp_batch* b = nullptr;
+  FPB_CALL(env, nullptr, fp_batch_create(device, &b), "fp_batch_create");
+  napi_value ext;
+  NAPI_CALL(env, napi_create_external(env, b, finalize_batch, nullptr, &ext));
+  return ext;
+}
+
+napi_value BatchDestroy(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_batch_destroy(bhandle(env, a.v[0]));
+  return nullptr;
+}
+
+napi_value BatchAdd(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_batch* b = bhandle(env, a.v[0]);
+  FPB_CALL(env, b, fp_batch_add(b, handle(env, a.v[1])), "fp_batch_add");
+  return nullptr;
+}
+
+napi_value BatchRemove(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_batch* b = bhandle(env, a.v[0]);
+  FPB_CALL(env, b, fp_batch_remove(b, handle(env, a.v[1])), "fp_batch_remove");
+  return nullptr;
+}
+
+napi_value BatchStep(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_batch* b = bhandle(env, a.v[0]);
+  const bool wait = a.n < 4 || num(env, a.v[3], 1) != 0;
+  const double dt = num(env, a.v[1]);
+  const int sub = (int)num(env, a.v[2], 5);
+  FPB_CALL(env, b, wait ? fp_batch_step(b, dt, sub) : fp_batch_step_async(b, dt, sub), "fp_batch_step");
+  return nullptr;
+}
+
+napi_value BatchSync(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_batch* b = bhandle(env, a.v[0]);
+  FPB_CALL(env, b, fp_batch_sync(b), "fp_batch_sync");
+  return nullptr;
+}
+
+napi_value BatchSetGraph(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_batch* b = bhandle(env, a.v[0]);
+  FPB_CALL(env, b, fp_batch_set_graph(b, (int)num(env, a.v[1], 1)), "fp_batch_set_graph");
+  return nullptr;
+}
+
+napi_value BatchInfo(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_batch* b = bhandle(env, a.v[0]);
+  fp_batch_info i;
+  FPB_CALL(env, b, fp_batch_get_info(b, &i), "fp_batch_get_info");
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "scenes", make_int(env, i.scenes));
+  napi_set_named_property(env, o, "graph", make_int(env, i.graph));
+  napi_set_named_property(env, o, "capacity", make_int(env, i.capacity));
+  napi_set_named_property(env, o, "deviceBytes", make_int(env, i.device_bytes));
+  napi_set_named_property(env, o, "frames", make_int(env, i.frames));
+  napi_set_named_property(env, o, "graphLaunches", make_int(env, i.graph_launches));
+  return o;
+}
+
+// { pos: Float32Array, offsets: number[] }: the snapshot copied into `out`
+// (a Float32Array of enough length) when given, else into a new array.
+napi_value BatchSnapshot(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_batch* b = bhandle(env, a.v[0]);
+  const float* pos = nullptr;
+  const int64_t* off = nullptr;
+  int32_t scenes = 0;
+  FPB_CALL(env, b, fp_batch_snapshot(b, &pos, &off, &scenes), "fp_batch_snapshot");
+  const size_t total = (size_t)off[scenes];
+  napi_value arr = nullptr;
+  if (a.n > 1) {
+    TYPED(env, a.v[1], float, napi_float32_array, total, dst);
+    if (dst) {
+      std::memcpy(dst, pos, total * sizeof(float));
+      arr = a.v[1];
+    }
+  }
+  if (!arr) {
+    void* data = nullptr;
+    napi_value ab;
+    NAPI_CALL(env, napi_create_arraybuffer(env, total * sizeof(float), &data, &ab));
+    if (total) std::memcpy(data, pos, total * sizeof(float));
+    NAPI_CALL(env, napi_create_typedarray(env, napi_float32_array, total, ab, 0, &arr));
+  }
+  napi_value offsets;
+  NAPI_CALL(env, napi_create_array_with_length(env, (size_t)scenes + 1, &offsets));
+  for (int k = 0; k <= scenes; k++) napi_set_element(env, offsets, (uint32_t)k, make_int(env, off[k]));
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "pos", arr);
+  napi_set_named_property(env, o, "offsets", offsets);
+  return o;
+}
+
+napi_value SceneBytes(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_params p;
+  fp_params_default(&p);
+  params_from_object(env, a.v[0], &p);
+  int64_t bytes = 0;
+  if (fp_scene_bytes(&p, (int)num(env, a.v[1]), (int)num(env, a.v[2]), &bytes) != 0) {
+    napi_throw_range_error(env, nullptr, "fp_scene_bytes: invalid arguments");
+    return nullptr;
+  }
+  return make_int(env, bytes);
+}
+
+napi_value PoolInfo(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  int64_t reserved = 0, cached = 0;
+  fp_pool_info(a.n > 0 ? (int)num(env, a.v[0], -1) : -1, &reserved, &cached);
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "reserved", make_int(env, reserved));
+  napi_set_named_property(env, o, "cached", make_int(env, cached));
+  return o;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct {
     const char* name;
@@ -468,6 +633,10 @@ napi_value Init(napi_env env, napi_value exports) {
       {"clearFluid", ClearFluid},   {"readState", ReadState},   {"writeState", WriteState},
       {"readConstraints", ReadConstraints}, {"readGrid", ReadGrid}, {"buildGrid", BuildGrid},
       {"readDensity", ReadDensity},
+      {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy}, {"batchAdd", BatchAdd},
+      {"batchRemove", BatchRemove}, {"batchStep", BatchStep},       {"batchSync", BatchSync},
+      {"batchSetGraph", BatchSetGraph}, {"batchInfo", BatchInfo},   {"batchSnapshot", BatchSnapshot},
+      {"sceneBytes", SceneBytes},   {"poolInfo", PoolInfo},
   };
   for (auto& f : fns) {
     napi_value fn;

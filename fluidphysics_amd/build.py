@@ -24,7 +24,7 @@ HIP_FLAGS = [
 
 NODE_INCLUDE = "/usr/include/node"
 
-SOURCES = ["fp_api.hip", "fp_grid.hip", "fp_fluid.hip", "fp_solid.hip", "fp_step.hip"]
+SOURCES = ["fp_api.hip", "fp_pool.hip", "fp_grid.hip", "fp_fluid.hip", "fp_solid.hip", "fp_step.hip"]
 
 
 def build_library(verbose=False):

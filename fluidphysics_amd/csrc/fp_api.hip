@@ -56,38 +56,64 @@ void prof_free(Prof* p) {
   p->pool.clear();
 }
 
+// A growing staging buffer (emit, grab): one pool block of its own.
 template <typename T>
 int dalloc(fp_solver* s, T** p, size_t count) {
   const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-  hipError_t e = hipMalloc((void**)p, bytes);
+  size_t cb = 0;
+  hipError_t e = fp_pool_get(s->device, bytes, (void**)p, &cb);
   if (e != hipSuccess) return fp_fail(s, FP_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-  s->device_bytes += (int64_t)bytes;
+  s->device_bytes += (int64_t)cb;
   e = hipMemsetAsync(*p, 0, bytes, s->stream);
   if (e != hipSuccess) return fp_fail(s, FP_EDEVICE, std::string("hipMemsetAsync: ") + hipGetErrorString(e));
   return FP_OK;
 }
 
 template <typename T>
-void dfree(T*& p) {
-  if (p) (void)hipFree(p);
+void dfree(fp_solver* s, T*& p) {
+  if (p) s->device_bytes -= (int64_t)fp_pool_put(s->device, (void*)p);
   p = nullptr;
 }
 
-void free_all(fp_solver* s) {
-  dfree(s->pos), dfree(s->prev), dfree(s->vel), dfree(s->active), dfree(s->rho), dfree(s->phash);
-  dfree(s->lcount), dfree(s->lstart), dfree(s->lblock_sums), dfree(s->ltmp), dfree(s->lkey), dfree(s->lslot);
-  dfree(s->sid), dfree(s->spos), dfree(s->spos2), dfree(s->svtmp), dfree(s->sphash), dfree(s->sP);
-  dfree(s->tpos), dfree(s->tkey), dfree(s->tslot), dfree(s->fsid), dfree(s->fslot), dfree(s->sphash2);
-  dfree(s->sbox);
-  dfree(s->lnsol);
-  dfree(s->nbr), dfree(s->ncount);
-  dfree(s->pslot), dfree(s->count), dfree(s->cell_start), dfree(s->entries_tmp), dfree(s->cell_entries);
-  dfree(s->block_sums);
-  dfree(s->cons_ab), dfree(s->rest[0]), dfree(s->rest[1]), dfree(s->alive[0]), dfree(s->alive[1]);
-  dfree(s->adj_off), dfree(s->adj), dfree(s->spos_c[0]), dfree(s->spos_c[1]), dfree(s->dflags), dfree(s->dpick_t);
-  dfree(s->dpick_i), dfree(s->grab_idx), dfree(s->grab_tgt), dfree(s->emit_idx), dfree(s->emit_p), dfree(s->emit_v);
+// The fixed buffers of a scene, in arena order.  `take(ptr, count)` is
+// called once per buffer: sizing (Carve without a base) and placement
+// (Carve with one) walk the same list.
+template <class F>
+void scene_buffers(fp_solver* s, F& take) {
+  const size_t N = (size_t)s->n, T = (size_t)s->T, M = s->cg.M, ns = (size_t)s->ns;
+  const size_t C = (size_t)std::max(s->C, 1);
+  take(s->pos, N), take(s->prev, N), take(s->vel, N), take(s->active, N), take(s->rho, N), take(s->phash, N);
+  take(s->lcount, M), take(s->lstart, M + 1), take(s->lnsol, M), take(s->lblock_sums, (size_t)s->lscan_blocks);
+  take(s->ltmp, N), take(s->lkey, N), take(s->lslot, N), take(s->sid, N), take(s->spos, N), take(s->spos2, N);
+  take(s->svtmp, N), take(s->sphash, N), take(s->sP, N), take(s->tpos, N), take(s->tkey, N), take(s->tslot, N);
+  take(s->fsid, N), take(s->fslot, N), take(s->sphash2, N), take(s->sbox, 6);
+  take(s->nbr, (size_t)(s->pair_slots + FLUID_LANES) * N), take(s->ncount, N * FLUID_LANES);
+  take(s->pslot, N), take(s->count, T), take(s->cell_start, T + 1), take(s->entries_tmp, N);
+  take(s->cell_entries, N), take(s->block_sums, (size_t)s->scan_blocks);
+  take(s->cons_ab, C), take(s->rest[0], C), take(s->rest[1], C), take(s->alive[0], C), take(s->alive[1], C);
+  take(s->adj_off, ns + 1), take(s->adj, 2 * C), take(s->spos_c[0], std::max<size_t>(ns, 1));
+  take(s->spos_c[1], std::max<size_t>(ns, 1)), take(s->dflags, (size_t)DF_WORDS), take(s->dpick_t, 1);
+  take(s->dpick_i, 1);
 }
 
+struct Carve {
+  char* base = nullptr;
+  size_t off = 0;
+  template <typename T>
+  void operator()(T*& p, size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += std::max<size_t>(count, 1) * sizeof(T);
+  }
+};
+
+void free_all(fp_solver* s) {
+  Carve none;  // null every arena pointer
+  scene_buffers(s, none);
+  if (s->arena) s->device_bytes -= (int64_t)fp_pool_put(s->device, s->arena);
+  s->arena = nullptr;
+  dfree(s, s->grab_idx), dfree(s, s->grab_tgt), dfree(s, s->emit_idx), dfree(s, s->emit_p), dfree(s, s->emit_v);
+}
 
 uint32_t next_pow2(double v, uint32_t lo) {
   uint32_t t = lo;
@@ -128,6 +154,27 @@ void setup_cell_grid(fp_solver* s) {
   }
   s->cg.kmask = s->cg.M - 1;
   s->lscan_blocks = (int)((s->cg.M + SCAN_TILE - 1) / SCAN_TILE);
+}
+
+// Particle counts and the reference table of a cube of side n.
+void scene_dims(fp_solver* s, int n) {
+  s->ns = n * n * n;
+  s->n = s->ns + s->max_fluid;
+  s->nf = s->max_fluid;
+  if (s->P.table_size > 0) {
+    s->T = s->P.table_size;
+  } else {
+    int t = 1;
+    while (t < 2 * s->n) t <<= 1;  // solver.js:51
+    s->T = t;
+  }
+  s->scan_blocks = (s->T + SCAN_TILE - 1) / SCAN_TILE;
+}
+
+// Cell grid and pair-list width, once the counts are known.
+void scene_tables(fp_solver* s) {
+  setup_cell_grid(s);
+  s->pair_slots = s->P.pair_slots > 0 ? s->P.pair_slots : 128;
 }
 
 int set_device(fp_solver* s) {
@@ -361,17 +408,7 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
   int rc = set_device(s);
   if (rc) return rc;
   const int ns = n * n * n;
-  s->ns = ns;
-  s->n = (int)cap64;
-  s->nf = s->max_fluid;
-  if (s->P.table_size > 0) {
-    s->T = s->P.table_size;
-  } else {
-    int t = 1;
-    while (t < 2 * s->n) t <<= 1;  // solver.js:51
-    s->T = t;
-  }
-  s->scan_blocks = (s->T + SCAN_TILE - 1) / SCAN_TILE;
+  scene_dims(s, n);
 
   // lattice (solver.js:74-89): double arithmetic, float32 stores
   const double half = (n - 1) * spacing * 0.5;
@@ -439,31 +476,19 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
   }
   s->h_group.assign(ns, 0);
 
-  setup_cell_grid(s);
-  s->pair_slots = s->P.pair_slots > 0 ? s->P.pair_slots : 128;
-  const int N = s->n, T = s->T;
-  const size_t M = s->cg.M;
-  if ((rc = dalloc(s, &s->pos, N)) || (rc = dalloc(s, &s->prev, N)) || (rc = dalloc(s, &s->vel, N)) ||
-      (rc = dalloc(s, &s->active, N)) || (rc = dalloc(s, &s->rho, N)) || (rc = dalloc(s, &s->phash, N)) ||
-      (rc = dalloc(s, &s->lcount, M)) || (rc = dalloc(s, &s->lstart, M + 1)) || (rc = dalloc(s, &s->lnsol, M)) ||
-      (rc = dalloc(s, &s->lblock_sums, s->lscan_blocks)) || (rc = dalloc(s, &s->ltmp, N)) ||
-      (rc = dalloc(s, &s->lkey, N)) || (rc = dalloc(s, &s->lslot, N)) || (rc = dalloc(s, &s->sid, N)) ||
-      (rc = dalloc(s, &s->spos, N)) || (rc = dalloc(s, &s->spos2, N)) ||
-      (rc = dalloc(s, &s->svtmp, N)) || (rc = dalloc(s, &s->sphash, N)) || (rc = dalloc(s, &s->sP, N)) ||
-      (rc = dalloc(s, &s->tpos, N)) || (rc = dalloc(s, &s->tkey, N)) || (rc = dalloc(s, &s->tslot, N)) ||
-      (rc = dalloc(s, &s->fsid, N)) || (rc = dalloc(s, &s->fslot, N)) || (rc = dalloc(s, &s->sphash2, N)) ||
-      (rc = dalloc(s, &s->sbox, 6)) || (rc = dalloc(s, &s->nbr, (size_t)(s->pair_slots + FLUID_LANES) * N)) ||
-      (rc = dalloc(s, &s->ncount, (size_t)N * FLUID_LANES)) ||
-      (rc = dalloc(s, &s->pslot, N)) || (rc = dalloc(s, &s->count, T)) ||
-      (rc = dalloc(s, &s->cell_start, (size_t)T + 1)) || (rc = dalloc(s, &s->entries_tmp, N)) ||
-      (rc = dalloc(s, &s->cell_entries, N)) || (rc = dalloc(s, &s->block_sums, s->scan_blocks)) ||
-      (rc = dalloc(s, &s->cons_ab, std::max(C, 1))) || (rc = dalloc(s, &s->rest[0], std::max(C, 1))) ||
-      (rc = dalloc(s, &s->rest[1], std::max(C, 1))) || (rc = dalloc(s, &s->alive[0], std::max(C, 1))) ||
-      (rc = dalloc(s, &s->alive[1], std::max(C, 1))) || (rc = dalloc(s, &s->adj_off, ns + 1)) ||
-      (rc = dalloc(s, &s->adj, std::max(2 * C, 1))) || (rc = dalloc(s, &s->spos_c[0], ns)) ||
-      (rc = dalloc(s, &s->spos_c[1], ns)) || (rc = dalloc(s, &s->dflags, DF_WORDS)) ||
-      (rc = dalloc(s, &s->dpick_t, 1)) || (rc = dalloc(s, &s->dpick_i, 1))) {
-    return rc;
+  scene_tables(s);
+  const int N = s->n;
+  {
+    Carve size;
+    scene_buffers(s, size);
+    size_t cb = 0;
+    const hipError_t e = fp_pool_get(s->device, size.off, &s->arena, &cb);
+    if (e != hipSuccess) return fp_fail(s, FP_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    s->device_bytes += (int64_t)cb;
+    FP_TRY(s, hipMemsetAsync(s->arena, 0, size.off, s->stream));
+    Carve place;
+    place.base = (char*)s->arena;
+    scene_buffers(s, place);
   }
   {
     // phash = "not in the grid" until the first build; an empty solid box
@@ -500,9 +525,9 @@ int fp_emit(fp_solver* s, const double* pos, const double* vel, const double* ma
   const int m = n - first;
   if (m > s->emit_cap) {
     FP_TRY(s, hipStreamSynchronize(s->stream));
-    dfree(s->emit_idx);
-    dfree(s->emit_p);
-    dfree(s->emit_v);
+    dfree(s, s->emit_idx);
+    dfree(s, s->emit_p);
+    dfree(s, s->emit_v);
     const int cap = std::max(m, 1024);
     if ((rc = dalloc(s, &s->emit_idx, cap)) || (rc = dalloc(s, &s->emit_p, cap)) || (rc = dalloc(s, &s->emit_v, cap)))
       return rc;
@@ -628,8 +653,8 @@ int fp_set_grabs(fp_solver* s, const int32_t* idx, const double* xyz, int32_t n)
   }
   if (n > s->grab_cap) {
     FP_TRY(s, hipStreamSynchronize(s->stream));
-    dfree(s->grab_idx);
-    dfree(s->grab_tgt);
+    dfree(s, s->grab_idx);
+    dfree(s, s->grab_tgt);
     const int cap = std::max(n, 256);
     if ((rc = dalloc(s, &s->grab_idx, cap)) || (rc = dalloc(s, &s->grab_tgt, cap))) return rc;
     s->grab_cap = cap;
@@ -860,6 +885,13 @@ struct fp_batch {
   Prof prof;
   std::string err;
   int64_t frames = 0;
+  // position snapshots: a device gather buffer and two pinned host slots
+  float* d_snap = nullptr;
+  size_t d_snap_cap = 0;
+  float* h_snap[2] = {nullptr, nullptr};
+  size_t h_snap_cap[2] = {0, 0};
+  int snap_slot = 0;
+  std::vector<int64_t> snap_off, snap_off_out[2];
 };
 
 static int batch_fail(fp_batch* b, int code, const std::string& msg) {
@@ -906,6 +938,9 @@ int fp_batch_destroy(fp_batch* b) {
   b->g.scenes.clear();
   fp_group_free(&b->g);
   prof_free(&b->prof);
+  if (b->d_snap) fp_pool_put(b->g.device, b->d_snap);
+  for (float* h : b->h_snap)
+    if (h) (void)hipHostFree(h);
   if (b->g.stream) (void)hipStreamDestroy(b->g.stream);
   delete b;
   return FP_OK;
@@ -1020,3 +1055,69 @@ int fp_batch_profile_read(fp_batch* b, fp_pass_stat* out, int32_t max, int32_t* 
 }
 
 }  // extern "C"
+
+int fp_scene_bytes(const fp_params* p, int32_t max_fluid, int32_t cube_n, int64_t* bytes) {
+  if (!p || !bytes || cube_n < 1 || cube_n > 1024 || max_fluid < 0) return FP_EINVAL;
+  if ((int64_t)cube_n * cube_n * cube_n + max_fluid >= (1ll << 27)) return FP_EINVAL;
+  fp_solver tmp;
+  tmp.P = *p;
+  tmp.max_fluid = max_fluid;
+  scene_dims(&tmp, cube_n);
+  // every lattice neighbour within sqrt(3) spacing (26-neighbourhood), once
+  const int64_t n = cube_n;
+  tmp.C = (int)(3 * n * n * (n - 1) + 6 * n * (n - 1) * (n - 1) + 4 * (n - 1) * (n - 1) * (n - 1));
+  scene_tables(&tmp);
+  Carve size;
+  scene_buffers(&tmp, size);
+  int64_t total = (int64_t)fp_pool_class(size.off);
+  if (max_fluid > 0) {
+    const size_t cap = (size_t)std::max(max_fluid, 1024);
+    total += (int64_t)(fp_pool_class(cap * sizeof(int)) + 2 * fp_pool_class(cap * sizeof(float4)));
+  }
+  *bytes = total;
+  return FP_OK;
+}
+
+int fp_batch_snapshot(fp_batch* b, const float** pos, const int64_t** offsets, int32_t* scenes) {
+  if (!b || !pos || !offsets || !scenes) return batch_fail(b, FP_EINVAL, "fp_batch_snapshot: NULL argument");
+  fp_group* g = &b->g;
+  FP_GTRY(g, hipSetDevice(g->device));
+  const int ns = (int)g->scenes.size();
+  b->snap_off.assign(ns + 1, 0);
+  for (int k = 0; k < ns; k++) b->snap_off[k + 1] = b->snap_off[k] + 3 * (int64_t)g->scenes[k]->n;
+  const int64_t total = b->snap_off[ns];
+  // the scene table holds each scene's offset: rebuild it if the members changed
+  int rc = fp_group_prepare(g, g->built_h > 0 ? g->built_h : 1.0 / 300.0);
+  if (rc) return batch_fail(b, rc, g->err);
+  const int slot = b->snap_slot;
+  if ((size_t)total > b->d_snap_cap || (size_t)total > b->h_snap_cap[slot]) {
+    FP_GTRY(g, hipStreamSynchronize(g->stream));
+    if ((size_t)total > b->d_snap_cap) {
+      if (b->d_snap) fp_pool_put(g->device, b->d_snap);
+      b->d_snap = nullptr;
+      b->d_snap_cap = 0;
+      FP_GTRY(g, fp_pool_get(g->device, std::max<int64_t>(total, 1) * sizeof(float), (void**)&b->d_snap, nullptr));
+      b->d_snap_cap = (size_t)total;
+    }
+    if ((size_t)total > b->h_snap_cap[slot]) {
+      if (b->h_snap[slot]) (void)hipHostFree(b->h_snap[slot]);
+      b->h_snap[slot] = nullptr;
+      b->h_snap_cap[slot] = 0;
+      const size_t cap = (size_t)total + (size_t)total / 4;  // room to grow
+      FP_GTRY(g, hipHostMalloc((void**)&b->h_snap[slot], std::max<size_t>(cap, 1) * sizeof(float), 0));
+      b->h_snap_cap[slot] = cap;
+    }
+  }
+  if (total > 0) {
+    rc = fp_group_snapshot(g, b->d_snap);
+    if (rc) return batch_fail(b, rc, g->err);
+    FP_GTRY(g, hipMemcpyAsync(b->h_snap[slot], b->d_snap, total * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+  }
+  FP_GTRY(g, hipStreamSynchronize(g->stream));
+  *pos = b->h_snap[slot];
+  b->snap_off_out[slot] = b->snap_off;
+  *offsets = b->snap_off_out[slot].data();
+  *scenes = ns;
+  b->snap_slot ^= 1;
+  return FP_OK;
+}

@@ -165,6 +165,7 @@ struct fp_solver {
   // kernel through tear_dev and read/cleared by the host after a stream sync
   int* tear_host = nullptr;
   int* tear_dev = nullptr;
+  void* arena = nullptr;  // one pool block holding every fixed buffer of the scene
   fp_group* group = nullptr;         // this scene alone, for fp_step
   fp_batch* batch = nullptr;         // the batch it belongs to, if any
   uint64_t version = 1;              // bumped when its parameters change
@@ -236,6 +237,7 @@ struct SceneArgs {
   ConsParams cp;
   int n, ns, C;
   float gvx, gvy, gvz, h;  // gravity * h, substep
+  int64_t snap_off;        // first float of the scene in a batch position snapshot
   float4 *pos, *prev, *vel;
   uint8_t* active;
   float2* rho;
@@ -305,6 +307,12 @@ struct fp_group {
 // ---- host helpers (fp_api.hip) ---------------------------------------------
 
 int fp_fail(fp_solver* s, int code, const std::string& msg);
+
+// ---- per-device memory pool (fp_pool.hip) -----------------------------------
+size_t fp_pool_class(size_t bytes);
+hipError_t fp_pool_get(int device, size_t bytes, void** out, size_t* class_bytes);
+size_t fp_pool_put(int device, void* ptr);  // returns the block's class bytes (0: not a pool block)
+int64_t fp_pool_trim_device(int device);
 hipEvent_t fp_event_get(Prof* p);
 
 #define FP_TRY(s, expr)                                                                 \
@@ -386,6 +394,7 @@ int fp_group_constraints(fp_group* g, int iters);
 // fp_step.hip
 int fp_group_frame(fp_group* g, double dt, int substeps);
 int fp_group_prepare(fp_group* g, double h);
+int fp_group_snapshot(fp_group* g, float* out);
 void fp_group_free(fp_group* g);
 
 // ---- single-scene launches (events, queries) -------------------------------

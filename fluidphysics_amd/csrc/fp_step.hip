@@ -319,6 +319,11 @@ int fp_group_prepare(fp_group* g, double h) {
     }
   }
   if (g->iters < 0) g->iters = 0;
+  int64_t snap = 0;
+  for (int k = 0; k < ns; k++) {
+    args[k].snap_off = snap;
+    snap += 3 * (int64_t)g->scenes[k]->n;
+  }
   if (ns > g->args_cap) {
     if (g->d_args) (void)hipFree(g->d_args);
     g->d_args = nullptr;
@@ -430,6 +435,27 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
       if ((rc = group_substep(g))) return rc;
   }
   for (fp_solver* s : g->scenes) s->substeps_done += substeps;
+  return FP_OK;
+}
+
+// Positions of every scene into one packed xyz buffer at each scene's offset.
+__global__ void __launch_bounds__(BLOCK)
+    kg_snapshot(const SceneArgs* __restrict__ S, const int2* __restrict__ W, int nw, float* __restrict__ out) {
+  FOR_ITEMS(S, W, nw) {
+    const int i = lb * BLOCK + (int)threadIdx.x;
+    if (i < A.n) {
+      const float4 p = A.pos[i];
+      float* o = out + A.snap_off + 3 * (int64_t)i;
+      o[0] = p.x;
+      o[1] = p.y;
+      o[2] = p.z;
+    }
+  }
+}
+
+int fp_group_snapshot(fp_group* g, float* out) {
+  GROUP_LAUNCH(g, kg_snapshot, W_PARTICLE, BLOCK, out);
+  FP_GTRY(g, hipGetLastError());
   return FP_OK;
 }
 

@@ -36,7 +36,8 @@ export const SOLID = 1;
 const TUNING = ['gravity', 'solidRadius', 'fluidRadius', 'compliance', 'plasticThreshold',
   'plasticRate', 'breakStrain', 'fluidH', 'restDensity', 'stiffness', 'nearStiffness',
   'viscosity', 'solidDamping', 'fluidDamping', 'floorFrictionSolid', 'floorFrictionFluid',
-  'cullRadius', 'maxSpeed', 'grabStiffness', 'cellSize', 'constraintIters', 'jacobiOmega'];
+  'cullRadius', 'maxSpeed', 'grabStiffness', 'cellSize', 'constraintIters', 'jacobiOmega',
+  'contactScale', 'gravityX', 'gravityZ'];
 
 export class Solver {
   constructor({ maxFluid = 2600, params = {}, device = -1 } = {}) {
@@ -48,6 +49,7 @@ export class Solver {
     this._dirtyParams = false;
     this._emits = [];
     this._views = null;
+    this._grabsPushed = false;
     this.grabs = [];
     for (const k of TUNING) {
       Object.defineProperty(this, k, {
@@ -116,15 +118,21 @@ export class Solver {
 
   fluidCount() { this._flush(); return addon.fluidCount(this._h); }
 
-  // step (src/solver.js:187-242); wait = false leaves the wait to sync()
-  step(dt, substeps = 5, wait = true) {
-    this._flush();
+  _pushGrabs() {
     const g = this.grabs;
+    if (!g.length && !this._grabsPushed) return;  // nothing held now or at the last step
+    this._grabsPushed = g.length > 0;
     const idx = new Int32Array(g.length), xyz = new Float64Array(3 * g.length);
     for (let k = 0; k < g.length; k++) {
       idx[k] = g[k].i; xyz[3 * k] = g[k].x; xyz[3 * k + 1] = g[k].y; xyz[3 * k + 2] = g[k].z;
     }
     addon.setGrabs(this._h, idx, xyz, g.length);
+  }
+
+  // step (src/solver.js:187-242); wait = false leaves the wait to sync()
+  step(dt, substeps = 5, wait = true) {
+    this._flush();
+    this._pushGrabs();
     addon.step(this._h, dt, substeps, wait ? 1 : 0);
     this._stale();
   }
@@ -197,3 +205,50 @@ export class Solver {
 
   destroy() { if (this._h) { addon.destroy(this._h); this._h = null; } }
 }
+
+// Many independent scenes stepped together (no counterpart in the reference,
+// which steps one scene): one kernel launch per pass over every member, the
+// frame replayed from a HIP graph.  Members keep their Solver API between
+// frames; all use the same constraintIters.
+export class Batch {
+  constructor({ device = -1 } = {}) {
+    this._h = addon.batchCreate(device);
+    this.scenes = [];
+  }
+
+  add(solver) { solver._flush(); addon.batchAdd(this._h, solver._h); this.scenes.push(solver); }
+
+  remove(solver) {
+    addon.batchRemove(this._h, solver._h);
+    this.scenes.splice(this.scenes.indexOf(solver), 1);
+  }
+
+  // one frame of every member (src/solver.js:187-242 each)
+  step(dt, substeps = 5, wait = true) {
+    for (const s of this.scenes) { s._flush(); s._pushGrabs(); }
+    addon.batchStep(this._h, dt, substeps, wait ? 1 : 0);
+    for (const s of this.scenes) s._stale();
+  }
+
+  sync() { addon.batchSync(this._h); }
+
+  graph(on = true) { addon.batchSetGraph(this._h, on ? 1 : 0); }
+
+  info() { return addon.batchInfo(this._h); }
+
+  // positions of every member after the frames queued so far: one device
+  // copy; { pos: Float32Array (into `out` if given), offsets } with member k
+  // at pos.subarray(offsets[k], offsets[k + 1])
+  snapshot(out = undefined) { return out ? addon.batchSnapshot(this._h, out) : addon.batchSnapshot(this._h); }
+
+  destroy() {
+    if (this._h) { addon.batchDestroy(this._h); this._h = null; }
+    this.scenes = [];
+  }
+}
+
+// Device bytes a solver with these parameters holds after addSolidCube(n)
+// and emits of at most maxFluid particles at once (an admission estimate).
+export function sceneBytes(params, maxFluid, n) { return addon.sceneBytes(params, maxFluid, n); }
+
+export function poolInfo(device = -1) { return addon.poolInfo(device); }

@@ -260,6 +260,38 @@ def batched_scenes(count=256, seed=4, frames=600):
     return [batched_scene(k, seed, frames) for k in range(count)]
 
 
+@dataclass
+class Request:
+    """One serving request: scene `index` of the batched workload (seed
+    `seed`), stepped for `horizon` frames from its arrival."""
+    id: int
+    arrival: float       # seconds after the start of the stream (or a frame index)
+    index: int
+    seed: int
+    horizon: int
+
+    def scene(self):
+        return batched_scene(self.index, self.seed, frames=self.horizon)
+
+
+def serving_requests(rate_hz, duration_s, seed=5):
+    """Open-loop Poisson stream: exponential inter-arrival gaps of mean
+    1/rate_hz over duration_s seconds, each request a batched-workload scene
+    (cube 6^3..20^3, 1k..64k fluid, tilted gravity, slices) with a horizon
+    log-uniform over 1..600 frames."""
+    out = []
+    t = 0.0
+    k = 0
+    while True:
+        u = uniform01(seed, 3, offset=3 * k)
+        t += -math.log(1.0 - u[0]) / rate_hz
+        if t >= duration_s:
+            return out
+        horizon = max(1, min(600, int(round(math.exp(u[1] * math.log(600.0))))))
+        out.append(Request(id=k, arrival=t, index=int(u[2] * (1 << 30)), seed=seed, horizon=horizon))
+        k += 1
+
+
 def hose_burst(hose, frame, spacing):
     """Particles the hose emits at `frame`: positions on a golden-angle disc
     across the nozzle (src/main.js:235-241 pattern), directions uniform in a

@@ -258,6 +258,34 @@ int fp_batch_set_graph(fp_batch* b, int32_t enable);
 int fp_batch_get_info(const fp_batch* b, fp_batch_info* out);
 int fp_batch_profile_enable(fp_batch* b, int32_t enable);
 int fp_batch_profile_read(fp_batch* b, fp_pass_stat* out, int32_t max, int32_t* n);
+/* Position snapshot of every scene of the batch after the frames queued so
+ * far (the per-frame output of a serving loop): float32 x, y, z per particle
+ * slot -- each scene's capacity, solids first, the layout of fp_read_state's
+ * pos -- scenes in the order they were added.  One gather kernel and one
+ * device-to-host copy into pinned memory the batch owns; returns when both
+ * finished.  *pos stays valid until the second-next fp_batch_snapshot call
+ * or fp_batch_destroy; (*offsets)[k] is scene k's first float and
+ * (*offsets)[*scenes] the total. */
+int fp_batch_snapshot(fp_batch* b, const float** pos, const int64_t** offsets, int32_t* scenes);
+
+/* ---- device memory -------------------------------------------------- */
+
+/* No counterpart in the reference (it allocates typed arrays per solver).
+ * Every solver allocates from a per-device pool: its fixed buffers are one
+ * block, blocks come in size classes of a quarter octave and return to the
+ * pool when the solver is destroyed, so creating and destroying scenes
+ * between frames neither allocates from the runtime (once warm) nor waits
+ * for the device. */
+
+/* Device bytes a solver with these parameters holds after
+ * fp_add_solid_cube(n) and emits of at most max_fluid particles per call
+ * (an admission estimate; needs no device). */
+int fp_scene_bytes(const fp_params* p, int32_t max_fluid, int32_t cube_n, int64_t* bytes);
+/* Bytes the pool of `device` (-1: current) holds from the runtime, and the
+ * part of them cached for reuse. */
+int fp_pool_info(int32_t device, int64_t* reserved, int64_t* cached);
+/* Return the cached blocks to the runtime (waits for the device). */
+int fp_pool_trim(int32_t device);
 
 #ifdef __cplusplus
 }

@@ -92,3 +92,53 @@ def test_headless_drive_timeline():
     assert by["slice"]["alive"] < by["settle"]["alive"]      # the strokes cut constraints
     assert by["hose"]["fluid"] > 0                            # the hose emitted water
     assert by["settle"]["minY"] >= 0.09 - 1e-3                # the cube rests on the floor
+
+
+def to_js_params(p):
+    """fp_params field names (snake_case) -> the addon's camelCase names."""
+    return {k.split("_")[0] + "".join(w.title() for w in k.split("_")[1:]): v for k, v in p.items()}
+
+
+def test_node_batch_bit_identical_to_ctypes_batch(tmp_path):
+    """The Node host's Batch (solver.mjs over the addon) and the ctypes
+    host's Batch step the same scenes to bit-identical snapshots, and the
+    addon's sceneBytes equals what each solver holds."""
+    _need_node()
+    from fluidphysics_amd import Batch, from_scene, scenes
+    frames, sub, dt = 6, 5, 1.0 / 60.0
+    scs = [scenes.batched_scene(k, frames=frames) for k in (50, 51, 52)]
+    for sc in scs:
+        for j, w in enumerate(sc.wedges):
+            w.frame = 2 + j
+    log("case node batch vs ctypes batch: %s" % [sc.name for sc in scs])
+    doc = []
+    for sc in scs:
+        doc.append(dict(params=to_js_params(sc.params_with()), maxFluid=sc.max_fluid, cube=list(sc.cube),
+                        pos=sc.fluid_pos.ravel().tolist(), vel=sc.fluid_vel.ravel().tolist(), mass=sc.fluid_mass,
+                        cuts=[dict(frame=w.frame, o=list(w.o), d1=list(w.d1), d2=list(w.d2)) for w in sc.wedges]))
+    inp = tmp_path / "scenes.json"
+    out = tmp_path / "snap.bin"
+    inp.write_text(json.dumps(doc))
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "run_batch.mjs"), str(inp), str(out),
+                        str(frames), str(sub), repr(dt)], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    for est, held in res["bytes"]:
+        assert est == held
+    total = res["offsets"][-1]
+    raw = np.frombuffer(out.read_bytes(), np.float32).reshape(frames, total)
+    b = Batch()
+    ss = [from_scene(sc) for sc in scs]
+    for s in ss:
+        b.add(s)
+    for f in range(frames):
+        for s, sc in zip(ss, scs):
+            for w in sc.wedges:
+                if w.frame == f:
+                    s.cut(*w.o, *w.d1, *w.d2)
+        b.step(dt, sub)
+        flat, off = b.snapshot()
+        assert list(off) == res["offsets"]
+        assert np.array_equal(flat, raw[f]), "frame %d" % f
+    assert res["info"]["graphLaunches"] == frames
+    b.close()
