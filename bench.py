@@ -83,7 +83,7 @@ def measured_traffic(workload, pass_name):
         return None
     if t.get("workload") != workload:
         return None
-    k = t.get("kernels", {}).get("k_" + pass_name)
+    k = t.get("kernels", {}).get("kg_" + pass_name)
     return int(k["hbm_bytes"]) if k else None
 
 
@@ -191,22 +191,27 @@ def batched_leg(count, frames, warmup, solo_frames, seed=4):
     n_mean = 0.5 * (n0 + n1)
     value = n_mean * sub * frames / sum(lat)
 
-    f0 = warmup + frames
-    for s in ss:  # each solver captures its own frame graph before timing
-        s.step(dt, sub, wait=False)
-    for s in ss:
-        s.sync()
-    lat1 = []
-    for f in range(f0 + 1, f0 + 1 + solo_frames):
-        t = time.perf_counter()
-        cuts(f)
-        for s in ss:
+    solo = None
+    if solo_frames > 0:
+        f0 = warmup + frames
+        for s in ss:  # each solver captures its own frame graph before timing
             s.step(dt, sub, wait=False)
         for s in ss:
             s.sync()
-        lat1.append(time.perf_counter() - t)
-    n2 = active()
-    value1 = 0.5 * (n1 + n2) * sub * solo_frames / sum(lat1)
+        lat1 = []
+        for f in range(f0 + 1, f0 + 1 + solo_frames):
+            t = time.perf_counter()
+            cuts(f)
+            for s in ss:
+                s.step(dt, sub, wait=False)
+            for s in ss:
+                s.sync()
+            lat1.append(time.perf_counter() - t)
+        n2 = active()
+        value1 = 0.5 * (n1 + n2) * sub * solo_frames / sum(lat1)
+        solo = {"value": round(value1, 1), "frames": solo_frames,
+                "frame_ms_p50": round(_pct(lat1, 0.50) * 1e3, 3),
+                "frame_ms_p99": round(_pct(lat1, 0.99) * 1e3, 3)}
     for s in ss:
         s.close()
     return {
@@ -223,10 +228,8 @@ def batched_leg(count, frames, warmup, solo_frames, seed=4):
         "graph_launches": int(inf.graph_launches),
         "device_bytes": int(inf.device_bytes),
         "setup_s": round(setup_s, 2),
-        "one_at_a_time": {"value": round(value1, 1), "frames": solo_frames,
-                          "frame_ms_p50": round(_pct(lat1, 0.50) * 1e3, 3),
-                          "frame_ms_p99": round(_pct(lat1, 0.99) * 1e3, 3)},
-        "speedup_vs_one_at_a_time": round(value / value1, 3),
+        "one_at_a_time": solo,
+        "speedup_vs_one_at_a_time": round(value / solo["value"], 3) if solo else None,
     }
 
 
@@ -260,6 +263,8 @@ def main():
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=60)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-solo-frames", type=int, default=-1,
+                    help="frames of the one-at-a-time comparison (-1: a third of --batched-steps, 0: skip)")
     ap.add_argument("--serving-rate", type=float, default=120.0)
     ap.add_argument("--serving-seconds", type=float, default=10.0)
     ap.add_argument("--serving-budget-gb", type=float, default=200.0)
@@ -344,11 +349,11 @@ def main():
 
     batched = None
     if not args.no_batched:
-        batched = batched_leg(args.batched_scenes, args.batched_steps, max(3, args.warmup // 2),
-                              max(10, args.batched_steps // 3))
-        print("bench: batched %.3e p-substeps/s, frame p50 %.2f ms p99 %.2f ms; one at a time %.3e" %
+        solo = args.batched_solo_frames if args.batched_solo_frames >= 0 else max(10, args.batched_steps // 3)
+        batched = batched_leg(args.batched_scenes, args.batched_steps, max(3, args.warmup // 2), solo)
+        print("bench: batched %.3e p-substeps/s, frame p50 %.2f ms p99 %.2f ms; one at a time %s" %
               (batched["value"], batched["frame_ms_p50"], batched["frame_ms_p99"],
-               batched["one_at_a_time"]["value"]), file=sys.stderr, flush=True)
+               batched["one_at_a_time"] and "%.3e" % batched["one_at_a_time"]["value"]), file=sys.stderr, flush=True)
 
     serving = None
     if not args.no_serving:
