@@ -62,26 +62,26 @@ __device__ __forceinline__ uint32_t fbits(float w) { return __float_as_uint(w); 
 __device__ __forceinline__ void run_ranges(const CellGrid& g, const int* __restrict__ lstart, uint32_t k0, int& a0,
                                            int& a1, int& b1) {
   if (k0 + 2u <= g.kmask) {
-    a0 = lstart[k0];
-    a1 = lstart[k0 + 3u];
+    a0 = gld(lstart + k0);
+    a1 = gld(lstart + k0 + 3u);
     b1 = 0;
   } else {
-    a0 = lstart[k0];
-    a1 = lstart[g.M];
-    b1 = lstart[k0 + 3u - g.M];
+    a0 = gld(lstart + k0);
+    a1 = gld(lstart + g.M);
+    b1 = gld(lstart + (k0 + 3u - g.M));
   }
 }
 
 // The same for a run of len (1..3) keys starting at key k (k <= kmask).
 __device__ __forceinline__ void run_ranges_n(const CellGrid& g, const int* __restrict__ lstart, uint32_t k,
                                              uint32_t len, int& a0, int& a1, int& b1) {
-  a0 = lstart[k];
+  a0 = gld(lstart + k);
   if (k + len <= g.M) {
-    a1 = lstart[k + len];
+    a1 = gld(lstart + (k + len));
     b1 = 0;
   } else {
-    a1 = lstart[g.M];
-    b1 = lstart[k + len - g.M];
+    a1 = gld(lstart + g.M);
+    b1 = gld(lstart + (k + len - g.M));
   }
 }
 
@@ -136,14 +136,14 @@ struct Bonds {
 
 __device__ __forceinline__ void load_bonds(int x, const int* __restrict__ adj_off, const int4* __restrict__ adj,
                                            const uint8_t* __restrict__ alive, Bonds& b) {
-  const int k0 = adj_off[x], k1 = adj_off[x + 1];
+  const int k0 = gld(adj_off + x), k1 = gld(adj_off + x + 1);
   b.held = k1 - k0 <= MAXB;
 #pragma unroll
   for (int j = 0; j < MAXB; j++) {
     int v = -1;
     if (b.held && k0 + j < k1) {
-      const int4 a = adj[k0 + j];
-      if (alive[a.y]) v = a.x;
+      const int4 a = gld(adj + k0 + j);
+      if (gld(alive + a.y)) v = a.x;
     }
     b.id[j] = v;
   }
@@ -157,13 +157,13 @@ __device__ __forceinline__ bool is_bonded(int x, int y, const Bonds& b, const in
     for (int j = 0; j < MAXB; j++) hit |= b.id[j] == y;
     return hit;
   }
-  const int e = adj_off[x + 1];
+  const int e = gld(adj_off + x + 1);
   int c = -1;
-  for (int k = adj_off[x]; k < e; k++) {
-    const int4 a = adj[k];
+  for (int k = gld(adj_off + x); k < e; k++) {
+    const int4 a = gld(adj + k);
     c = a.x == y ? a.y : c;
   }
-  return c >= 0 && alive[c] != 0;
+  return c >= 0 && gld(alive + c) != 0;
 }
 
 
@@ -214,11 +214,11 @@ __device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb) 
   float4* __restrict__ pos = A.pos;
   const int t = lb * BLOCK + threadIdx.x;
   const int x = t / CONTACT_LANES, lane = t % CONTACT_LANES;
-  const bool valid = x < fp.ns && active[x];
+  const bool valid = x < fp.ns && gld(active + x);
   float ax = 0.f, ay = 0.f, az = 0.f;
   float4 px = make_float4(0.f, 0.f, 0.f, 0.f);
   if (valid) {
-    px = pos[x];
+    px = gld(pos + x);
     Bonds bonds;
     load_bonds(x, cc.adj_off, cc.adj, cc.alive, bonds);
     const Cell c0 = cell_of(px.x, px.y, px.z, fp.cg.inv_cell);
@@ -232,15 +232,16 @@ __device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb) 
                    w1);
       int q = r0;
       for (; q + 4 <= r1; q += 4) {
-        const int y0 = cc.sid[q], y1 = cc.sid[q + 1], y2 = cc.sid[q + 2], y3 = cc.sid[q + 3];
-        const float4 p0 = cc.spos[q], p1 = cc.spos[q + 1], p2 = cc.spos[q + 2], p3 = cc.spos[q + 3];
+        const int y0 = gld(cc.sid + q), y1 = gld(cc.sid + q + 1), y2 = gld(cc.sid + q + 2), y3 = gld(cc.sid + q + 3);
+        const float4 p0 = gld(cc.spos + q), p1 = gld(cc.spos + q + 1), p2 = gld(cc.spos + q + 2),
+                     p3 = gld(cc.spos + q + 3);
         solid_candidate(fp, cc, x, bonds, px, y0, p0, ax, ay, az);
         solid_candidate(fp, cc, x, bonds, px, y1, p1, ax, ay, az);
         solid_candidate(fp, cc, x, bonds, px, y2, p2, ax, ay, az);
         solid_candidate(fp, cc, x, bonds, px, y3, p3, ax, ay, az);
       }
-      for (; q < r1; q++) solid_candidate(fp, cc, x, bonds, px, cc.sid[q], cc.spos[q], ax, ay, az);
-      for (q = 0; q < w1; q++) solid_candidate(fp, cc, x, bonds, px, cc.sid[q], cc.spos[q], ax, ay, az);
+      for (; q < r1; q++) solid_candidate(fp, cc, x, bonds, px, gld(cc.sid + q), gld(cc.spos + q), ax, ay, az);
+      for (q = 0; q < w1; q++) solid_candidate(fp, cc, x, bonds, px, gld(cc.sid + q), gld(cc.spos + q), ax, ay, az);
     }
   }
 #pragma unroll
@@ -249,7 +250,7 @@ __device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb) 
     ay += __shfl_xor(ay, off, CONTACT_LANES);
     az += __shfl_xor(az, off, CONTACT_LANES);
   }
-  if (valid && lane == 0) pos[x] = make_float4(px.x + ax, px.y + ay, px.z + az, px.w);
+  if (valid && lane == 0) gst(pos + x, make_float4(px.x + ax, px.y + ay, px.z + az, px.w));
 }
 
 __global__ void __launch_bounds__(BLOCK) kg_contacts_solid(const SceneArgs* __restrict__ S,
@@ -273,11 +274,11 @@ __device__ __forceinline__ void contacts_fluid_body(const SceneArgs& A, int lb) 
   uint32_t k1 = 0;
   float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
   if (e < fp.n) {
-    const int x = e < cc.lstart[fp.cg.M] ? cc.sid[e] : -1;
+    const int x = e < gld(cc.lstart + fp.cg.M) ? gld(cc.sid + e) : -1;
     if (x < fp.ns) {
-      tkey[e] = NO_KEY;
+      gst(tkey + e, (uint32_t)NO_KEY);
     } else {
-      const float4 px = cc.spos[e];
+      const float4 px = gld(cc.spos + e);
       const Cell c0 = cell_of(px.x, px.y, px.z, fp.cg.inv_cell);
       float ax = 0.f, ay = 0.f, az = 0.f;
       if (fp.ns > 0) {
@@ -298,14 +299,14 @@ __device__ __forceinline__ void contacts_fluid_body(const SceneArgs& A, int lb) 
 #pragma unroll
             for (uint32_t j = 0; j < 3; j++) {
               const uint32_t k = (kr + j) & fp.cg.kmask;
-              a[j] = cc.lstart[k];
-              b[j] = cc.lstart[k + 1];
-              m[j] = cc.lnsol[k];
+              a[j] = gld(cc.lstart + k);
+              b[j] = gld(cc.lstart + k + 1);
+              m[j] = gld(cc.lnsol + k);
             }
 #pragma unroll
             for (uint32_t j = 0; j < 3; j++) {
               if (j < lo || j >= hi || a[j] == b[j]) continue;
-              for (int q = a[j]; q < a[j] + m[j]; q++) contact_add(fp, px, fp.rf, cc.spos[q], fp.rs, ax, ay, az);
+              for (int q = a[j]; q < a[j] + m[j]; q++) contact_add(fp, px, fp.rf, gld(cc.spos + q), fp.rs, ax, ay, az);
             }
           }
         }
@@ -323,9 +324,9 @@ __device__ __forceinline__ void contacts_fluid_body(const SceneArgs& A, int lb) 
   // consecutive grid slots mostly share a cell: one atomic per run
   const uint32_t slot = wave_bucket_slot(lcount, k1, has);
   if (has) {
-    tkey[e] = k1;
-    tslot[e] = slot;
-    tpos[e] = out;
+    gst(tkey + e, k1);
+    gst(tslot + e, slot);
+    gst(tpos + e, out);
   }
 }
 
@@ -353,26 +354,22 @@ __device__ __forceinline__ int lanes_sum(int v) {
 // ---- the reference's fluid pairs of the particle in slot e ------------------
 
 
+// The particle of a fluid slot as the pair passes see it.  Only the slot,
+// the lane and the position are held; the particle index, its flags (both
+// in p.w) and its cell are derived where needed, which keeps the gather
+// loops' live registers few.
 struct PairSelf {
-  int e, x;
+  int e;
   int v, h;  // lane (over the fluid order) and its index among the slot's lanes
-  bool moved, dup;
-  Cell c;
-  uint32_t gh;
-  float4 p;
+  float4 p;  // w = particle index | flags << PF_SHIFT
+  __device__ __forceinline__ int x() const { return (int)(fbits(p.w) & SID_MASK); }
+  __device__ __forceinline__ bool moved() const { return ((fbits(p.w) >> PF_SHIFT) & PF_MOVED) != 0u; }
+  __device__ __forceinline__ bool dup() const { return ((fbits(p.w) >> PF_SHIFT) & PF_DUP) != 0u; }
 };
 
-// The reference-hash multiplicities, out of line (scalar arguments only,
-// so the call passes registers): probes of cell c against bucket target,
-// and probes of the cell of (x, y, z) against bucket target.
-__device__ __attribute__((noinline)) int mult_cell(double cx, double cy, double cz, uint32_t mask, uint32_t target) {
-  Cell c;
-  c.x = cx;
-  c.y = cy;
-  c.z = cz;
-  return probe_multiplicity(c, mask, target);
-}
-
+// The reference-hash multiplicity, out of line (scalar arguments only, so
+// the call passes registers): probes of the cell of (x, y, z) against
+// bucket target.
 __device__ __attribute__((noinline)) int mult_pos(float x, float y, float z, double inv, uint32_t mask,
                                                   uint32_t target) {
   return probe_multiplicity(cell_of(x, y, z, inv), mask, target);
@@ -387,10 +384,10 @@ __device__ __forceinline__ int slow_weight(const FluidParams& fp, const PairSour
   const int y = (int)(by & SID_MASK);
   const uint32_t fy = by >> PF_SHIFT;
   // y > x: x's probes against y's grid bucket; y < x: y's probes against x's
-  const bool special = (y > me.x) ? (me.dup | ((fy & PF_MOVED) != 0u)) : (me.moved | ((fy & PF_DUP) != 0u));
+  const bool special = (y > me.x()) ? (me.dup() | ((fy & PF_MOVED) != 0u)) : (me.moved() | ((fy & PF_DUP) != 0u));
   if (!special) return 1;
-  return y > me.x ? mult_cell(me.c.x, me.c.y, me.c.z, fp.cg.hmask, src.sphash[q])
-                  : mult_pos(py.x, py.y, py.z, fp.cg.inv_cell, fp.cg.hmask, me.gh);
+  return y > me.x() ? mult_pos(me.p.x, me.p.y, me.p.z, fp.cg.inv_cell, fp.cg.hmask, src.sphash[q])
+                    : mult_pos(py.x, py.y, py.z, fp.cg.inv_cell, fp.cg.hmask, src.sphash[me.e]);
 }
 
 // Weight of the candidate in slot q (0: not a pair of x).  The common case
@@ -407,7 +404,7 @@ __device__ __forceinline__ int pair_weight(const FluidParams& fp, const PairSour
   const bool cand = q != me.e;
   const bool inside = cand & (d2 < fp.H2_lo);
   const bool band = cand & (d2 >= fp.H2_lo) & (d2 < fp.H2_hi);
-  const bool special = (y > me.x) ? (me.dup | ((fy & PF_MOVED) != 0u)) : (me.moved | ((fy & PF_DUP) != 0u));
+  const bool special = (y > me.x()) ? (me.dup() | ((fy & PF_MOVED) != 0u)) : (me.moved() | ((fy & PF_DUP) != 0u));
   int w = inside ? 1 : 0;
   if (__builtin_expect(band | (inside & special), 0)) w = slow_weight(fp, src, me, q, py, d2);
   return w;
@@ -466,6 +463,7 @@ struct DensityGather {
   float rho = 0.f, rhon = 0.f;
   int cnt = 0;
   uint32_t* wp;  // where entry cnt goes while cnt < cap
+  int cap, stride;
 };
 
 __device__ __forceinline__ void density_take(const FluidParams& fp, const PairList& pl, int q, int w, float d2,
@@ -476,8 +474,8 @@ __device__ __forceinline__ void density_take(const FluidParams& fp, const PairLi
   g.rhon += w ? u2 * u : 0.f;
   if (w) {
     // entries past pl.cap all land in the sink row pl.cap, which nothing reads
-    *g.wp = (uint32_t)q | ((uint32_t)(w - 1) << NB_QBITS);
-    g.wp += g.cnt < pl.cap ? pl.stride : 0;
+    gst(g.wp, (uint32_t)q | ((uint32_t)(w - 1) << NB_QBITS));
+    g.wp += g.cnt < g.cap ? g.stride : 0;
     g.cnt++;
   }
 }
@@ -513,18 +511,19 @@ __device__ __forceinline__ void density_batch(const FluidParams& fp, const PairS
 
 __device__ __forceinline__ void load_batch(const float4* __restrict__ row, int q, int last, float4* b) {
 #pragma unroll
-  for (int i = 0; i < DENSITY_BATCH; i++) b[i] = row[min(q + i, last)];
+  for (int i = 0; i < DENSITY_BATCH; i++) b[i] = gld(row + min(q + i, last));
 }
 
 __device__ __forceinline__ void density_gather(const FluidParams& fp, const PairSource& src, const PairList& pl,
                                                const PairSelf& me, DensityGather& g) {
-  const uint32_t kx = cell_key(fp.cg, me.c);
+  const Cell c = cell_of(me.p.x, me.p.y, me.p.z, fp.cg.inv_cell);
+  const uint32_t kx = cell_key(fp.cg, c);
   const float4* __restrict__ row = src.spos2;
-  const bool me_flagged = me.moved | me.dup;
+  const bool me_flagged = me.moved() | me.dup();
   // distances from x to the lower and upper faces of its cell
-  const float fx0 = (float)((double)me.p.x - me.c.x * fp.cs), fx1 = (float)((me.c.x + 1.0) * fp.cs - (double)me.p.x);
-  const float fy0 = (float)((double)me.p.y - me.c.y * fp.cs), fy1 = (float)((me.c.y + 1.0) * fp.cs - (double)me.p.y);
-  const float fz0 = (float)((double)me.p.z - me.c.z * fp.cs), fz1 = (float)((me.c.z + 1.0) * fp.cs - (double)me.p.z);
+  const float fx0 = (float)((double)me.p.x - c.x * fp.cs), fx1 = (float)((c.x + 1.0) * fp.cs - (double)me.p.x);
+  const float fy0 = (float)((double)me.p.y - c.y * fp.cs), fy1 = (float)((c.y + 1.0) * fp.cs - (double)me.p.y);
+  const float fz0 = (float)((double)me.p.z - c.z * fp.cs), fz1 = (float)((c.z + 1.0) * fp.cs - (double)me.p.z);
   RowRange nx = row_range(fp, src.lstart, kx, 0, fx0, fx1, fy0, fy1, fz0, fz1);
 #pragma unroll 1
   for (int r = 0; r < 9; r++) {
@@ -533,22 +532,29 @@ __device__ __forceinline__ void density_gather(const FluidParams& fp, const Pair
     if (rr.r1 > rr.r0) {
       const int last = rr.r1 - 1;
       // this lane's batches of the row: h, h + LANES, ...; the next batch's
-      // loads are in flight while the current one is evaluated
+      // loads are in flight while the current one is evaluated.  Two
+      // buffers alternate (a copy from one to the other would wait for the
+      // loads in flight), and the next batch is loaded unconditionally
+      // (indices clamp to the row's last slot): loads under a branch make
+      // the compiler wait for them as well as for the batch in use.
       constexpr int step = DENSITY_BATCH * FLUID_LANES;
       int q = rr.r0 + DENSITY_BATCH * me.h;
-      float4 b[DENSITY_BATCH];
-      if (q <= last) load_batch(row, q, last, b);
-      for (; q <= last; q += step) {
-        float4 c[DENSITY_BATCH];
-#pragma unroll
-        for (int i = 0; i < DENSITY_BATCH; i++) c[i] = b[i];
-        if (q + step <= last) load_batch(row, q + step, last, b);
-        density_batch(fp, src, pl, me, me_flagged, q, last, c, g);
+      float4 b0[DENSITY_BATCH], b1[DENSITY_BATCH];
+      if (q <= last) load_batch(row, q, last, b0);
+      while (q <= last) {
+        const int q1 = q + step;
+        load_batch(row, q1, last, b1);
+        density_batch(fp, src, pl, me, me_flagged, q, last, b0, g);
+        if (q1 > last) break;
+        const int q2 = q1 + step;
+        load_batch(row, q2, last, b0);
+        density_batch(fp, src, pl, me, me_flagged, q1, last, b1, g);
+        q = q2;
       }
     }
     for (int q = 0; q < (me.h == 0 ? rr.w1 : 0); q++) {
       float dx, dy, dz, d2;
-      const int w = pair_weight(fp, src, me, q, row[q], dx, dy, dz, d2);
+      const int w = pair_weight(fp, src, me, q, gld(row + q), dx, dy, dz, d2);
       density_take(fp, pl, q, w, d2, g);
     }
   }
@@ -559,10 +565,11 @@ __device__ __forceinline__ void density_gather(const FluidParams& fp, const Pair
 template <class F>
 __device__ __forceinline__ void visit_rows_simple(const FluidParams& fp, const PairSource& src, const PairSelf& me,
                                                   F& f) {
-  const uint32_t kx = cell_key(fp.cg, me.c);
-  const float fx0 = (float)((double)me.p.x - me.c.x * fp.cs), fx1 = (float)((me.c.x + 1.0) * fp.cs - (double)me.p.x);
-  const float fy0 = (float)((double)me.p.y - me.c.y * fp.cs), fy1 = (float)((me.c.y + 1.0) * fp.cs - (double)me.p.y);
-  const float fz0 = (float)((double)me.p.z - me.c.z * fp.cs), fz1 = (float)((me.c.z + 1.0) * fp.cs - (double)me.p.z);
+  const Cell c = cell_of(me.p.x, me.p.y, me.p.z, fp.cg.inv_cell);
+  const uint32_t kx = cell_key(fp.cg, c);
+  const float fx0 = (float)((double)me.p.x - c.x * fp.cs), fx1 = (float)((c.x + 1.0) * fp.cs - (double)me.p.x);
+  const float fy0 = (float)((double)me.p.y - c.y * fp.cs), fy1 = (float)((c.y + 1.0) * fp.cs - (double)me.p.y);
+  const float fz0 = (float)((double)me.p.z - c.z * fp.cs), fz1 = (float)((c.z + 1.0) * fp.cs - (double)me.p.z);
 #pragma unroll 1
   for (int r = 0; r < 9; r++) {
     const RowRange rr = row_range(fp, src.lstart, kx, r, fx0, fx1, fy0, fy1, fz0, fz1);
@@ -585,7 +592,7 @@ __device__ __forceinline__ void visit_rows_simple(const FluidParams& fp, const P
 // Replays the recorded pairs of me into f, in the order they were found.
 // Entries are read one batch of four ahead; a batch's gathers (the
 // neighbour's position when F::kNeedPos, and f.fetch(q)) are issued
-// together before any of them is used.
+// together, before the next batch's entries, and before any of them is used.
 template <class F>
 __device__ __forceinline__ void replay_pairs(const PairSource& src, const PairList& pl, const PairSelf& me, int cnt,
                                              F& f) {
@@ -594,22 +601,25 @@ __device__ __forceinline__ void replay_pairs(const PairSource& src, const PairLi
   const int last = cnt - 1;
   uint32_t v[4];
 #pragma unroll
-  for (int i = 0; i < 4; i++) v[i] = cnt > 0 ? row[(size_t)min(i, last) * st] : 0u;
+  for (int i = 0; i < 4; i++) v[i] = cnt > 0 ? gld(row + (size_t)min(i, last) * st) : 0u;
   for (int k = 0; k < cnt; k += 4) {
+    // this batch's gathers first, then the next batch's entries: the loads
+    // complete in issue order, so waiting for the gathers leaves the
+    // entries in flight
+    float4 py[4];
+    typename F::Extra ex[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int q = (int)(v[i] & NB_QMASK);
+      if (F::kNeedPos) py[i] = gld_xyz(src.spos2 + q);
+      ex[i] = f.fetch(q);
+    }
     uint32_t u[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) u[i] = v[i];
     if (k + 4 < cnt) {
 #pragma unroll
-      for (int i = 0; i < 4; i++) v[i] = row[(size_t)min(k + 4 + i, last) * st];
-    }
-    float4 py[4];
-    typename F::Extra ex[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int q = (int)(u[i] & NB_QMASK);
-      if (F::kNeedPos) py[i] = src.spos2[q];
-      ex[i] = f.fetch(q);
+      for (int i = 0; i < 4; i++) v[i] = gld(row + (size_t)min(k + 4 + i, last) * st);
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -634,7 +644,7 @@ __device__ __forceinline__ void replay_pairs(const PairSource& src, const PairLi
 template <class F>
 __device__ __forceinline__ void recorded_pairs(const FluidParams& fp, const PairSource& src, const PairList& pl,
                                                const PairSelf& me, F& f) {
-  const int cnt = pl.ncount[me.v];
+  const int cnt = gld(pl.ncount + me.v);
   // a slot whose lanes did not all fit is gathered again by all its lanes
   if (__builtin_expect(lanes_sum(cnt > pl.cap ? 1 : 0) == 0, 1)) {
     replay_pairs(src, pl, me, cnt, f);
@@ -648,14 +658,8 @@ __device__ __forceinline__ bool pair_self(const FluidParams& fp, const PairSourc
   me.v = v;
   me.h = v % FLUID_LANES;
   me.e = v / FLUID_LANES;
-  if (me.e >= src.lstart[fp.cg.M]) return false;
-  me.p = src.spos2[me.e];
-  const uint32_t b = fbits(me.p.w);
-  me.x = (int)(b & SID_MASK);
-  me.moved = (b >> PF_SHIFT) & PF_MOVED;
-  me.dup = (b >> PF_SHIFT) & PF_DUP;
-  me.c = cell_of(me.p.x, me.p.y, me.p.z, fp.cg.inv_cell);
-  me.gh = src.sphash[me.e];
+  if (me.e >= gld(src.lstart + fp.cg.M)) return false;
+  me.p = gld(src.spos2 + me.e);
   return true;
 }
 
@@ -664,12 +668,12 @@ __device__ __forceinline__ void density_store(const FluidParams& fp, const PairL
                                               const DensityGather& g, float2* __restrict__ sP,
                                               float2* __restrict__ rho_out, int* __restrict__ dflags) {
   const bool over = lanes_sum(g.cnt > pl.cap ? 1 : 0) != 0;
-  pl.ncount[me.v] = g.cnt > pl.cap ? pl.cap + 1 : g.cnt;
+  gst(pl.ncount + me.v, g.cnt > pl.cap ? pl.cap + 1 : g.cnt);
   const float rho = lanes_sum(g.rho), rhon = lanes_sum(g.rhon);
   if (me.h != 0) return;
   if (over) atomicAdd(&dflags[DF_PAIR_OVF], 1);
-  rho_out[me.x] = make_float2(rho, rhon);
-  sP[me.e] = make_float2(fp.stiff * (rho - fp.rest_density), fp.near_stiff * rhon);
+  gst(rho_out + me.x(), make_float2(rho, rhon));
+  gst(sP + me.e, make_float2(fp.stiff * (rho - fp.rest_density), fp.near_stiff * rhon));
 }
 
 __device__ __forceinline__ void density_body(const SceneArgs& A, int lb) {
@@ -682,11 +686,35 @@ __device__ __forceinline__ void density_body(const SceneArgs& A, int lb) {
   if (!pair_self(fp, src, fluid_lane(lb), me)) return;
   DensityGather g;
   g.wp = pl.nbr + me.v;
+  g.cap = pl.cap;
+  g.stride = pl.stride;
   density_gather(fp, src, pl, me, g);
   density_store(fp, pl, me, g, A.sP, A.rho, A.dflags);
 }
 
-__global__ void __launch_bounds__(FLUID_BLOCK) kg_density(const SceneArgs* __restrict__ S,
+// Minimum waves per SIMD asked of the register allocator for the three
+// gather passes (0: the allocator's choice).  These loops are bound by
+// memory latency and issue, so resident waves matter more than registers.
+#ifndef DENSITY_WAVES
+#define DENSITY_WAVES 0
+#endif
+#ifndef DISPLACE_WAVES
+#define DISPLACE_WAVES 0
+#endif
+#ifndef XSPH_WAVES
+#define XSPH_WAVES 0
+#endif
+#define WAVES_ATTR_0
+#define WAVES_ATTR_45 __attribute__((amdgpu_waves_per_eu(4, 5)))
+#define WAVES_ATTR_5 __attribute__((amdgpu_waves_per_eu(5)))
+#define WAVES_ATTR_6 __attribute__((amdgpu_waves_per_eu(6)))
+#define WAVES_ATTR_7 __attribute__((amdgpu_waves_per_eu(7)))
+#define WAVES_ATTR_8 __attribute__((amdgpu_waves_per_eu(8)))
+#define WAVES_ATTR_(n) WAVES_ATTR_##n
+#define WAVES_ATTR(n) WAVES_ATTR_(n)
+#define DENSITY_ATTR WAVES_ATTR(DENSITY_WAVES)
+
+__global__ void __launch_bounds__(FLUID_BLOCK) DENSITY_ATTR kg_density(const SceneArgs* __restrict__ S,
                                                           const int2* __restrict__ W, int nw) {
   FOR_ITEMS(S, W, nw) density_body(A, lb);
 }
@@ -698,7 +726,7 @@ struct DisplaceAcc {
   float2 Px;
   float inv_H, max_pair;
   float sx = 0.f, sy = 0.f, sz = 0.f;
-  __device__ __forceinline__ float2 fetch(int q) const { return sP[q]; }
+  __device__ __forceinline__ float2 fetch(int q) const { return gld(sP + q); }
   __device__ __forceinline__ void use(int, int w, float dx, float dy, float dz, float d2, float2 Py) {
     const float d = fast_sqrt(d2);
     if (d < 1e-9f) return;
@@ -727,7 +755,7 @@ __device__ __forceinline__ void displace_body(const SceneArgs& A, int lb) {
   if (!pair_self(fp, src, fluid_lane(lb), me)) return;
   DisplaceAcc acc;
   acc.sP = sP;
-  acc.Px = sP[me.e];
+  acc.Px = gld(sP + me.e);
   acc.inv_H = fp.inv_H;
   acc.max_pair = fp.max_pair;
   recorded_pairs(fp, src, pl, me, acc);
@@ -740,11 +768,11 @@ __device__ __forceinline__ void displace_body(const SceneArgs& A, int lb) {
     sy *= g;
     sz *= g;
   }
-  const float w = pos[me.x].w;
-  pos[me.x] = make_float4(me.p.x + sx, me.p.y + sy, me.p.z + sz, w);
+  const float w = gld(pos + me.x()).w;
+  gst(pos + me.x(), make_float4(me.p.x + sx, me.p.y + sy, me.p.z + sz, w));
 }
 
-__global__ void __launch_bounds__(FLUID_BLOCK) kg_displace(const SceneArgs* __restrict__ S,
+__global__ void __launch_bounds__(FLUID_BLOCK) WAVES_ATTR(DISPLACE_WAVES) kg_displace(const SceneArgs* __restrict__ S,
                                                            const int2* __restrict__ W, int nw) {
   FOR_ITEMS(S, W, nw) displace_body(A, lb);
 }
@@ -755,7 +783,7 @@ struct XsphAcc {
   const float4* __restrict__ svtmp;
   float sx = 0.f, sy = 0.f, sz = 0.f;
   int m = 0;
-  __device__ __forceinline__ float4 fetch(int q) const { return svtmp[q]; }
+  __device__ __forceinline__ float4 fetch(int q) const { return gld_xyz(svtmp + q); }
   __device__ __forceinline__ void use(int, int w, float, float, float, float, float4 v) {
     const float fw = (float)w;
     sx += fw * v.x;
@@ -779,24 +807,24 @@ __device__ __forceinline__ void xsph_body(const SceneArgs& A, int lb) {
   const uint8_t* __restrict__ active = A.active;
   PairSelf me;
   if (!pair_self(fp, src, fluid_lane(lb), me)) return;
-  if (!active[me.x]) return;
+  if (!gld(active + me.x())) return;
   XsphAcc acc;
   acc.svtmp = svtmp;
   recorded_pairs(fp, src, pl, me, acc);
   const int m = lanes_sum(acc.m);
   const float sx = lanes_sum(acc.sx), sy = lanes_sum(acc.sy), sz = lanes_sum(acc.sz);
   if (me.h != 0) return;
-  const float4 v = svtmp[me.e];
+  const float4 v = gld(svtmp + me.e);
   if (m == 0) {
-    vel[me.x] = v;
+    gst(vel + me.x(), v);
     return;
   }
   const float inv_m = 1.f / (float)m;
-  vel[me.x] = make_float4(v.x + fp.visc * (sx * inv_m - v.x), v.y + fp.visc * (sy * inv_m - v.y),
-                          v.z + fp.visc * (sz * inv_m - v.z), 0.f);
+  gst(vel + me.x(), make_float4(v.x + fp.visc * (sx * inv_m - v.x), v.y + fp.visc * (sy * inv_m - v.y),
+                                v.z + fp.visc * (sz * inv_m - v.z), 0.f));
 }
 
-__global__ void __launch_bounds__(FLUID_BLOCK) kg_xsph(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
+__global__ void __launch_bounds__(FLUID_BLOCK) WAVES_ATTR(XSPH_WAVES) kg_xsph(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
                                                        int nw) {
   FOR_ITEMS(S, W, nw) xsph_body(A, lb);
 }

@@ -158,6 +158,56 @@ __device__ __forceinline__ uint32_t row_key(const CellGrid& g, uint32_t k, int a
   return (k + (uint32_t)a * g.mx + (uint32_t)b * g.my - 1u) & g.kmask;
 }
 
+// ---- global-memory access -------------------------------------------------------
+//
+// The kernels read their arrays' addresses from a device table of scenes, so
+// the compiler cannot see that they point to global memory and emits flat
+// loads and stores.  A flat access also counts against the counter that
+// scalar loads wait on: every reload of a scene parameter inside a loop then
+// waits for all the vector loads in flight, prefetches included.  gld / gst
+// access through the global address space instead.
+#define FP_GLOBAL __attribute__((address_space(1)))
+typedef float fp_f4v __attribute__((ext_vector_type(4)));
+typedef float fp_f2v __attribute__((ext_vector_type(2)));
+typedef int fp_i4v __attribute__((ext_vector_type(4)));
+typedef float fp_f3v __attribute__((ext_vector_type(3)));
+
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const FP_GLOBAL T*)p;
+}
+__device__ __forceinline__ float4 gld(const float4* p) {
+  const fp_f4v v = *(const FP_GLOBAL fp_f4v*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 gld(const float2* p) {
+  const fp_f2v v = *(const FP_GLOBAL fp_f2v*)p;
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ int4 gld(const int4* p) {
+  const fp_i4v v = *(const FP_GLOBAL fp_i4v*)p;
+  return make_int4(v.x, v.y, v.z, v.w);
+}
+// x, y, z of a float4 (w = 0) with a 12-byte load: a 16-byte load whose w is
+// dead still writes its register, and the compiler then waits for the load
+// before it can reuse that register.
+__device__ __forceinline__ float4 gld_xyz(const float4* p) {
+  const fp_f3v v = *(const FP_GLOBAL fp_f3v*)p;
+  return make_float4(v.x, v.y, v.z, 0.f);
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  *(FP_GLOBAL T*)p = v;
+}
+__device__ __forceinline__ void gst(float4* p, const float4& v) {
+  fp_f4v w = {v.x, v.y, v.z, v.w};
+  *(FP_GLOBAL fp_f4v*)p = w;
+}
+__device__ __forceinline__ void gst(float2* p, const float2& v) {
+  fp_f2v w = {v.x, v.y};
+  *(FP_GLOBAL fp_f2v*)p = w;
+}
+
 // ---- solids' bounding box ------------------------------------------------------
 
 // Floats as ints whose signed order is the floats' order.
