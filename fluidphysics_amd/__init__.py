@@ -413,6 +413,10 @@ class Batch:
     def graph(self, enable=True):
         self._check(self._lib.fp_batch_set_graph(self._h, 1 if enable else 0))
 
+    def contact_lanes(self, lanes=0):
+        """Pin the solid contact pass's lanes per solid (4..32; 0 = tune)."""
+        self._check(self._lib.fp_batch_set_contact_lanes(self._h, int(lanes)))
+
     def info(self):
         inf = _abi.FpBatchInfo()
         self._check(self._lib.fp_batch_get_info(self._h, ctypes.byref(inf)))

@@ -112,6 +112,8 @@ class FpBatchInfo(ctypes.Structure):
         ("device_bytes", ctypes.c_int64),
         ("frames", ctypes.c_int64),
         ("graph_launches", ctypes.c_int64),
+        ("contact_lanes", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -172,6 +174,7 @@ SIGNATURES = {
     "fp_batch_step_async": (_I, [_P, _D, _I]),
     "fp_batch_sync": (_I, [_P]),
     "fp_batch_set_graph": (_I, [_P, _I]),
+    "fp_batch_set_contact_lanes": (_I, [_P, _I]),
     "fp_batch_get_info": (_I, [_P, ctypes.POINTER(FpBatchInfo)]),
     "fp_batch_profile_enable": (_I, [_P, _I]),
     "fp_batch_profile_read": (_I, [_P, ctypes.POINTER(FpPassStat), _I, _IP]),

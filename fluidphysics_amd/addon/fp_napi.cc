@@ -541,6 +541,14 @@ napi_value BatchSetGraph(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+napi_value BatchSetContactLanes(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_batch* b = bhandle(env, a.v[0]);
+  FPB_CALL(env, b, fp_batch_set_contact_lanes(b, (int)num(env, a.v[1], 0)), "fp_batch_set_contact_lanes");
+  return nullptr;
+}
+
 napi_value BatchInfo(napi_env env, napi_callback_info info) {
   Args a;
   NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
@@ -555,6 +563,7 @@ napi_value BatchInfo(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, o, "deviceBytes", make_int(env, i.device_bytes));
   napi_set_named_property(env, o, "frames", make_int(env, i.frames));
   napi_set_named_property(env, o, "graphLaunches", make_int(env, i.graph_launches));
+  napi_set_named_property(env, o, "contactLanes", make_int(env, i.contact_lanes));
   return o;
 }
 
@@ -636,6 +645,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy}, {"batchAdd", BatchAdd},
       {"batchRemove", BatchRemove}, {"batchStep", BatchStep},       {"batchSync", BatchSync},
       {"batchSetGraph", BatchSetGraph}, {"batchInfo", BatchInfo},   {"batchSnapshot", BatchSnapshot},
+      {"batchSetContactLanes", BatchSetContactLanes},
       {"sceneBytes", SceneBytes},   {"poolInfo", PoolInfo},
   };
   for (auto& f : fns) {

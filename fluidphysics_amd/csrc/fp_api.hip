@@ -1011,6 +1011,17 @@ int fp_batch_set_graph(fp_batch* b, int32_t enable) {
   return FP_OK;
 }
 
+int fp_batch_set_contact_lanes(fp_batch* b, int32_t lanes) {
+  if (!b) return batch_fail(nullptr, FP_EINVAL, "fp_batch_set_contact_lanes: NULL batch");
+  if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32)
+    return batch_fail(b, FP_EINVAL, "fp_batch_set_contact_lanes: lanes must be 0, 4, 8, 16 or 32");
+  b->g.clanes_pinned = lanes != 0;
+  b->g.clanes_best = lanes;
+  b->g.tune_k = 0;
+  b->g.tune_pending = -1;
+  return FP_OK;
+}
+
 int fp_batch_get_info(const fp_batch* b, fp_batch_info* out) {
   if (!b || !out) return FP_EINVAL;
   std::memset(out, 0, sizeof(*out));
@@ -1022,6 +1033,7 @@ int fp_batch_get_info(const fp_batch* b, fp_batch_info* out) {
   }
   out->frames = b->frames;
   out->graph_launches = b->g.graph_launches;
+  out->contact_lanes = b->g.clanes_best;
   return FP_OK;
 }
 

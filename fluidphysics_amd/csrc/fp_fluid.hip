@@ -52,8 +52,7 @@
 #ifndef DENSITY_BATCH
 #define DENSITY_BATCH 2  // candidates per step of the density loop
 #endif
-#define CONTACT_LANES 4  // lanes per solid in the contact pass
-#define MAXB 32          // bonds per solid held in registers
+#define MAXB 32          // bonds per solid staged in LDS for the contact pass
 
 __device__ __forceinline__ uint32_t fbits(float w) { return __float_as_uint(w); }
 
@@ -126,47 +125,6 @@ __device__ __forceinline__ bool row_cells(const FaceDist& f, int r, float r2, ui
 
 // ---- contacts -----------------------------------------------------------------
 
-// The alive bonds of solid x, in registers: the other endpoints of x's
-// alive constraints (-1 past the end).  Rows longer than MAXB are not held;
-// is_bonded scans the CSR row for them.
-struct Bonds {
-  int id[MAXB];
-  bool held;
-};
-
-__device__ __forceinline__ void load_bonds(int x, const int* __restrict__ adj_off, const int4* __restrict__ adj,
-                                           const uint8_t* __restrict__ alive, Bonds& b) {
-  const int k0 = gld(adj_off + x), k1 = gld(adj_off + x + 1);
-  b.held = k1 - k0 <= MAXB;
-#pragma unroll
-  for (int j = 0; j < MAXB; j++) {
-    int v = -1;
-    if (b.held && k0 + j < k1) {
-      const int4 a = gld(adj + k0 + j);
-      if (gld(alive + a.y)) v = a.x;
-    }
-    b.id[j] = v;
-  }
-}
-
-__device__ __forceinline__ bool is_bonded(int x, int y, const Bonds& b, const int* __restrict__ adj_off,
-                                          const int4* __restrict__ adj, const uint8_t* __restrict__ alive) {
-  if (b.held) {
-    bool hit = false;
-#pragma unroll
-    for (int j = 0; j < MAXB; j++) hit |= b.id[j] == y;
-    return hit;
-  }
-  const int e = gld(adj_off + x + 1);
-  int c = -1;
-  for (int k = gld(adj_off + x); k < e; k++) {
-    const int4 a = gld(adj + k);
-    c = a.x == y ? a.y : c;
-  }
-  return c >= 0 && gld(alive + c) != 0;
-}
-
-
 __device__ __forceinline__ float unorder(int v) { return __int_as_float(v >= 0 ? v : v ^ 0x7fffffff); }
 
 // The Jacobi correction of x from y (solver.js:267-275 with x's share).
@@ -183,79 +141,132 @@ __device__ __forceinline__ void contact_add(const FluidParams& fp, const float4&
   az += dz * f;
 }
 
-// Solid x: CONTACT_LANES lanes split its 9 rows (row r to lane r % lanes),
-// each sums its candidates in slot order, then the lanes' sums are added
-// pairwise.  Writes pos[x] = grid-time position + correction.
-__device__ __forceinline__ void solid_candidate(const FluidParams& fp, const ContactCtx& cc, int x, const Bonds& bonds,
-                                                const float4& px, int y, const float4& py, float& ax, float& ay,
-                                                float& az) {
-  if (y == x) return;
-  if (y < fp.ns) {
-    // a bonded pair does not collide; test the bond only when close
-    const float dx = px.x - py.x, dy = px.y - py.y, dz = px.z - py.z;
-    const float d2 = dx * dx + dy * dy + dz * dz;
-    const float rsum = fp.rs + fp.rs;
-    if (d2 >= rsum * rsum || d2 < 1e-12f) return;
-    if (is_bonded(x, y, bonds, cc.adj_off, cc.adj, cc.alive)) return;
-    contact_add(fp, px, fp.rs, py, fp.rs, ax, ay, az);
-  } else {
-    contact_add(fp, px, fp.rs, py, fp.rf, ax, ay, az);
+// Is y a bonded (alive constraint) neighbour of x?  `held` lists the other
+// endpoints of x's alive constraints (-1 past the end) when x has at most
+// MAXB of them; otherwise x's CSR row is scanned.
+__device__ __forceinline__ bool bonded(int x, int y, bool held, const int* held_ids, const ContactCtx& cc) {
+  if (held) {
+    bool hit = false;
+#pragma unroll 8
+    for (int j = 0; j < MAXB; j++) hit |= held_ids[j] == y;
+    return hit;
   }
+  const int e = gld(cc.adj_off + x + 1);
+  int c = -1;
+  for (int k = gld(cc.adj_off + x); k < e; k++) {
+    const int4 a = gld(cc.adj + k);
+    c = a.x == y ? a.y : c;
+  }
+  return c >= 0 && gld(cc.alive + c) != 0;
 }
 
-// Solid x: CONTACT_LANES lanes split its 9 rows (row r to lane r % lanes),
-// each sums its candidates in slot order (cells beyond the contact range
-// skipped, loads four at a time), then the lanes' sums are added pairwise.
+// Solid x: L lanes (4..32) share its 27 neighbour cells -- cell u of the
+// row-major 3 x 3 x 3 block goes to lane u % L; cells beyond the contact
+// range are skipped -- and x's bonds are staged once in LDS and consulted
+// only for a solid candidate within contact range.  Each cell's candidates
+// are summed in slot order into its own partial; the 32 partials (cells 27..31
+// empty) are then added by one fixed binary tree, pairs differing in the
+// highest bit of the cell number first: inside a lane for the bits above
+// log2(L), then across lanes.  Every L builds the same tree from the same
+// partials, so the correction is the same bits whatever L the launch uses.
 // Writes pos[x] = grid-time position + correction.
-__device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb) {
+template <int L>
+__device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb, int* __restrict__ lds_bonds) {
+  constexpr int M = 32 / L;  // partials per lane
   const FluidParams& fp = A.fp;
   const ContactCtx& cc = A.cc;
-  const uint8_t* __restrict__ active = A.active;
   float4* __restrict__ pos = A.pos;
   const int t = lb * BLOCK + threadIdx.x;
-  const int x = t / CONTACT_LANES, lane = t % CONTACT_LANES;
-  const bool valid = x < fp.ns && gld(active + x);
-  float ax = 0.f, ay = 0.f, az = 0.f;
+  const int x = t / L, lane = t % L;
+  int* held_ids = lds_bonds + (threadIdx.x / L) * MAXB;
+  const bool valid = x < fp.ns && gld(A.active + x);
+  int k0 = 0, k1 = 0;
+  if (valid) {
+    k0 = gld(cc.adj_off + x);
+    k1 = gld(cc.adj_off + x + 1);
+  }
+  const bool held = k1 - k0 <= MAXB;
+#pragma unroll
+  for (int j = lane; j < MAXB; j += L) {
+    int v = -1;
+    if (valid && held && k0 + j < k1) {
+      const int4 a = gld(cc.adj + k0 + j);
+      if (gld(cc.alive + a.y)) v = a.x;
+    }
+    held_ids[j] = v;
+  }
+  __syncthreads();
+  float px_[M], py_[M], pz_[M];
+#pragma unroll
+  for (int j = 0; j < M; j++) px_[j] = py_[j] = pz_[j] = 0.f;
   float4 px = make_float4(0.f, 0.f, 0.f, 0.f);
   if (valid) {
     px = gld(pos + x);
-    Bonds bonds;
-    load_bonds(x, cc.adj_off, cc.adj, cc.alive, bonds);
     const Cell c0 = cell_of(px.x, px.y, px.z, fp.cg.inv_cell);
-    const uint32_t k0 = cell_key(fp.cg, c0);
+    const uint32_t kx = cell_key(fp.cg, c0);
     const FaceDist fd = face_dist(px.x, px.y, px.z, c0, fp.cs);
-    for (int r = lane; r < 9; r += CONTACT_LANES) {
+    const float rr2 = 4.f * fp.rs * fp.rs;
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+      const int u = lane + L * j;
+      if (u >= 27) continue;
+      const int r = u / 3;
+      const uint32_t z = (uint32_t)(u % 3);
       uint32_t lo, hi;
-      if (!row_cells(fd, r, fp.contact_r2, lo, hi)) continue;
-      int r0, r1, w1;
-      run_ranges_n(fp.cg, cc.lstart, (row_key(fp.cg, k0, r / 3 - 1, r % 3 - 1) + lo) & fp.cg.kmask, hi - lo, r0, r1,
-                   w1);
-      int q = r0;
-      for (; q + 4 <= r1; q += 4) {
-        const int y0 = gld(cc.sid + q), y1 = gld(cc.sid + q + 1), y2 = gld(cc.sid + q + 2), y3 = gld(cc.sid + q + 3);
-        const float4 p0 = gld(cc.spos + q), p1 = gld(cc.spos + q + 1), p2 = gld(cc.spos + q + 2),
-                     p3 = gld(cc.spos + q + 3);
-        solid_candidate(fp, cc, x, bonds, px, y0, p0, ax, ay, az);
-        solid_candidate(fp, cc, x, bonds, px, y1, p1, ax, ay, az);
-        solid_candidate(fp, cc, x, bonds, px, y2, p2, ax, ay, az);
-        solid_candidate(fp, cc, x, bonds, px, y3, p3, ax, ay, az);
+      if (!row_cells(fd, r, fp.contact_r2, lo, hi) || z < lo || z >= hi) continue;
+      const uint32_t k = (row_key(fp.cg, kx, r / 3 - 1, r % 3 - 1) + z) & fp.cg.kmask;
+      const int q0 = gld(cc.lstart + k), q1 = gld(cc.lstart + k + 1);
+      float ax = 0.f, ay = 0.f, az = 0.f;
+      for (int q = q0; q < q1; q += 2) {
+        const int qb = min(q + 1, q1 - 1);
+        const int y0 = gld(cc.sid + q), y1 = gld(cc.sid + qb);
+        const float4 p0 = gld(cc.spos + q), p1 = gld(cc.spos + qb);
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+          const int y = i ? y1 : y0;
+          const float4& py = i ? p1 : p0;
+          if ((i && q + 1 >= q1) || y == x) continue;
+          if (y < fp.ns) {
+            // a bonded pair does not collide; test the bond only when close
+            const float dx = px.x - py.x, dy = px.y - py.y, dz = px.z - py.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 >= rr2 || d2 < 1e-12f) continue;
+            if (bonded(x, y, held, held_ids, cc)) continue;
+            contact_add(fp, px, fp.rs, py, fp.rs, ax, ay, az);
+          } else {
+            contact_add(fp, px, fp.rs, py, fp.rf, ax, ay, az);
+          }
+        }
       }
-      for (; q < r1; q++) solid_candidate(fp, cc, x, bonds, px, gld(cc.sid + q), gld(cc.spos + q), ax, ay, az);
-      for (q = 0; q < w1; q++) solid_candidate(fp, cc, x, bonds, px, gld(cc.sid + q), gld(cc.spos + q), ax, ay, az);
+      px_[j] = ax;
+      py_[j] = ay;
+      pz_[j] = az;
     }
   }
 #pragma unroll
-  for (int off = 1; off < CONTACT_LANES; off <<= 1) {
-    ax += __shfl_xor(ax, off, CONTACT_LANES);
-    ay += __shfl_xor(ay, off, CONTACT_LANES);
-    az += __shfl_xor(az, off, CONTACT_LANES);
+  for (int half = M / 2; half >= 1; half /= 2)
+#pragma unroll
+    for (int j = 0; j < half; j++) {
+      px_[j] += px_[j + half];
+      py_[j] += py_[j + half];
+      pz_[j] += pz_[j + half];
+    }
+  float ax = px_[0], ay = py_[0], az = pz_[0];
+#pragma unroll
+  for (int off = L / 2; off >= 1; off /= 2) {
+    ax += __shfl_xor(ax, off, L);
+    ay += __shfl_xor(ay, off, L);
+    az += __shfl_xor(az, off, L);
   }
   if (valid && lane == 0) gst(pos + x, make_float4(px.x + ax, px.y + ay, px.z + az, px.w));
+  __syncthreads();  // the block's next item reuses lds_bonds
 }
 
+template <int L>
 __global__ void __launch_bounds__(BLOCK) kg_contacts_solid(const SceneArgs* __restrict__ S,
                                                            const int2* __restrict__ W, int nw) {
-  FOR_ITEMS(S, W, nw) contacts_solid_body(A, lb);
+  __shared__ int lds_bonds[(BLOCK / L) * MAXB];
+  FOR_ITEMS(S, W, nw) contacts_solid_body<L>(A, lb, lds_bonds);
 }
 
 // Fluid in grid slot e: corrections from the solids of its 27 cells (rows in
@@ -856,19 +867,20 @@ FluidParams fp_fluid_params(const fp_solver* s) {
   fp.near_stiff = (float)P.near_stiffness;
   fp.rest_density = (float)P.rest_density;
   fp.contact_scale = (float)P.contact_scale;
+
   fp.visc = (float)P.viscosity;
   return fp;
 }
 
 // Blocks of one scene in a pass of the given kind (fp_internal.h WorkKind).
-int fp_scene_blocks(const fp_solver* s, int kind) {
+int fp_scene_blocks(const fp_solver* s, int kind, int contact_lanes) {
   switch (kind) {
     case W_PARTICLE:
       return fp_blocks(s->n);
     case W_SCAN:
       return s->lscan_blocks;
     case W_SOLID:
-      return fp_blocks(s->ns * CONTACT_LANES);
+      return fp_blocks(s->ns * contact_lanes);
     case W_FLUID:
       return (int)(((int64_t)s->n * FLUID_LANES + FLUID_BLOCK - 1) / FLUID_BLOCK);
     case W_CONS:
@@ -881,7 +893,31 @@ int fp_scene_blocks(const fp_solver* s, int kind) {
 
 int fp_group_contacts(fp_group* g) {
   PassScope ps(g->prof, g->stream, P_CONTACTS);
-  GROUP_LAUNCH(g, kg_contacts_solid, W_SOLID, BLOCK);
+  hipEvent_t* ev = nullptr;
+  if (g->tune_record) {
+    const size_t need = 2 * (size_t)(g->tune_used + 1);
+    while (g->tune_ev.size() < need) {
+      hipEvent_t e;
+      FP_GTRY(g, hipEventCreate(&e));
+      g->tune_ev.push_back(e);
+    }
+    ev = &g->tune_ev[2 * (size_t)g->tune_used++];
+    FP_GTRY(g, hipEventRecord(ev[0], g->stream));
+  }
+  switch (g->clanes) {
+    case 32:
+      GROUP_LAUNCH(g, kg_contacts_solid<32>, W_SOLID, BLOCK);
+      break;
+    case 16:
+      GROUP_LAUNCH(g, kg_contacts_solid<16>, W_SOLID, BLOCK);
+      break;
+    case 8:
+      GROUP_LAUNCH(g, kg_contacts_solid<8>, W_SOLID, BLOCK);
+      break;
+    default:
+      GROUP_LAUNCH(g, kg_contacts_solid<4>, W_SOLID, BLOCK);
+  }
+  if (ev) FP_GTRY(g, hipEventRecord(ev[1], g->stream));
   GROUP_LAUNCH(g, kg_contacts_fluid, W_PARTICLE, BLOCK);
   FP_GTRY(g, hipGetLastError());
   return FP_OK;

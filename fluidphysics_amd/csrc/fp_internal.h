@@ -267,7 +267,7 @@ struct SceneArgs {
 enum WorkKind {
   W_PARTICLE,  // one thread per particle slot (BLOCK per block)
   W_SCAN,      // one scan tile of the cell table
-  W_SOLID,     // CONTACT_LANES threads per solid
+  W_SOLID,     // contact lanes (4..32, chosen per group) per solid
   W_FLUID,     // FLUID_LANES threads per particle slot (FLUID_BLOCK per block)
   W_CONS,      // 32 threads per solid
   W_CONSTR,    // one thread per constraint
@@ -301,6 +301,21 @@ struct fp_group {
   int graph_iters = -1;
   bool use_graph = true;
   int64_t graph_launches = 0;
+  // autotuned launch shape: lanes per solid of the solid contact pass (the
+  // result does not depend on it).  The first frames try 4, 8, 16 and 32
+  // lanes, timed with events, and the fastest is kept; a group whose solid
+  // count changes by more than 2x tunes again.
+  int clanes = 4;                 // lanes of the current frame
+  int items_clanes = 0;           // lanes the work lists were built for
+  int clanes_best = 0;            // 0 while tuning
+  bool clanes_pinned = false;     // set by the caller: no tuning
+  int tune_k = 0;                 // candidate being timed
+  int tune_pending = -1;          // candidate whose events await reading
+  bool tune_record = false;       // record events around the contact launches
+  int tune_used = 0;              // event pairs recorded in the pending frame
+  float tune_ms[4] = {0.f, 0.f, 0.f, 0.f};
+  int64_t tuned_solids = 0;
+  std::vector<hipEvent_t> tune_ev;
   std::string err;
 };
 
@@ -377,7 +392,7 @@ static inline int fp_blocks(int n) { return (n + BLOCK - 1) / BLOCK; }
 
 FluidParams fp_fluid_params(const fp_solver* s);
 SceneArgs fp_scene_args(const fp_solver* s, double h);
-int fp_scene_blocks(const fp_solver* s, int kind);
+int fp_scene_blocks(const fp_solver* s, int kind, int contact_lanes);
 
 // ---- group launches (one per pass, all scenes) -----------------------------
 
@@ -394,6 +409,8 @@ int fp_group_constraints(fp_group* g, int iters);
 // fp_step.hip
 int fp_group_frame(fp_group* g, double dt, int substeps);
 int fp_group_prepare(fp_group* g, double h);
+void fp_group_tune_begin(fp_group* g);
+int fp_group_tune_end(fp_group* g);
 int fp_group_snapshot(fp_group* g, float* out);
 void fp_group_free(fp_group* g);
 

@@ -270,15 +270,17 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
 }
 
 void fp_group_free(fp_group* g) {
+  for (hipEvent_t e : g->tune_ev) (void)hipEventDestroy(e);
+  g->tune_ev.clear();
   if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   g->gexec = nullptr;
   g->graph = nullptr;
-  if (g->d_args) (void)hipFree(g->d_args);
+  if (g->d_args) fp_pool_put(g->device, g->d_args);
   g->d_args = nullptr;
   g->args_cap = 0;
   for (int k = 0; k < W_NUM; k++) {
-    if (g->d_items[k]) (void)hipFree(g->d_items[k]);
+    if (g->d_items[k]) fp_pool_put(g->device, g->d_items[k]);
     g->d_items[k] = nullptr;
     g->items_cap[k] = 0;
     g->n_items[k] = 0;
@@ -296,7 +298,8 @@ static void drop_graph(fp_group* g) {
 // Scene table and work lists for substep h: rebuilt when the members, their
 // parameters or h changed.  A changed membership also drops the graph.
 int fp_group_prepare(fp_group* g, double h) {
-  bool changed = g->dirty || g->built_h != h || g->versions.size() != g->scenes.size();
+  bool changed = g->dirty || g->built_h != h || g->versions.size() != g->scenes.size() ||
+                 g->items_clanes != g->clanes;
   if (!changed)
     for (size_t k = 0; k < g->scenes.size(); k++)
       if (g->versions[k] != g->scenes[k]->version) changed = true;
@@ -325,36 +328,47 @@ int fp_group_prepare(fp_group* g, double h) {
     snap += 3 * (int64_t)g->scenes[k]->n;
   }
   if (ns > g->args_cap) {
-    if (g->d_args) (void)hipFree(g->d_args);
+    FP_GTRY(g, hipStreamSynchronize(g->stream));
+    if (g->d_args) fp_pool_put(g->device, g->d_args);
     g->d_args = nullptr;
-    FP_GTRY(g, hipMalloc((void**)&g->d_args, sizeof(SceneArgs) * std::max(ns, 1)));
+    g->args_cap = 0;
+    FP_GTRY(g, fp_pool_get(g->device, sizeof(SceneArgs) * std::max(ns, 1), (void**)&g->d_args, nullptr));
     g->args_cap = ns;
     drop_graph(g);
   }
   if (ns) {
     FP_GTRY(g, hipMemcpyAsync(g->d_args, args.data(), sizeof(SceneArgs) * ns, hipMemcpyHostToDevice, g->stream));
   }
-  if (g->dirty || g->versions.size() != g->scenes.size()) {
+  // work lists: rebuilt for a new membership, or when a parameter change
+  // (contact_lanes) changed a scene's block count; either drops the graph
+  std::vector<int2> items[W_NUM];
+  bool rebuild = g->dirty || g->versions.size() != g->scenes.size() || g->items_clanes != g->clanes;
+  for (int kind = 0; kind < W_NUM; kind++) {
+    for (int k = 0; k < ns; k++) {
+      const int nb = fp_scene_blocks(g->scenes[k], kind, g->clanes);
+      for (int b = 0; b < nb; b++) items[kind].push_back(make_int2(k, b));
+    }
+    rebuild |= (int)items[kind].size() != g->n_items[kind];
+  }
+  if (rebuild) {
     drop_graph(g);
     for (int kind = 0; kind < W_NUM; kind++) {
-      std::vector<int2> items;
-      for (int k = 0; k < ns; k++) {
-        const int nb = fp_scene_blocks(g->scenes[k], kind);
-        for (int b = 0; b < nb; b++) items.push_back(make_int2(k, b));
-      }
-      const int n = (int)items.size();
+      const int n = (int)items[kind].size();
       if (n > g->items_cap[kind]) {
-        if (g->d_items[kind]) (void)hipFree(g->d_items[kind]);
+        FP_GTRY(g, hipStreamSynchronize(g->stream));
+        if (g->d_items[kind]) fp_pool_put(g->device, g->d_items[kind]);
         g->d_items[kind] = nullptr;
-        FP_GTRY(g, hipMalloc((void**)&g->d_items[kind], sizeof(int2) * std::max(n, 1)));
+        g->items_cap[kind] = 0;
+        FP_GTRY(g, fp_pool_get(g->device, sizeof(int2) * std::max(n, 1), (void**)&g->d_items[kind], nullptr));
         g->items_cap[kind] = n;
       }
       if (n) {
-        FP_GTRY(g, hipMemcpyAsync(g->d_items[kind], items.data(), sizeof(int2) * n, hipMemcpyHostToDevice,
+        FP_GTRY(g, hipMemcpyAsync(g->d_items[kind], items[kind].data(), sizeof(int2) * n, hipMemcpyHostToDevice,
                                   g->stream));
       }
       g->n_items[kind] = n;
     }
+    g->items_clanes = g->clanes;
   }
   // the host vectors are read by the copies above before they go out of scope
   FP_GTRY(g, hipStreamSynchronize(g->stream));
@@ -404,12 +418,68 @@ static int group_substep(fp_group* g) {
 // One frame (substeps of dt / substeps) of every scene of the group, queued
 // on the group's stream.  Without grabs or profiling the frame is captured
 // into a HIP graph once and replayed.
+// ---- contact-lane autotuning ---------------------------------------------------
+
+static const int kTuneLanes[4] = {4, 8, 16, 32};
+
+// Before a frame: read the timing of the last tuning frame, pick the lane
+// count for this one.
+void fp_group_tune_begin(fp_group* g) {
+  int64_t solids = 0;
+  for (fp_solver* s : g->scenes) solids += s->ns;
+  if (g->clanes_best && !g->clanes_pinned && (solids > 2 * g->tuned_solids || 2 * solids < g->tuned_solids)) {
+    g->clanes_best = 0;  // the work changed size: tune again
+    g->tune_k = 0;
+  }
+  if (!g->clanes_best && solids == 0) {
+    g->clanes_best = 4;
+    g->tuned_solids = 0;
+  }
+  g->tune_record = false;
+  if (g->clanes_best) {
+    g->clanes = g->clanes_best;
+    return;
+  }
+  g->clanes = kTuneLanes[g->tune_k];
+  g->tune_record = true;
+  g->tune_used = 0;
+  g->tuned_solids = solids;
+}
+
+// After a tuning frame has been queued: its events are read at the next
+// frame (or sync), once they have completed.
+int fp_group_tune_end(fp_group* g) {
+  if (g->tune_pending < 0) return FP_OK;
+  const int k = g->tune_pending;
+  g->tune_pending = -1;
+  float total = 0.f;
+  for (int i = 0; i < g->tune_used; i++) {
+    float ms = 0.f;
+    FP_GTRY(g, hipEventSynchronize(g->tune_ev[2 * i + 1]));
+    FP_GTRY(g, hipEventElapsedTime(&ms, g->tune_ev[2 * i], g->tune_ev[2 * i + 1]));
+    total += ms;
+  }
+  g->tune_ms[k] = total / std::max(1, g->tune_used);
+  g->tune_k = k + 1;
+  if (g->tune_k == 4) {
+    int best = 0;
+    for (int i = 1; i < 4; i++)
+      if (g->tune_ms[i] < g->tune_ms[best]) best = i;
+    g->clanes_best = kTuneLanes[best];
+    g->tune_k = 0;
+  }
+  return FP_OK;
+}
+
 int fp_group_frame(fp_group* g, double dt, int substeps) {
   const double h = dt / substeps;
-  int rc = fp_group_prepare(g, h);
+  int rc = fp_group_tune_end(g);
+  if (rc) return rc;
+  fp_group_tune_begin(g);
+  rc = fp_group_prepare(g, h);
   if (rc) return rc;
   const bool profiling = g->prof && g->prof->on;
-  const bool graphable = g->use_graph && !g->any_grabs && !profiling;
+  const bool graphable = g->use_graph && !g->any_grabs && !profiling && !g->tune_record;
   if (graphable) {
     if (!g->gexec || g->graph_sub != substeps || g->graph_iters != g->iters) {
       drop_graph(g);
@@ -433,6 +503,10 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
   } else {
     for (int k = 0; k < substeps; k++)
       if ((rc = group_substep(g))) return rc;
+    if (g->tune_record) {
+      g->tune_record = false;
+      g->tune_pending = g->clanes == kTuneLanes[g->tune_k] ? g->tune_k : -1;
+    }
   }
   for (fp_solver* s : g->scenes) s->substeps_done += substeps;
   return FP_OK;

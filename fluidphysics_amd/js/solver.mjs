@@ -234,6 +234,9 @@ export class Batch {
 
   graph(on = true) { addon.batchSetGraph(this._h, on ? 1 : 0); }
 
+  // pin the solid contact pass's lanes per solid (4..32; 0 = tune); results do not depend on it
+  contactLanes(lanes = 0) { addon.batchSetContactLanes(this._h, lanes); }
+
   info() { return addon.batchInfo(this._h); }
 
   // positions of every member after the frames queued so far: one device

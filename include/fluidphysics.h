@@ -241,6 +241,10 @@ typedef struct fp_batch_info {
   int64_t device_bytes;        /* HBM held by the scenes */
   int64_t frames;              /* fp_batch_step calls */
   int64_t graph_launches;      /* frames run by replaying the captured graph */
+  int32_t contact_lanes;       /* lanes per solid the solid contact pass uses: tuned over the
+                                  first frames (4, 8, 16, 32 timed, the fastest kept; 0 while
+                                  tuning).  Results do not depend on it. */
+  int32_t reserved;
 } fp_batch_info;
 
 int fp_batch_create(int32_t device, fp_batch** out);
@@ -255,6 +259,10 @@ int fp_batch_step_async(fp_batch* b, double dt, int32_t substeps);
 int fp_batch_sync(fp_batch* b);
 /* 0: launch the passes each frame instead of replaying a captured graph. */
 int fp_batch_set_graph(fp_batch* b, int32_t enable);
+/* Pin the lanes per solid of the solid contact pass (4, 8, 16 or 32), or 0 to
+ * tune them over the next frames.  A launch-shape choice only: results are
+ * the same for every value. */
+int fp_batch_set_contact_lanes(fp_batch* b, int32_t lanes);
 int fp_batch_get_info(const fp_batch* b, fp_batch_info* out);
 int fp_batch_profile_enable(fp_batch* b, int32_t enable);
 int fp_batch_profile_read(fp_batch* b, fp_pass_stat* out, int32_t max, int32_t* n);

@@ -20,6 +20,7 @@ pytestmark = pytest.mark.gpu
 DT = 1.0 / 60.0
 SUB = 5
 KEYS = ("pos", "prev", "vel", "active")
+TUNE_FRAMES = 4  # the first frames time the contact lane counts 4, 8, 16, 32 (launched directly)
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -97,7 +98,8 @@ def test_batch_bit_identical_to_scenes_alone():
             _same(_snap(s), ref[k][f], "%s frame %d" % (scs[k].name, f))
     inf = b.info()
     assert inf.scenes == len(scs) and inf.frames == frames
-    assert inf.graph_launches == frames
+    assert inf.graph_launches == frames - TUNE_FRAMES
+    assert inf.contact_lanes in (4, 8, 16, 32)
     b.close()
 
 
@@ -123,7 +125,7 @@ def test_graph_replay_equals_direct_launches():
             snaps.append([_snap(s) for s in ss])
             ss[0].fluidCount()
         inf = b.info()
-        assert inf.graph_launches == (frames if use_graph else 0)
+        assert inf.graph_launches == (frames - TUNE_FRAMES if use_graph else 0)
         runs.append(snaps)
         b.close()
     for f in range(frames):
@@ -176,6 +178,38 @@ def test_membership_changes_and_mixed_parameters():
     gone.step(DT, SUB)
     _same(_snap(gone), ref[scs[1].name][5], "%s alone after removal" % scs[1].name)
     b.close()
+
+
+def test_contact_lane_counts_give_identical_results():
+    """The solid contact pass adds its per-cell partials by one fixed tree
+    whatever its lane count: the tuning frames (each a different count) and
+    the tuned frames give the same bits as a batch pinned to each count."""
+    from fluidphysics_amd import Batch, from_scene
+    frames = 6
+    scs = _scenes([16, 17], frames)
+    log("case contact lanes: %s" % [sc.name for sc in scs])
+    runs = []
+    for pin in (None, 4, 8, 16, 32):
+        b = Batch()
+        if pin:
+            b.contact_lanes(pin)
+        ss = [from_scene(sc) for sc in scs]
+        for s in ss:
+            b.add(s)
+        snaps = []
+        for f in range(frames):
+            for s, sc in zip(ss, scs):
+                _cuts(s, sc, f)
+            b.step(DT, SUB)
+            snaps.append([_snap(s) for s in ss])
+        if pin:
+            assert b.info().contact_lanes == pin
+        runs.append(snaps)
+        b.close()
+    for r in runs[1:]:
+        for f in range(frames):
+            for k in range(len(scs)):
+                _same(r[f][k], runs[0][f][k], "%s frame %d" % (scs[k].name, f))
 
 
 def test_batch_rejects_mismatched_constraint_iters():

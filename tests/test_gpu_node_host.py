@@ -140,5 +140,5 @@ def test_node_batch_bit_identical_to_ctypes_batch(tmp_path):
         flat, off = b.snapshot()
         assert list(off) == res["offsets"]
         assert np.array_equal(flat, raw[f]), "frame %d" % f
-    assert res["info"]["graphLaunches"] == frames
+    assert res["info"]["graphLaunches"] == frames - 4  # after the contact-lane tuning frames
     b.close()
