@@ -121,17 +121,17 @@ uint32_t next_pow2(double v, uint32_t lo) {
   return t;
 }
 
-// Cell-grid dimensions.  Direct layout: NZ and NY cover the basin (walls +
-// 1 m) and 4 m of height, the table holds at least 4 z-y planes and the
-// reference table size, at most 2^25 keys (far cells beyond that alias and
-// are rejected by the distance tests).  A scene whose particles would fill
-// under a quarter of that table gets a compact one instead, 4 keys per
-// particle slot (at least 4096), with hashed column offsets: scans of the
-// counting sort cost 12 B per key, so small scenes (batches of them) keep
-// their tables small.
+// Cell-grid dimensions (sort cells: the reference's cellSize / CELL_SUB).
+// Direct layout: NZ and NY cover the basin (walls + 1 m) and 4 m of height,
+// the table holds at least 4 z-y planes and the reference table size, at most
+// 2^25 keys (far cells beyond that alias and are rejected by the distance
+// tests).  A scene whose particles would fill under a quarter of that table
+// gets a compact one instead, 4 keys per particle slot (at least 4096), with
+// hashed column offsets: scans of the counting sort cost 12 B per key, so
+// small scenes (batches of them) keep their tables small.
 void setup_cell_grid(fp_solver* s) {
   const fp_params& P = s->P;
-  const double cs = P.cell_size;
+  const double cs = P.cell_size / CELL_SUB;
   const double half = std::max(std::fabs(P.wall_x), std::fabs(P.wall_z)) + 1.0;
   const uint32_t NZ = next_pow2(std::ceil(2.0 * half / cs), 4);
   const uint32_t NY = next_pow2(std::ceil(4.0 / cs), 4);
@@ -142,6 +142,7 @@ void setup_cell_grid(fp_solver* s) {
   const uint32_t direct = next_pow2((double)M, 4);
   const uint32_t compact = next_pow2(4.0 * s->n, 4096);
   s->cg.inv_cell = 1.0 / cs;
+  s->cg.ref_inv = 1.0 / P.cell_size;
   s->cg.hmask = (uint32_t)(s->T - 1);
   if ((uint64_t)compact * 4 <= direct) {
     s->cg.M = compact;

@@ -92,10 +92,11 @@ __device__ __forceinline__ bool within_h(const FluidParams& fp, const float4& a,
   return dx * dx + dy * dy + dz * dz < fp.H2d;
 }
 
-// Distances from p to the lower and upper faces of its cell c (cell size cs):
-// the floor of the distance to any particle of a neighbouring cell.
+// Distances from p to the lower and upper faces of its sort cell c (cell
+// size cs): with the cell size, they give the floor of the distance to any
+// particle of a neighbouring cell.
 struct FaceDist {
-  float x0, x1, y0, y1, z0, z1;
+  float x0, x1, y0, y1, z0, z1, cs;
 };
 
 __device__ __forceinline__ FaceDist face_dist(float px, float py, float pz, const Cell& c, double cs) {
@@ -106,20 +107,37 @@ __device__ __forceinline__ FaceDist face_dist(float px, float py, float pz, cons
   f.y1 = (float)((c.y + 1.0) * cs - (double)py);
   f.z0 = (float)((double)pz - c.z * cs);
   f.z1 = (float)((c.z + 1.0) * cs - (double)pz);
+  f.cs = (float)cs;
   return f;
 }
 
-// Cells of row r (a, b) = (r / 3 - 1, r % 3 - 1) that can hold a particle
-// closer than sqrt(r2): z offsets lo .. hi-1 of {-1, 0, +1} as key offsets
-// 0..3 from the row's first key; false when the whole row is farther.  NaN
-// positions keep every cell.
+// Gap along one axis from p to the slab of cells at offset o (|o| <= NB_R).
+__device__ __forceinline__ float axis_gap(float f0, float f1, float cs, int o) {
+  return o < 0 ? f0 + (float)(-o - 1) * cs : (o > 0 ? f1 + (float)(o - 1) * cs : 0.f);
+}
+
+// Cells of neighbour row r, (a, b) = (r / NB_RUN - NB_R, r % NB_RUN - NB_R),
+// that can hold a particle closer than sqrt(r2): key offsets lo .. hi-1 of
+// the row's NB_RUN keys; false when the whole row is farther.  r2 carries a
+// margin over the float rounding of the gaps.  NaN positions keep every cell.
 __device__ __forceinline__ bool row_cells(const FaceDist& f, int r, float r2, uint32_t& lo, uint32_t& hi) {
-  const int a = r / 3 - 1, b = r % 3 - 1;
-  const float ex = a < 0 ? f.x0 : f.x1, ey = b < 0 ? f.y0 : f.y1;
-  const float dxy2 = (a != 0 ? ex * ex : 0.f) + (b != 0 ? ey * ey : 0.f);
-  if (!(dxy2 < r2) && dxy2 == dxy2) return false;
-  lo = (dxy2 + f.z0 * f.z0 < r2) ? 0u : 1u;
-  hi = (dxy2 + f.z1 * f.z1 < r2) ? 3u : 2u;
+  const int a = r / NB_RUN - NB_R, b = r % NB_RUN - NB_R;
+  const float ex = axis_gap(f.x0, f.x1, f.cs, a), ey = axis_gap(f.y0, f.y1, f.cs, b);
+  const float dxy2 = ex * ex + ey * ey;
+  if (!(dxy2 == dxy2)) {
+    lo = 0u;
+    hi = (uint32_t)NB_RUN;
+    return true;
+  }
+  if (!(dxy2 < r2)) return false;
+  lo = (uint32_t)NB_R;
+  hi = (uint32_t)NB_R + 1u;
+#pragma unroll
+  for (int o = 1; o <= NB_R; o++) {
+    const float gl = axis_gap(f.z0, f.z1, f.cs, -o), gh = axis_gap(f.z0, f.z1, f.cs, o);
+    if (dxy2 + gl * gl < r2) lo = (uint32_t)(NB_R - o);
+    if (dxy2 + gh * gh < r2) hi = (uint32_t)(NB_R + o + 1);
+  }
   return true;
 }
 
@@ -160,16 +178,22 @@ __device__ __forceinline__ bool bonded(int x, int y, bool held, const int* held_
   return c >= 0 && gld(cc.alive + c) != 0;
 }
 
-// Solid x: L lanes (4..32) share its 27 neighbour cells -- cell u of the
-// row-major 3 x 3 x 3 block goes to lane u % L; cells beyond the contact
-// range are skipped -- and x's bonds are staged once in LDS and consulted
-// only for a solid candidate within contact range.  Each cell's candidates
-// are summed in slot order into its own partial; the 32 partials (cells 27..31
-// empty) are then added by one fixed binary tree, pairs differing in the
-// highest bit of the cell number first: inside a lane for the bits above
-// log2(L), then across lanes.  Every L builds the same tree from the same
-// partials, so the correction is the same bits whatever L the launch uses.
-// Writes pos[x] = grid-time position + correction.
+// Solid x: L lanes (4..32) share its neighbour block in units -- single
+// cells when the block has at most 32 (CELL_SUB 1: 27), else rows -- unit u
+// goes to lane u % L; cells beyond the contact range are skipped -- and x's
+// bonds are staged once in LDS and consulted only for a solid candidate
+// within contact range.  Each unit's candidates are summed in slot order into
+// its own partial; the 32 partials (units past the last empty) are then
+// added by one fixed binary tree, pairs differing in the highest bit of the
+// unit number first: inside a lane for the bits above log2(L), then across lanes.
+// Every L builds the same tree from the same partials, so the correction is
+// the same bits whatever L the launch uses.  Writes pos[x] = grid-time
+// position + correction.
+// contact units: one cell each when the neighbour block has at most 32 cells
+#define CONTACT_CELL_UNITS (NB_ROWS * NB_RUN <= 32)
+#define CONTACT_UNITS (CONTACT_CELL_UNITS ? NB_ROWS * NB_RUN : NB_ROWS)
+static_assert(CONTACT_UNITS <= 32, "the contact reduction tree has 32 leaves");
+
 template <int L>
 __device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb, int* __restrict__ lds_bonds) {
   constexpr int M = 32 / L;  // partials per lane
@@ -209,32 +233,42 @@ __device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb, 
 #pragma unroll
     for (int j = 0; j < M; j++) {
       const int u = lane + L * j;
-      if (u >= 27) continue;
-      const int r = u / 3;
-      const uint32_t z = (uint32_t)(u % 3);
+      if (u >= CONTACT_UNITS) continue;
+      const int r = CONTACT_CELL_UNITS ? u / NB_RUN : u;
       uint32_t lo, hi;
-      if (!row_cells(fd, r, fp.contact_r2, lo, hi) || z < lo || z >= hi) continue;
-      const uint32_t k = (row_key(fp.cg, kx, r / 3 - 1, r % 3 - 1) + z) & fp.cg.kmask;
-      const int q0 = gld(cc.lstart + k), q1 = gld(cc.lstart + k + 1);
+      if (!row_cells(fd, r, fp.contact_r2, lo, hi)) continue;
+      if (CONTACT_CELL_UNITS) {
+        const uint32_t z = (uint32_t)(u % NB_RUN);
+        if (z < lo || z >= hi) continue;
+        lo = z;
+        hi = z + 1;
+      }
+      int q0, q1, w1;
+      run_ranges_n(fp.cg, cc.lstart, (row_key(fp.cg, kx, r / NB_RUN - NB_R, r % NB_RUN - NB_R) + lo) & fp.cg.kmask,
+                   hi - lo, q0, q1, w1);
       float ax = 0.f, ay = 0.f, az = 0.f;
-      for (int q = q0; q < q1; q += 2) {
-        const int qb = min(q + 1, q1 - 1);
-        const int y0 = gld(cc.sid + q), y1 = gld(cc.sid + qb);
-        const float4 p0 = gld(cc.spos + q), p1 = gld(cc.spos + qb);
+      // the row's slots, then (past the table's end) its wrapped part
+      for (int part = 0; part < 2; part++) {
+        const int e0 = part ? 0 : q0, e1 = part ? w1 : q1;
+        for (int q = e0; q < e1; q += 2) {
+          const int qb = min(q + 1, e1 - 1);
+          const int y0 = gld(cc.sid + q), y1 = gld(cc.sid + qb);
+          const float4 p0 = gld(cc.spos + q), p1 = gld(cc.spos + qb);
 #pragma unroll
-        for (int i = 0; i < 2; i++) {
-          const int y = i ? y1 : y0;
-          const float4& py = i ? p1 : p0;
-          if ((i && q + 1 >= q1) || y == x) continue;
-          if (y < fp.ns) {
-            // a bonded pair does not collide; test the bond only when close
-            const float dx = px.x - py.x, dy = px.y - py.y, dz = px.z - py.z;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            if (d2 >= rr2 || d2 < 1e-12f) continue;
-            if (bonded(x, y, held, held_ids, cc)) continue;
-            contact_add(fp, px, fp.rs, py, fp.rs, ax, ay, az);
-          } else {
-            contact_add(fp, px, fp.rs, py, fp.rf, ax, ay, az);
+          for (int i = 0; i < 2; i++) {
+            const int y = i ? y1 : y0;
+            const float4& py = i ? p1 : p0;
+            if ((i && q + 1 >= e1) || y == x) continue;
+            if (y < fp.ns) {
+              // a bonded pair does not collide; test the bond only when close
+              const float dx = px.x - py.x, dy = px.y - py.y, dz = px.z - py.z;
+              const float d2 = dx * dx + dy * dy + dz * dz;
+              if (d2 >= rr2 || d2 < 1e-12f) continue;
+              if (bonded(x, y, held, held_ids, cc)) continue;
+              contact_add(fp, px, fp.rs, py, fp.rs, ax, ay, az);
+            } else {
+              contact_add(fp, px, fp.rs, py, fp.rf, ax, ay, az);
+            }
           }
         }
       }
@@ -290,7 +324,7 @@ __device__ __forceinline__ void contacts_fluid_body(const SceneArgs& A, int lb) 
       gst(tkey + e, (uint32_t)NO_KEY);
     } else {
       const float4 px = gld(cc.spos + e);
-      const Cell c0 = cell_of(px.x, px.y, px.z, fp.cg.inv_cell);
+      const Cell c0 = cell_of(px.x, px.y, px.z, fp.cg.inv_cell);  // sort cell
       float ax = 0.f, ay = 0.f, az = 0.f;
       if (fp.ns > 0) {
         const float g = fp.rs + fp.rf;
@@ -301,21 +335,22 @@ __device__ __forceinline__ void contacts_fluid_body(const SceneArgs& A, int lb) 
           const uint32_t k0 = cell_key(fp.cg, c0);
           const FaceDist fd = face_dist(px.x, px.y, px.z, c0, fp.cs);
 #pragma unroll 1
-          for (int r = 0; r < 9; r++) {
+          for (int r = 0; r < NB_ROWS; r++) {
             uint32_t lo, hi;
             if (!row_cells(fd, r, fp.contact_r2, lo, hi)) continue;
-            const uint32_t kr = row_key(fp.cg, k0, r / 3 - 1, r % 3 - 1);
-            // the row's cell bounds and solid counts, all loads in flight together
-            int a[3], b[3], m[3];
+            const uint32_t kr = row_key(fp.cg, k0, r / NB_RUN - NB_R, r % NB_RUN - NB_R);
+            // the row's cell bounds and solid counts, all loads in flight
+            // together (lnsol holds a value only for a non-empty cell)
+            int a[NB_RUN], b[NB_RUN], m[NB_RUN];
 #pragma unroll
-            for (uint32_t j = 0; j < 3; j++) {
+            for (uint32_t j = 0; j < NB_RUN; j++) {
               const uint32_t k = (kr + j) & fp.cg.kmask;
               a[j] = gld(cc.lstart + k);
               b[j] = gld(cc.lstart + k + 1);
               m[j] = gld(cc.lnsol + k);
             }
 #pragma unroll
-            for (uint32_t j = 0; j < 3; j++) {
+            for (uint32_t j = 0; j < NB_RUN; j++) {
               if (j < lo || j >= hi || a[j] == b[j]) continue;
               for (int q = a[j]; q < a[j] + m[j]; q++) contact_add(fp, px, fp.rf, gld(cc.spos + q), fp.rs, ax, ay, az);
             }
@@ -323,11 +358,12 @@ __device__ __forceinline__ void contacts_fluid_body(const SceneArgs& A, int lb) 
         }
       }
       const float ox = px.x + ax, oy = px.y + ay, oz = px.z + az;
-      const Cell c1 = cell_of(ox, oy, oz, fp.cg.inv_cell);
+      // the reference's cells before and after the contacts decide the flags
+      const Cell r0 = cell_of(px.x, px.y, px.z, fp.cg.ref_inv), r1 = cell_of(ox, oy, oz, fp.cg.ref_inv);
       uint32_t fl = 0;
-      if (c1.x != c0.x || c1.y != c0.y || c1.z != c0.z) fl |= PF_MOVED;
-      if (probe_any_dup(c1, fp.cg.hmask)) fl |= PF_DUP;
-      k1 = cell_key(fp.cg, c1);
+      if (r1.x != r0.x || r1.y != r0.y || r1.z != r0.z) fl |= PF_MOVED;
+      if (probe_any_dup(r1, fp.cg.hmask)) fl |= PF_DUP;
+      k1 = cell_key(fp.cg, cell_of(ox, oy, oz, fp.cg.inv_cell));
       out = make_float4(ox, oy, oz, __uint_as_float((uint32_t)x | (fl << PF_SHIFT)));
       has = true;
     }
@@ -397,8 +433,8 @@ __device__ __forceinline__ int slow_weight(const FluidParams& fp, const PairSour
   // y > x: x's probes against y's grid bucket; y < x: y's probes against x's
   const bool special = (y > me.x()) ? (me.dup() | ((fy & PF_MOVED) != 0u)) : (me.moved() | ((fy & PF_DUP) != 0u));
   if (!special) return 1;
-  return y > me.x() ? mult_pos(me.p.x, me.p.y, me.p.z, fp.cg.inv_cell, fp.cg.hmask, src.sphash[q])
-                    : mult_pos(py.x, py.y, py.z, fp.cg.inv_cell, fp.cg.hmask, src.sphash[me.e]);
+  return y > me.x() ? mult_pos(me.p.x, me.p.y, me.p.z, fp.cg.ref_inv, fp.cg.hmask, src.sphash[q])
+                    : mult_pos(py.x, py.y, py.z, fp.cg.ref_inv, fp.cg.hmask, src.sphash[me.e]);
 }
 
 // Weight of the candidate in slot q (0: not a pair of x).  The common case
@@ -425,17 +461,14 @@ struct RowRange {
   int r0, r1, w1;  // [r0, r1) and, past the table's end, [0, w1)
 };
 
-// Slot range(s) of row r of x (empty when culled).
+// Slot range(s) of neighbour row r of x (empty when culled).
 __device__ __forceinline__ RowRange row_range(const FluidParams& fp, const int* __restrict__ lstart, uint32_t kx, int r,
-                                              float fx0, float fx1, float fy0, float fy1, float fz0, float fz1) {
+                                              const FaceDist& f) {
   RowRange rr{0, 0, 0};
-  const int a = r / 3 - 1, b = r % 3 - 1;
-  const float ex = a < 0 ? fx0 : fx1, ey = b < 0 ? fy0 : fy1;
-  const float dxy2 = (a != 0 ? ex * ex : 0.f) + (b != 0 ? ey * ey : 0.f);
-  if (!(dxy2 < fp.cull_r2) && dxy2 == dxy2) return rr;  // NaN positions visit every cell
-  const uint32_t lo = (dxy2 + fz0 * fz0 < fp.cull_r2) ? 0u : 1u;
-  const uint32_t hi = (dxy2 + fz1 * fz1 < fp.cull_r2) ? 3u : 2u;
-  run_ranges_n(fp.cg, lstart, (row_key(fp.cg, kx, a, b) + lo) & fp.cg.kmask, hi - lo, rr.r0, rr.r1, rr.w1);
+  uint32_t lo, hi;
+  if (!row_cells(f, r, fp.cull_r2, lo, hi)) return rr;
+  run_ranges_n(fp.cg, lstart, (row_key(fp.cg, kx, r / NB_RUN - NB_R, r % NB_RUN - NB_R) + lo) & fp.cg.kmask, hi - lo,
+               rr.r0, rr.r1, rr.w1);
   return rr;
 }
 
@@ -459,8 +492,8 @@ __device__ __forceinline__ RowRange row_range(const FluidParams& fp, const int* 
 // one, so no float32 rounding of it is more faithful than another.
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// Density sums and the pair list of x: every candidate of the 9 culled rows
-// of the fluid order, in slot order.  A z-cell of a row, or a whole row,
+// Density sums and the pair list of x: every candidate of the culled
+// neighbour rows of the fluid order, in slot order.  A z-cell of a row, or a whole row,
 // whose box lies farther than H from x is skipped: it cannot hold a pair
 // (cull_r2 carries a margin over the float and double tests).
 //
@@ -531,15 +564,12 @@ __device__ __forceinline__ void density_gather(const FluidParams& fp, const Pair
   const uint32_t kx = cell_key(fp.cg, c);
   const float4* __restrict__ row = src.spos2;
   const bool me_flagged = me.moved() | me.dup();
-  // distances from x to the lower and upper faces of its cell
-  const float fx0 = (float)((double)me.p.x - c.x * fp.cs), fx1 = (float)((c.x + 1.0) * fp.cs - (double)me.p.x);
-  const float fy0 = (float)((double)me.p.y - c.y * fp.cs), fy1 = (float)((c.y + 1.0) * fp.cs - (double)me.p.y);
-  const float fz0 = (float)((double)me.p.z - c.z * fp.cs), fz1 = (float)((c.z + 1.0) * fp.cs - (double)me.p.z);
-  RowRange nx = row_range(fp, src.lstart, kx, 0, fx0, fx1, fy0, fy1, fz0, fz1);
+  const FaceDist f = face_dist(me.p.x, me.p.y, me.p.z, c, fp.cs);
+  RowRange nx = row_range(fp, src.lstart, kx, 0, f);
 #pragma unroll 1
-  for (int r = 0; r < 9; r++) {
+  for (int r = 0; r < NB_ROWS; r++) {
     const RowRange rr = nx;
-    if (r < 8) nx = row_range(fp, src.lstart, kx, r + 1, fx0, fx1, fy0, fy1, fz0, fz1);
+    if (r < NB_ROWS - 1) nx = row_range(fp, src.lstart, kx, r + 1, f);
     if (rr.r1 > rr.r0) {
       const int last = rr.r1 - 1;
       // this lane's batches of the row: h, h + LANES, ...; the next batch's
@@ -578,12 +608,10 @@ __device__ __forceinline__ void visit_rows_simple(const FluidParams& fp, const P
                                                   F& f) {
   const Cell c = cell_of(me.p.x, me.p.y, me.p.z, fp.cg.inv_cell);
   const uint32_t kx = cell_key(fp.cg, c);
-  const float fx0 = (float)((double)me.p.x - c.x * fp.cs), fx1 = (float)((c.x + 1.0) * fp.cs - (double)me.p.x);
-  const float fy0 = (float)((double)me.p.y - c.y * fp.cs), fy1 = (float)((c.y + 1.0) * fp.cs - (double)me.p.y);
-  const float fz0 = (float)((double)me.p.z - c.z * fp.cs), fz1 = (float)((c.z + 1.0) * fp.cs - (double)me.p.z);
+  const FaceDist fd = face_dist(me.p.x, me.p.y, me.p.z, c, fp.cs);
 #pragma unroll 1
-  for (int r = 0; r < 9; r++) {
-    const RowRange rr = row_range(fp, src.lstart, kx, r, fx0, fx1, fy0, fy1, fz0, fz1);
+  for (int r = 0; r < NB_ROWS; r++) {
+    const RowRange rr = row_range(fp, src.lstart, kx, r, fd);
     // the same batches as density_gather gives this lane
     for (int q0 = rr.r0 + DENSITY_BATCH * me.h; q0 < rr.r1; q0 += DENSITY_BATCH * FLUID_LANES) {
       for (int q = q0; q < min(q0 + DENSITY_BATCH, rr.r1); q++) {
@@ -855,7 +883,7 @@ FluidParams fp_fluid_params(const fp_solver* s) {
   fp.H2d = P.fluid_h * P.fluid_h;
   fp.H2_lo = (float)(fp.H2d * (1.0 - 1e-5));
   fp.H2_hi = (float)(fp.H2d * (1.0 + 1e-5));
-  fp.cs = P.cell_size;
+  fp.cs = P.cell_size / CELL_SUB;
   fp.cull_r2 = (float)(fp.H2d * (1.0 + 1e-4));
   {
     const double rc = P.solid_radius + std::max(P.solid_radius, P.fluid_radius);

@@ -174,9 +174,8 @@ __device__ __forceinline__ void cell_count_body(const SceneArgs& A, int lb) {
   uint32_t k = 0;
   if (act) {
     p = A.pos[i];
-    const Cell c = cell_of(p.x, p.y, p.z, cg.inv_cell);
-    A.phash[i] = cell_hash(c, cg.hmask);
-    k = cell_key(cg, c);
+    A.phash[i] = cell_hash(cell_of(p.x, p.y, p.z, cg.ref_inv), cg.hmask);
+    k = cell_key(cg, cell_of(p.x, p.y, p.z, cg.inv_cell));
     A.lkey[i] = k;
   } else if (in) {
     A.phash[i] = 0xffffffffu;

@@ -179,7 +179,7 @@ struct FluidParams {
   float rs, rf;
   float inv_H, H2_lo, H2_hi, max_pair, max_total;
   double H2d;
-  double cs;       // cell size
+  double cs;       // sort-cell size (the reference's cellSize / CELL_SUB)
   float cull_r2;   // a cell whose box lies at least this far (squared) holds no pair
   float contact_r2;  // the same for contacts (rs + max(rs, rf), squared, with margin)
   float stiff, near_stiff, rest_density;

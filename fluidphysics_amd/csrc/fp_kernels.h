@@ -131,10 +131,28 @@ __device__ __forceinline__ int probe_multiplicity(const Cell& c, uint32_t mask, 
 
 // ---- the cell grid of the hot path -------------------------------------------
 
+// The hot path sorts particles by cells CELL_SUB times finer than the
+// reference's cellSize along each axis (1: the reference's cells).  The
+// interaction radius is at most one reference cell, so a particle's
+// neighbours lie in the NB_RUN^3 block of sort cells around its own: NB_ROWS
+// rows along z, each a run of NB_RUN consecutive keys.  Half cells (2) test
+// half the candidates (a sphere fills 40% of a 5^3 block of half cells, 20%
+// of a 3^3 block of whole ones) but visit 2.7x the rows and scan 8x the
+// keys; on MI355X whole cells measured faster for every workload here.  The
+// reference's own cell (ref_inv) decides hashing either way: buckets, probe
+// multiplicities, the moved and duplicate-probe flags.
+#ifndef CELL_SUB
+#define CELL_SUB 1
+#endif
+#define NB_R CELL_SUB                 // neighbour radius in sort cells
+#define NB_RUN (2 * NB_R + 1)         // sort cells of one neighbour row
+#define NB_ROWS (NB_RUN * NB_RUN)     // neighbour rows
+
 // Cell keys of the hot path's counting sort.  Coordinates beyond +-2^30 (and
 // NaN) share one sentinel cell, so such particles still meet each other.
 struct CellGrid {
-  double inv_cell;
+  double inv_cell;     // 1 / sort-cell size
+  double ref_inv;      // 1 / the reference's cellSize
   uint32_t hmask;      // reference hash table: T - 1
   uint32_t M, kmask;   // cell table size (power of 2), M - 1
   uint32_t mx, my;     // key multipliers of the x and y cell coordinates
@@ -144,8 +162,8 @@ __device__ __forceinline__ uint32_t cell_coord_u32(double c) {
   return fabs(c) < 1073741824.0 ? (uint32_t)(int32_t)c : 0x40000000u;
 }
 
-// key = (cx * mx + cy * my + cz) mod M (32-bit wrap): the three z-neighbours
-// of a cell are consecutive keys.  With mx = NY*NZ, my = NZ and M covering
+// key = (cx * mx + cy * my + cz) mod M (32-bit wrap): the z-neighbours of a
+// cell are consecutive keys.  With mx = NY*NZ, my = NZ and M covering
 // the walls every cell of the basin has its own key; with odd multipliers
 // and a smaller M (small scenes) columns of cells land at scattered offsets
 // and the few that overlap are told apart by the distance tests.
@@ -153,9 +171,9 @@ __device__ __forceinline__ uint32_t cell_key(const CellGrid& g, const Cell& c) {
   return (cell_coord_u32(c.x) * g.mx + cell_coord_u32(c.y) * g.my + cell_coord_u32(c.z)) & g.kmask;
 }
 
-// First key of the z-run of neighbour row (a, b) in {-1,0,1}^2 of key k.
+// First key of the z-run of neighbour row (a, b) in [-NB_R, NB_R]^2 of key k.
 __device__ __forceinline__ uint32_t row_key(const CellGrid& g, uint32_t k, int a, int b) {
-  return (k + (uint32_t)a * g.mx + (uint32_t)b * g.my - 1u) & g.kmask;
+  return (k + (uint32_t)a * g.mx + (uint32_t)b * g.my - (uint32_t)NB_R) & g.kmask;
 }
 
 // ---- global-memory access -------------------------------------------------------

@@ -48,9 +48,8 @@ __device__ __forceinline__ void integrate_body(const SceneArgs& A, int lb) {
     A.pos[i] = p;
     A.vel[i] = v;
     if (COUNT) {
-      const Cell c = cell_of(p.x, p.y, p.z, A.fp.cg.inv_cell);
-      A.phash[i] = cell_hash(c, A.fp.cg.hmask);
-      k = cell_key(A.fp.cg, c);
+      A.phash[i] = cell_hash(cell_of(p.x, p.y, p.z, A.fp.cg.ref_inv), A.fp.cg.hmask);
+      k = cell_key(A.fp.cg, cell_of(p.x, p.y, p.z, A.fp.cg.inv_cell));
       A.lkey[i] = k;
     }
   } else if (in) {
