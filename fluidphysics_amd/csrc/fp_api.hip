@@ -90,7 +90,7 @@ void scene_buffers(fp_solver* s, F& take) {
   take(s->nbr, (size_t)(s->pair_slots + FLUID_LANES) * N), take(s->ncount, N * FLUID_LANES);
   take(s->pslot, N), take(s->count, T), take(s->cell_start, T + 1), take(s->entries_tmp, N);
   take(s->cell_entries, N), take(s->block_sums, (size_t)s->scan_blocks);
-  take(s->cons_ab, C), take(s->rest[0], C), take(s->rest[1], C), take(s->alive[0], C), take(s->alive[1], C);
+  take(s->cons_ab, C), take(s->cent, C), take(s->erest, 2 * C), take(s->ealive, 2 * C);
   take(s->adj_off, ns + 1), take(s->adj, 2 * C), take(s->spos_c[0], std::max<size_t>(ns, 1));
   take(s->spos_c[1], std::max<size_t>(ns, 1)), take(s->dflags, (size_t)DF_WORDS), take(s->dpick_t, 1);
   take(s->dpick_i, 1);
@@ -178,6 +178,43 @@ void scene_tables(fp_solver* s) {
   s->pair_slots = s->P.pair_slots > 0 ? s->P.pair_slots : 128;
 }
 
+// Per-constraint rest / alive <-> the two CSR-entry copies.
+int constraints_to_host(fp_solver* s, float* rest, uint8_t* alive) {
+  const int C = s->C;
+  if (!C) return FP_OK;
+  std::vector<float> er;
+  std::vector<uint8_t> ea;
+  if (rest) {
+    er.resize(2 * (size_t)C);
+    FP_TRY(s, d2h(s, er.data(), s->erest, 2 * (size_t)C * sizeof(float)));
+  }
+  if (alive) {
+    ea.resize(2 * (size_t)C);
+    FP_TRY(s, d2h(s, ea.data(), s->ealive, 2 * (size_t)C));
+  }
+  for (int c = 0; c < C; c++) {
+    if (rest) rest[c] = er[s->h_cent[c].x];
+    if (alive) alive[c] = ea[s->h_cent[c].x];
+  }
+  return FP_OK;
+}
+
+int constraints_to_device(fp_solver* s, const float* rest, const uint8_t* alive) {
+  const int C = s->C;
+  if (!C) return FP_OK;
+  if (rest) {
+    std::vector<float> er(2 * (size_t)C);
+    for (int c = 0; c < C; c++) er[s->h_cent[c].x] = er[s->h_cent[c].y] = rest[c];
+    FP_TRY(s, h2d(s, s->erest, er.data(), 2 * (size_t)C * sizeof(float)));
+  }
+  if (alive) {
+    std::vector<uint8_t> ea(2 * (size_t)C);
+    for (int c = 0; c < C; c++) ea[s->h_cent[c].x] = ea[s->h_cent[c].y] = alive[c] ? 1 : 0;
+    FP_TRY(s, h2d(s, s->ealive, ea.data(), 2 * (size_t)C));
+  }
+  return FP_OK;
+}
+
 int set_device(fp_solver* s) {
   FP_TRY(s, hipSetDevice(s->device));
   return FP_OK;
@@ -222,7 +259,8 @@ int settle(fp_solver* s) {
   s->steps_pending = false;
   if (s->groups_dirty) {
     std::vector<uint8_t> al(s->C);
-    if (s->C) FP_TRY(s, d2h(s, al.data(), s->alive[0], s->C));
+    int rc = constraints_to_host(s, nullptr, al.data());
+    if (rc) return rc;
     recompute_groups_host(s, al);
     s->groups_dirty = false;
   }
@@ -455,24 +493,30 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
   const int C = (int)hrest.size();
   s->C = C;
   s->h_snap_rest = hrest;
-  // static CSR adjacency: entries (other endpoint, id, bits of 1/max(deg_a, deg_b))
+  // static CSR adjacency: entries (other endpoint, bits of 1/max(deg_a,
+  // deg_b)), each with its own copy of the constraint's rest and alive flag
   std::vector<int> hoff(ns + 1, 0);
   for (int c = 0; c < C; c++) {
     hoff[s->h_cons_a[c] + 1]++;
     hoff[s->h_cons_b[c] + 1]++;
   }
   for (int i = 0; i < ns; i++) hoff[i + 1] += hoff[i];
-  std::vector<int4> hadj(std::max(1, 2 * C));
+  std::vector<int2> hadj(std::max(1, 2 * C));
+  std::vector<float> herest(std::max(1, 2 * C));
   std::vector<int> fill(ns, 0);
   std::vector<int2> hab(std::max(1, C));
+  s->h_cent.assign(C, make_int2(0, 0));
   auto deg = [&](int i) { return hoff[i + 1] - hoff[i]; };
   for (int c = 0; c < C; c++) {
     const int a = s->h_cons_a[c], b = s->h_cons_b[c];
     float sc = 1.f / (float)std::max(deg(a), deg(b));
     int scb;
     std::memcpy(&scb, &sc, 4);
-    hadj[hoff[a] + fill[a]++] = make_int4(b, c, scb, 0);
-    hadj[hoff[b] + fill[b]++] = make_int4(a, c, scb, 0);
+    const int ka = hoff[a] + fill[a]++, kb = hoff[b] + fill[b]++;
+    hadj[ka] = make_int2(b, scb);
+    hadj[kb] = make_int2(a, scb);
+    herest[ka] = herest[kb] = hrest[c];
+    s->h_cent[c] = make_int2(ka, kb);
     hab[c] = make_int2(a, b);
   }
   s->h_group.assign(ns, 0);
@@ -498,14 +542,15 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
     const int box[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
     FP_TRY(s, h2d(s, s->sbox, box, sizeof(box)));
   }
-  std::vector<uint8_t> ones(std::max(C, 1), 1);
+  std::vector<uint8_t> ones(std::max(2 * C, 1), 1);
   FP_TRY(s, h2d(s, s->pos, hpos.data(), N * sizeof(float4)));
   FP_TRY(s, h2d(s, s->active, hact.data(), N));
   if (C) {
     FP_TRY(s, h2d(s, s->cons_ab, hab.data(), C * sizeof(int2)));
-    FP_TRY(s, h2d(s, s->rest[0], hrest.data(), C * sizeof(float)));
-    FP_TRY(s, h2d(s, s->alive[0], ones.data(), C));
-    FP_TRY(s, h2d(s, s->adj, hadj.data(), 2 * C * sizeof(int4)));
+    FP_TRY(s, h2d(s, s->cent, s->h_cent.data(), C * sizeof(int2)));
+    FP_TRY(s, h2d(s, s->erest, herest.data(), 2 * C * sizeof(float)));
+    FP_TRY(s, h2d(s, s->ealive, ones.data(), 2 * C));
+    FP_TRY(s, h2d(s, s->adj, hadj.data(), 2 * C * sizeof(int2)));
   }
   FP_TRY(s, h2d(s, s->adj_off, hoff.data(), (ns + 1) * sizeof(int)));
   // prev stays zero until the first integrate (solver.js:43)
@@ -574,8 +619,7 @@ int fp_reset_solid(fp_solver* s) {
   FP_TRY(s, h2d(s, s->vel, v.data(), ns * sizeof(float4)));
   if (s->C) {
     std::vector<uint8_t> ones(s->C, 1);
-    FP_TRY(s, h2d(s, s->rest[0], s->h_snap_rest.data(), s->C * sizeof(float)));
-    FP_TRY(s, h2d(s, s->alive[0], ones.data(), s->C));
+    if ((rc = constraints_to_device(s, s->h_snap_rest.data(), ones.data()))) return rc;
   }
   s->h_group.assign(ns, 0);
   s->groups_dirty = false;
@@ -795,9 +839,7 @@ int fp_read_constraints(fp_solver* s, int32_t* a, int32_t* b, float* rest, uint8
   if ((rc = settle(s))) return rc;
   if (a) std::memcpy(a, s->h_cons_a.data(), s->C * sizeof(int32_t));
   if (b) std::memcpy(b, s->h_cons_b.data(), s->C * sizeof(int32_t));
-  if (rest && s->C) FP_TRY(s, d2h(s, rest, s->rest[0], s->C * sizeof(float)));
-  if (alive && s->C) FP_TRY(s, d2h(s, alive, s->alive[0], s->C));
-  return FP_OK;
+  return constraints_to_host(s, rest, alive);
 }
 
 int fp_write_constraints(fp_solver* s, const float* rest, const uint8_t* alive) {
@@ -806,9 +848,7 @@ int fp_write_constraints(fp_solver* s, const float* rest, const uint8_t* alive) 
   int rc = set_device(s);
   if (rc) return rc;
   if ((rc = settle(s))) return rc;
-  if (rest && s->C) FP_TRY(s, h2d(s, s->rest[0], rest, s->C * sizeof(float)));
-  if (alive && s->C) FP_TRY(s, h2d(s, s->alive[0], alive, s->C));
-  return FP_OK;
+  return constraints_to_device(s, rest, alive);
 }
 
 int fp_build_grid(fp_solver* s) {

@@ -170,12 +170,9 @@ __device__ __forceinline__ bool bonded(int x, int y, bool held, const int* held_
     return hit;
   }
   const int e = gld(cc.adj_off + x + 1);
-  int c = -1;
-  for (int k = gld(cc.adj_off + x); k < e; k++) {
-    const int4 a = gld(cc.adj + k);
-    c = a.x == y ? a.y : c;
-  }
-  return c >= 0 && gld(cc.alive + c) != 0;
+  bool hit = false;
+  for (int k = gld(cc.adj_off + x); k < e; k++) hit |= gld(cc.adj + k).x == y && gld(cc.ealive + k) != 0;
+  return hit;
 }
 
 // Solid x: L lanes (4..32) share its neighbour block in units -- single
@@ -214,8 +211,7 @@ __device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb, 
   for (int j = lane; j < MAXB; j += L) {
     int v = -1;
     if (valid && held && k0 + j < k1) {
-      const int4 a = gld(cc.adj + k0 + j);
-      if (gld(cc.alive + a.y)) v = a.x;
+      if (gld(cc.ealive + k0 + j)) v = gld(cc.adj + k0 + j).x;
     }
     held_ids[j] = v;
   }
@@ -913,8 +909,6 @@ int fp_scene_blocks(const fp_solver* s, int kind, int contact_lanes) {
       return (int)(((int64_t)s->n * FLUID_LANES + FLUID_BLOCK - 1) / FLUID_BLOCK);
     case W_CONS:
       return (int)(((int64_t)s->ns * 32 + BLOCK - 1) / BLOCK);
-    case W_CONSTR:
-      return fp_blocks(s->C);
   }
   return 0;
 }

@@ -24,9 +24,12 @@
 //     pairs the density pass found, replayed by displacement and XSPH
 //   count/cell_start/entries_tmp/cell_entries: the reference's hash grid,
 //     rebuilt from phash only when read
-//   cons_ab int2[C]; rest[2] f32[C]; alive[2] u8[C] (ping-pong per iteration);
-//   adj_off i32[Ns+1]; adj int4[2C] = (other endpoint, constraint id, bits
-//     of 1/max(deg_a, deg_b), 0); spos_c[2] float4[Ns] between iterations
+//   cons_ab int2[C] endpoints; cent int2[C] the constraint's two CSR entries;
+//   adj_off i32[Ns+1]; adj int2[2C] = (other endpoint, bits of
+//     1/max(deg_a, deg_b)); erest f32[2C], ealive u8[2C]: each CSR entry's
+//     copy of its constraint's rest length and alive flag (the two copies are
+//     updated identically, in place, each by its own endpoint's lane);
+//     spos_c[2] float4[Ns] between iterations
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -137,10 +140,11 @@ struct fp_solver {
   int* block_sums = nullptr;
   // constraints
   int2* cons_ab = nullptr;
-  float* rest[2] = {nullptr, nullptr};     // [0] is the state between substeps
-  uint8_t* alive[2] = {nullptr, nullptr};
+  int2* cent = nullptr;       // CSR entries (of a, of b) of each constraint
+  float* erest = nullptr;     // per CSR entry
+  uint8_t* ealive = nullptr;  // per CSR entry
   int* adj_off = nullptr;
-  int4* adj = nullptr;
+  int2* adj = nullptr;
   float4* spos_c[2] = {nullptr, nullptr};
   int* dflags = nullptr;
   double* dpick_t = nullptr;
@@ -153,6 +157,7 @@ struct fp_solver {
   float4 *emit_p = nullptr, *emit_v = nullptr;
   int emit_cap = 0;
   std::vector<int> h_cons_a, h_cons_b;
+  std::vector<int2> h_cent;  // CSR entries of each constraint (host copy of cent)
   std::vector<float> h_snap_pos, h_snap_rest;
   std::vector<int32_t> h_group;
   bool groups_dirty = false;
@@ -220,8 +225,8 @@ struct ContactCtx {
   const int* __restrict__ lstart;   // grid-order cell starts
   const int* __restrict__ lnsol;    // solids of each non-empty cell (they come first)
   const int* __restrict__ adj_off;
-  const int4* __restrict__ adj;
-  const uint8_t* __restrict__ alive;
+  const int2* __restrict__ adj;
+  const uint8_t* __restrict__ ealive;  // per CSR entry
   const int* __restrict__ sbox;  // solids' bounding box at grid time (ordered-int bits)
 };
 
@@ -256,10 +261,11 @@ struct SceneArgs {
   int* sbox;
   int* dflags;
   int* tear;  // host-mapped
-  float* rest[2];
-  uint8_t* alive[2];
+  float* erest;
+  uint8_t* ealive;
+  int2* cent;
   int* adj_off;
-  int4* adj;
+  int2* adj;
   float4* spos_c[2];
 };
 
@@ -270,7 +276,6 @@ enum WorkKind {
   W_SOLID,     // contact lanes (4..32, chosen per group) per solid
   W_FLUID,     // FLUID_LANES threads per particle slot (FLUID_BLOCK per block)
   W_CONS,      // 32 threads per solid
-  W_CONSTR,    // one thread per constraint
   W_NUM
 };
 

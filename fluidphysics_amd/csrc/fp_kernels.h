@@ -188,6 +188,7 @@ __device__ __forceinline__ uint32_t row_key(const CellGrid& g, uint32_t k, int a
 typedef float fp_f4v __attribute__((ext_vector_type(4)));
 typedef float fp_f2v __attribute__((ext_vector_type(2)));
 typedef int fp_i4v __attribute__((ext_vector_type(4)));
+typedef int fp_i2v __attribute__((ext_vector_type(2)));
 typedef float fp_f3v __attribute__((ext_vector_type(3)));
 
 template <class T>
@@ -201,6 +202,10 @@ __device__ __forceinline__ float4 gld(const float4* p) {
 __device__ __forceinline__ float2 gld(const float2* p) {
   const fp_f2v v = *(const FP_GLOBAL fp_f2v*)p;
   return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ int2 gld(const int2* p) {
+  const fp_i2v v = *(const FP_GLOBAL fp_i2v*)p;
+  return make_int2(v.x, v.y);
 }
 __device__ __forceinline__ int4 gld(const int4* p) {
   const fp_i4v v = *(const FP_GLOBAL fp_i4v*)p;

@@ -9,8 +9,8 @@
 //              endpoints, so the pair's corrections stay equal and opposite.
 //              Tearing (d > rest * breakStrain) and plasticity (|C/rest| >
 //              threshold -> rest += C * rate) are decided from the iteration's
-//              snapshot, so both endpoints agree; the first endpoint (consA)
-//              writes the next rest / alive.
+//              snapshot, so both endpoints agree; each endpoint updates its
+//              own CSR entry's copy of the rest length and alive flag.
 // cut          the knife wedge test of solver.js:458-486 in IEEE double with
 //              the reference's operation order (the library is built with
 //              -ffp-contract=off), one thread per constraint.
@@ -22,65 +22,50 @@
 #define CLANES 32
 
 // One Jacobi iteration `it` of one scene: positions from pos (it = 0) or
-// spos_c[(it - 1) & 1] into spos_c[it & 1]; rest / alive from buffer it & 1
-// into buffer (it + 1) & 1 (every substep starts from buffer 0).
+// spos_c[(it - 1) & 1] into spos_c[it & 1].  Each CSR entry holds its own
+// copy of the constraint's rest length and alive flag; both endpoints
+// compute the same update from the same snapshot (d is the same bits either
+// way round), so each lane updates its own entry in place, and only when
+// the value changes (a tear, a plastic step).
 __device__ __forceinline__ void constraints_body(const SceneArgs& A, int lb, int it) {
   const ConsParams& cp = A.cp;
   const float4* __restrict__ sin = it == 0 ? A.pos : A.spos_c[(it - 1) & 1];
   float4* __restrict__ sout = A.spos_c[it & 1];
-  const float* __restrict__ rest_in = A.rest[it & 1];
-  const uint8_t* __restrict__ alive_in = A.alive[it & 1];
-  float* __restrict__ rest_out = A.rest[(it + 1) & 1];
-  uint8_t* __restrict__ alive_out = A.alive[(it + 1) & 1];
-  const int* __restrict__ adj_off = A.adj_off;
-  const int4* __restrict__ adj = A.adj;
+  float* __restrict__ erest = A.erest;
+  uint8_t* __restrict__ ealive = A.ealive;
   const int lane = threadIdx.x & (CLANES - 1);
   const int a = (lb * BLOCK + (int)threadIdx.x) / CLANES;
   const bool valid = a < cp.ns;
   float4 pa = make_float4(0.f, 0.f, 0.f, 0.f);
   int k0 = 0, k1 = 0;
   if (valid) {
-    pa = sin[a];
-    k0 = adj_off[a];
-    k1 = adj_off[a + 1];
+    pa = gld(sin + a);
+    k0 = gld(A.adj_off + a);
+    k1 = gld(A.adj_off + a + 1);
   }
   float cx = 0.f, cy = 0.f, cz = 0.f;
   bool tore = false;
   for (int k = k0 + lane; k < k1; k += CLANES) {
-    const int4 e = adj[k];
-    const int b = e.x, c = e.y;
-    const bool owner = a < b;  // the lattice builder always has consA < consB
-    const float rest = rest_in[c];
-    if (!alive_in[c]) {
-      if (owner) {
-        rest_out[c] = rest;
-        alive_out[c] = 0;
-      }
-      continue;
-    }
-    const float4 pb = sin[b];
+    if (!gld(ealive + k)) continue;
+    const int2 e = gld(A.adj + k);
+    const float rest = gld(erest + k);
+    const float4 pb = gld(sin + e.x);
     const float w = pa.w + pb.w;
     const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
     const float d = sqrtf(dx * dx + dy * dy + dz * dz);
-    float nrest = rest;
-    uint8_t nal = 1;
     if (w != 0.f && d >= 1e-9f) {
       if (d > rest * cp.break_strain) {
-        nal = 0;
+        gst(ealive + k, (uint8_t)0);
         tore = true;
       } else {
         const float C = d - rest;
         const float dl = -C / (w + cp.alpha);
-        const float s = dl / d * pa.w * (cp.omega * __int_as_float(e.z));
+        const float s = dl / d * pa.w * (cp.omega * __int_as_float(e.y));
         cx += dx * s;
         cy += dy * s;
         cz += dz * s;
-        if (fabsf(C / rest) > cp.plastic_thr) nrest = rest + C * cp.plastic_rate;
+        if (fabsf(C / rest) > cp.plastic_thr) gst(erest + k, rest + C * cp.plastic_rate);
       }
-    }
-    if (owner) {
-      rest_out[c] = nrest;
-      alive_out[c] = nal;
     }
   }
 #pragma unroll
@@ -89,7 +74,7 @@ __device__ __forceinline__ void constraints_body(const SceneArgs& A, int lb, int
     cy += __shfl_xor(cy, off, CLANES);
     cz += __shfl_xor(cz, off, CLANES);
   }
-  if (valid && lane == 0) sout[a] = make_float4(pa.x + cx, pa.y + cy, pa.z + cz, pa.w);
+  if (valid && lane == 0) gst(sout + a, make_float4(pa.x + cx, pa.y + cy, pa.z + cz, pa.w));
   if (tore) *A.tear = 1;
 }
 
@@ -98,26 +83,15 @@ __global__ void __launch_bounds__(BLOCK) kg_constraints(const SceneArgs* __restr
   FOR_ITEMS(S, W, nw) constraints_body(A, lb, it);
 }
 
-// After an odd number of iterations the constraints' state is in buffer 1:
-// copy it back so every substep (and a captured frame) starts from buffer 0.
-__device__ __forceinline__ void constraints_back_body(const SceneArgs& A, int lb) {
-  const int c = lb * BLOCK + threadIdx.x;
-  if (c >= A.C) return;
-  A.rest[0][c] = A.rest[1][c];
-  A.alive[0][c] = A.alive[1][c];
-}
-
-__global__ void __launch_bounds__(BLOCK) kg_constraints_back(const SceneArgs* __restrict__ S,
-                                                             const int2* __restrict__ W, int nw) {
-  FOR_ITEMS(S, W, nw) constraints_back_body(A, lb);
-}
-
 __global__ void __launch_bounds__(BLOCK) k_cut(int nc, const float4* __restrict__ pos, const int2* __restrict__ ab,
-                                               uint8_t* __restrict__ alive, double ox, double oy, double oz,
+                                               const int2* __restrict__ cent, uint8_t* __restrict__ ealive,
+                                               double ox, double oy, double oz,
                                                double d1x, double d1y, double d1z, double d2x, double d2y,
                                                double d2z, double nx, double ny, double nz, int* __restrict__ ncut) {
   const int c = blockIdx.x * BLOCK + threadIdx.x;
-  if (c >= nc || !alive[c]) return;
+  if (c >= nc) return;
+  const int2 ke = cent[c];
+  if (!ealive[ke.x]) return;
   const int2 e = ab[c];
   const float4 fa = pos[e.x], fb = pos[e.y];
   const double ax = fa.x, ay = fa.y, az = fa.z, bx = fb.x, by = fb.y, bz = fb.z;
@@ -131,7 +105,8 @@ __global__ void __launch_bounds__(BLOCK) k_cut(int nc, const float4* __restrict_
   const double c1 = (d1y * pz - d1z * py) * nx + (d1z * px - d1x * pz) * ny + (d1x * py - d1y * px) * nz;
   const double c2 = (py * d2z - pz * d2y) * nx + (pz * d2x - px * d2z) * ny + (px * d2y - py * d2x) * nz;
   if (c1 < 0 || c2 < 0) return;
-  alive[c] = 0;
+  ealive[ke.x] = 0;
+  ealive[ke.y] = 0;
   atomicAdd(ncut, 1);
 }
 
@@ -180,7 +155,6 @@ int fp_group_constraints(fp_group* g, int iters) {
   if (iters < 1) return FP_OK;
   PassScope ps(g->prof, g->stream, P_CONSTRAINTS);
   for (int it = 0; it < iters; it++) GROUP_LAUNCH(g, kg_constraints, W_CONS, BLOCK, it);
-  if (iters & 1) GROUP_LAUNCH(g, kg_constraints_back, W_CONSTR, BLOCK);
   FP_GTRY(g, hipGetLastError());
   return FP_OK;
 }
@@ -188,8 +162,8 @@ int fp_group_constraints(fp_group* g, int iters) {
 int fp_launch_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[3], double nx, double ny,
                   double nz) {
   PassScope ps(s, P_CUT);
-  hipLaunchKernelGGL(k_cut, dim3(fp_blocks(s->C)), dim3(BLOCK), 0, s->stream, s->C, s->pos, s->cons_ab,
-                     s->alive[0], o[0], o[1], o[2], d1[0], d1[1], d1[2], d2[0], d2[1], d2[2], nx, ny, nz,
+  hipLaunchKernelGGL(k_cut, dim3(fp_blocks(s->C)), dim3(BLOCK), 0, s->stream, s->C, s->pos, s->cons_ab, s->cent,
+                     s->ealive, o[0], o[1], o[2], d1[0], d1[1], d1[2], d2[0], d2[1], d2[2], nx, ny, nz,
                      s->dflags + DF_CUT);
   FP_TRY(s, hipGetLastError());
   return FP_OK;

@@ -213,7 +213,7 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
   a.pl.ncount = s->ncount;
   a.pl.stride = s->n * FLUID_LANES;
   a.pl.cap = s->pair_slots / FLUID_LANES;
-  a.cc = ContactCtx{s->spos, s->sid, s->lstart, s->lnsol, s->adj_off, s->adj, s->alive[0], s->sbox};
+  a.cc = ContactCtx{s->spos, s->sid, s->lstart, s->lnsol, s->adj_off, s->adj, s->ealive, s->sbox};
   a.bp = bounds_params(s, h);
   a.cp.ns = s->ns;
   a.cp.iters = (s->ns > 0 && s->C > 0) ? std::max(1, (int)P.constraint_iters) : 0;
@@ -257,10 +257,9 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
   a.sbox = s->sbox;
   a.dflags = s->dflags;
   a.tear = s->tear_dev;
-  a.rest[0] = s->rest[0];
-  a.rest[1] = s->rest[1];
-  a.alive[0] = s->alive[0];
-  a.alive[1] = s->alive[1];
+  a.erest = s->erest;
+  a.ealive = s->ealive;
+  a.cent = s->cent;
   a.adj_off = s->adj_off;
   a.adj = s->adj;
   a.spos_c[0] = s->spos_c[0];
