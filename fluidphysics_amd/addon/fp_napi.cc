@@ -143,6 +143,7 @@ const Field kFields[] = {
     FI("constraintIters", constraint_iters),
     FI("tableSize", table_size),
     FI("pairSlots", pair_slots),
+    FI("fluidLanes", fluid_lanes),
     FD("jacobiOmega", jacobi_omega),
     FD("contactScale", contact_scale),
     FD("gravityX", gravity_x),

@@ -87,7 +87,7 @@ void scene_buffers(fp_solver* s, F& take) {
   take(s->ltmp, N), take(s->lkey, N), take(s->lslot, N), take(s->sid, N), take(s->spos, N), take(s->spos2, N);
   take(s->svtmp, N), take(s->sphash, N), take(s->sP, N), take(s->tpos, N), take(s->tkey, N), take(s->tslot, N);
   take(s->fsid, N), take(s->fslot, N), take(s->sphash2, N), take(s->sbox, 6);
-  take(s->nbr, (size_t)(s->pair_slots + FLUID_LANES) * N), take(s->ncount, N * FLUID_LANES);
+  take(s->nbr, (size_t)(s->pair_slots + s->fluid_lanes) * N), take(s->ncount, N * (size_t)s->fluid_lanes);
   take(s->pslot, N), take(s->count, T), take(s->cell_start, T + 1), take(s->entries_tmp, N);
   take(s->cell_entries, N), take(s->block_sums, (size_t)s->scan_blocks);
   take(s->cons_ab, C), take(s->cent, C), take(s->erest, 2 * C), take(s->ealive, 2 * C);
@@ -175,7 +175,9 @@ void scene_dims(fp_solver* s, int n) {
 // Cell grid and pair-list width, once the counts are known.
 void scene_tables(fp_solver* s) {
   setup_cell_grid(s);
-  s->pair_slots = s->P.pair_slots > 0 ? s->P.pair_slots : 128;
+  s->fluid_lanes = s->P.fluid_lanes > 0 ? s->P.fluid_lanes
+                                         : (s->max_fluid >= FLUID_LANES_AUTO ? FLUID_LANES_BIG : FLUID_LANES_SMALL);
+  s->pair_slots = s->P.pair_slots > 0 ? s->P.pair_slots : 64 * s->fluid_lanes;
 }
 
 // Per-constraint rest / alive <-> the two CSR-entry copies.
@@ -276,6 +278,8 @@ int check_params(fp_solver* s, const fp_params& p) {
   if (p.constraint_iters < 0 || p.constraint_iters > 1000)
     return fp_fail(s, FP_EINVAL, "constraint_iters out of range");
   if (p.pair_slots < 0 || p.pair_slots > 1024) return fp_fail(s, FP_EINVAL, "pair_slots out of range [0, 1024]");
+  if (p.fluid_lanes != 0 && p.fluid_lanes != 1 && p.fluid_lanes != 2 && p.fluid_lanes != 4 && p.fluid_lanes != 8)
+    return fp_fail(s, FP_EINVAL, "fluid_lanes must be 0, 1, 2, 4 or 8");
   return FP_OK;
 }
 
@@ -402,8 +406,9 @@ int fp_set_params(fp_solver* s, const fp_params* p) {
   if (!s || !p) return fp_fail(s, FP_EINVAL, "fp_set_params: NULL argument");
   int rc = check_params(s, *p);
   if (rc) return rc;
-  if (s->has_cube && (p->table_size != s->P.table_size || p->pair_slots != s->P.pair_slots))
-    return fp_fail(s, FP_ESTATE, "table_size / pair_slots are fixed once the scene is allocated");
+  if (s->has_cube &&
+      (p->table_size != s->P.table_size || p->pair_slots != s->P.pair_slots || p->fluid_lanes != s->P.fluid_lanes))
+    return fp_fail(s, FP_ESTATE, "table_size / pair_slots / fluid_lanes are fixed once the scene is allocated");
   s->P = *p;
   s->version++;
   return FP_OK;

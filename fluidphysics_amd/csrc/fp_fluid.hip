@@ -378,19 +378,17 @@ __global__ void __launch_bounds__(BLOCK) kg_contacts_fluid(const SceneArgs* __re
   FOR_ITEMS(S, W, nw) contacts_fluid_body(A, lb);
 }
 
-// Lane index of this thread over the fluid order: FLUID_LANES consecutive
-// lanes share one fluid slot (slot = lane / FLUID_LANES).
+// Lane index of this thread over the fluid order: the scene's L fluid lanes
+// (consecutive) share one fluid slot (slot = lane / L).
 __device__ __forceinline__ int fluid_lane(int lb) { return lb * FLUID_BLOCK + (int)threadIdx.x; }
 
-// Sum over the FLUID_LANES lanes of one slot (pairwise, the same in every lane).
-__device__ __forceinline__ float lanes_sum(float v) {
-#pragma unroll
-  for (int off = 1; off < FLUID_LANES; off <<= 1) v += __shfl_xor(v, off, FLUID_LANES);
+// Sum over the L lanes of one slot (pairwise, the same in every lane).
+__device__ __forceinline__ float lanes_sum(float v, int L) {
+  for (int off = 1; off < L; off <<= 1) v += __shfl_xor(v, off, L);
   return v;
 }
-__device__ __forceinline__ int lanes_sum(int v) {
-#pragma unroll
-  for (int off = 1; off < FLUID_LANES; off <<= 1) v += __shfl_xor(v, off, FLUID_LANES);
+__device__ __forceinline__ int lanes_sum(int v, int L) {
+  for (int off = 1; off < L; off <<= 1) v += __shfl_xor(v, off, L);
   return v;
 }
 
@@ -574,7 +572,7 @@ __device__ __forceinline__ void density_gather(const FluidParams& fp, const Pair
       // loads in flight), and the next batch is loaded unconditionally
       // (indices clamp to the row's last slot): loads under a branch make
       // the compiler wait for them as well as for the batch in use.
-      constexpr int step = DENSITY_BATCH * FLUID_LANES;
+      const int step = DENSITY_BATCH << fp.fl_log2;
       int q = rr.r0 + DENSITY_BATCH * me.h;
       float4 b0[DENSITY_BATCH], b1[DENSITY_BATCH];
       if (q <= last) load_batch(row, q, last, b0);
@@ -609,7 +607,7 @@ __device__ __forceinline__ void visit_rows_simple(const FluidParams& fp, const P
   for (int r = 0; r < NB_ROWS; r++) {
     const RowRange rr = row_range(fp, src.lstart, kx, r, fd);
     // the same batches as density_gather gives this lane
-    for (int q0 = rr.r0 + DENSITY_BATCH * me.h; q0 < rr.r1; q0 += DENSITY_BATCH * FLUID_LANES) {
+    for (int q0 = rr.r0 + DENSITY_BATCH * me.h; q0 < rr.r1; q0 += DENSITY_BATCH << fp.fl_log2) {
       for (int q = q0; q < min(q0 + DENSITY_BATCH, rr.r1); q++) {
         float dx, dy, dz, d2;
         const int w = pair_weight(fp, src, me, q, src.spos2[q], dx, dy, dz, d2);
@@ -681,7 +679,7 @@ __device__ __forceinline__ void recorded_pairs(const FluidParams& fp, const Pair
                                                const PairSelf& me, F& f) {
   const int cnt = gld(pl.ncount + me.v);
   // a slot whose lanes did not all fit is gathered again by all its lanes
-  if (__builtin_expect(lanes_sum(cnt > pl.cap ? 1 : 0) == 0, 1)) {
+  if (__builtin_expect(lanes_sum(cnt > pl.cap ? 1 : 0, 1 << fp.fl_log2) == 0, 1)) {
     replay_pairs(src, pl, me, cnt, f);
   } else {
     visit_rows_simple(fp, src, me, f);
@@ -691,8 +689,8 @@ __device__ __forceinline__ void recorded_pairs(const FluidParams& fp, const Pair
 // Who is in the fluid slot of lane v (false past the last slot).
 __device__ __forceinline__ bool pair_self(const FluidParams& fp, const PairSource& src, int v, PairSelf& me) {
   me.v = v;
-  me.h = v % FLUID_LANES;
-  me.e = v / FLUID_LANES;
+  me.h = v & ((1 << fp.fl_log2) - 1);
+  me.e = v >> fp.fl_log2;
   if (me.e >= gld(src.lstart + fp.cg.M)) return false;
   me.p = gld(src.spos2 + me.e);
   return true;
@@ -702,9 +700,10 @@ __device__ __forceinline__ bool pair_self(const FluidParams& fp, const PairSourc
 __device__ __forceinline__ void density_store(const FluidParams& fp, const PairList& pl, const PairSelf& me,
                                               const DensityGather& g, float2* __restrict__ sP,
                                               float2* __restrict__ rho_out, int* __restrict__ dflags) {
-  const bool over = lanes_sum(g.cnt > pl.cap ? 1 : 0) != 0;
+  const int L = 1 << fp.fl_log2;
+  const bool over = lanes_sum(g.cnt > pl.cap ? 1 : 0, L) != 0;
   gst(pl.ncount + me.v, g.cnt > pl.cap ? pl.cap + 1 : g.cnt);
-  const float rho = lanes_sum(g.rho), rhon = lanes_sum(g.rhon);
+  const float rho = lanes_sum(g.rho, L), rhon = lanes_sum(g.rhon, L);
   if (me.h != 0) return;
   if (over) atomicAdd(&dflags[DF_PAIR_OVF], 1);
   gst(rho_out + me.x(), make_float2(rho, rhon));
@@ -794,7 +793,8 @@ __device__ __forceinline__ void displace_body(const SceneArgs& A, int lb) {
   acc.inv_H = fp.inv_H;
   acc.max_pair = fp.max_pair;
   recorded_pairs(fp, src, pl, me, acc);
-  float sx = lanes_sum(acc.sx), sy = lanes_sum(acc.sy), sz = lanes_sum(acc.sz);
+  const int L = 1 << fp.fl_log2;
+  float sx = lanes_sum(acc.sx, L), sy = lanes_sum(acc.sy, L), sz = lanes_sum(acc.sz, L);
   if (me.h != 0) return;
   const float s2 = sx * sx + sy * sy + sz * sz;
   if (s2 > fp.max_total * fp.max_total) {
@@ -846,8 +846,9 @@ __device__ __forceinline__ void xsph_body(const SceneArgs& A, int lb) {
   XsphAcc acc;
   acc.svtmp = svtmp;
   recorded_pairs(fp, src, pl, me, acc);
-  const int m = lanes_sum(acc.m);
-  const float sx = lanes_sum(acc.sx), sy = lanes_sum(acc.sy), sz = lanes_sum(acc.sz);
+  const int L = 1 << fp.fl_log2;
+  const int m = lanes_sum(acc.m, L);
+  const float sx = lanes_sum(acc.sx, L), sy = lanes_sum(acc.sy, L), sz = lanes_sum(acc.sz, L);
   if (me.h != 0) return;
   const float4 v = gld(svtmp + me.e);
   if (m == 0) {
@@ -891,6 +892,7 @@ FluidParams fp_fluid_params(const fp_solver* s) {
   fp.near_stiff = (float)P.near_stiffness;
   fp.rest_density = (float)P.rest_density;
   fp.contact_scale = (float)P.contact_scale;
+  fp.fl_log2 = s->fluid_lanes == 8 ? 3 : s->fluid_lanes == 4 ? 2 : s->fluid_lanes == 2 ? 1 : 0;
 
   fp.visc = (float)P.viscosity;
   return fp;
@@ -906,7 +908,7 @@ int fp_scene_blocks(const fp_solver* s, int kind, int contact_lanes) {
     case W_SOLID:
       return fp_blocks(s->ns * contact_lanes);
     case W_FLUID:
-      return (int)(((int64_t)s->n * FLUID_LANES + FLUID_BLOCK - 1) / FLUID_BLOCK);
+      return (int)(((int64_t)s->n * s->fluid_lanes + FLUID_BLOCK - 1) / FLUID_BLOCK);
     case W_CONS:
       return (int)(((int64_t)s->ns * 32 + BLOCK - 1) / BLOCK);
   }

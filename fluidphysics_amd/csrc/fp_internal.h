@@ -19,7 +19,7 @@
 //   density, displacement, XSPH): fsid, fslot i32[N] (inverse), spos2
 //     float4 (= tpos), sphash2 u32, sP float2 (P, Pnear), svtmp float4
 //     (velocity before XSPH)
-//   nbr u32[(K+L)*N] (L = FLUID_LANES lanes per fluid slot, each with K/L
+//   nbr u32[(K+L)*N] (L = the scene's fluid lanes per slot, each with K/L
 //     entries: entry k of lane v at k*N*L + v; row K/L is a sink), ncount i32[N*L]: the
 //     pairs the density pass found, replayed by displacement and XSPH
 //   count/cell_start/entries_tmp/cell_entries: the reference's hash grid,
@@ -50,9 +50,13 @@
 #define SID_MASK 0x3fffffffu
 #define PF_SHIFT 30
 #define NO_KEY 0xffffffffu
-#ifndef FLUID_LANES
-#define FLUID_LANES 2  // lanes per fluid particle in density, displacement and XSPH
-#endif
+// Lanes per fluid particle in density, displacement and XSPH: a property of
+// the scene (fp_params.fluid_lanes; 0 = FLUID_LANES_BIG for scenes of at
+// least FLUID_LANES_AUTO fluid slots, else FLUID_LANES_SMALL).  It orders the
+// pair list and the lane sums, so it is part of the scene's arithmetic.
+#define FLUID_LANES_SMALL 2
+#define FLUID_LANES_BIG 4
+#define FLUID_LANES_AUTO 131072
 
 // device flag words (dflags)
 #define DF_UNUSED0 0     // (the tear flag lives in host-mapped memory: fp_solver::tear_host)
@@ -171,6 +175,7 @@ struct fp_solver {
   int* tear_host = nullptr;
   int* tear_dev = nullptr;
   void* arena = nullptr;  // one pool block holding every fixed buffer of the scene
+  int fluid_lanes = FLUID_LANES_SMALL;  // lanes per fluid particle (resolved from fp_params)
   fp_group* group = nullptr;         // this scene alone, for fp_step
   fp_batch* batch = nullptr;         // the batch it belongs to, if any
   uint64_t version = 1;              // bumped when its parameters change
@@ -181,6 +186,7 @@ struct fp_solver {
 struct FluidParams {
   CellGrid cg;
   int ns, n, nf;
+  int fl_log2;  // log2 of the lanes per fluid particle
   float rs, rf;
   float inv_H, H2_lo, H2_hi, max_pair, max_total;
   double H2d;
@@ -274,7 +280,7 @@ enum WorkKind {
   W_PARTICLE,  // one thread per particle slot (BLOCK per block)
   W_SCAN,      // one scan tile of the cell table
   W_SOLID,     // contact lanes (4..32, chosen per group) per solid
-  W_FLUID,     // FLUID_LANES threads per particle slot (FLUID_BLOCK per block)
+  W_FLUID,     // fluid lanes (the scene's) per particle slot (FLUID_BLOCK per block)
   W_CONS,      // 32 threads per solid
   W_NUM
 };

@@ -211,8 +211,8 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
   a.src.lstart = s->lstart;
   a.pl.nbr = s->nbr;
   a.pl.ncount = s->ncount;
-  a.pl.stride = s->n * FLUID_LANES;
-  a.pl.cap = s->pair_slots / FLUID_LANES;
+  a.pl.stride = s->n * s->fluid_lanes;
+  a.pl.cap = s->pair_slots / s->fluid_lanes;
   a.cc = ContactCtx{s->spos, s->sid, s->lstart, s->lnsol, s->adj_off, s->adj, s->ealive, s->sbox};
   a.bp = bounds_params(s, h);
   a.cp.ns = s->ns;

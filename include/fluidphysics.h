@@ -70,8 +70,12 @@ typedef struct fp_params {
   int32_t pair_slots;          /* pairs recorded per fluid particle by the density pass for the
                                   displacement and XSPH passes (the reference's pair list,
                                   :284-327); a particle with more is gathered again from the
-                                  grid.  0 = 128 */
-  int32_t reserved0;           /* 0 */
+                                  grid.  0 = 64 per fluid lane */
+  int32_t fluid_lanes;         /* lanes per fluid particle in the density, displacement and
+                                  XSPH passes: 1, 2, 4 or 8; 0 = 4 for a maxFluid of 131072 or
+                                  more, else 2.  It orders the pair list and the lanes' partial
+                                  sums: part of the scene's arithmetic (a scene steps
+                                  identically alone and in a batch) */
   double jacobi_omega;         /* Jacobi relaxation: each constraint's correction is scaled by
                                   jacobi_omega / max(deg_a, deg_b); default 4.0 */
   double contact_scale;        /* scale on each Jacobi contact correction, 1.0 */
