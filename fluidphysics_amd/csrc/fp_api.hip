@@ -244,8 +244,23 @@ void recompute_groups_host(fp_solver* s, const std::vector<uint8_t>& alive) {
   for (int i = 0; i < n; i++) s->h_group[i] = find(i);
 }
 
+// The component relabel the reference runs at the end of a step that tore
+// or followed a cut (solver.js:238-241) is due: run it now, from the alive
+// flags as the step left them.  It is run lazily -- when group is read, or
+// before a cut or a constraint write changes the flags -- which gives the
+// same labels as running it at the end of every such step.
+int materialize_groups(fp_solver* s) {
+  if (!s->groups_pending) return FP_OK;
+  std::vector<uint8_t> al(s->C);
+  int rc = constraints_to_host(s, nullptr, al.data());
+  if (rc) return rc;
+  recompute_groups_host(s, al);
+  s->groups_pending = false;
+  return FP_OK;
+}
+
 // Sync point: wait for the stream, pick up the tear flag of finished steps,
-// relabel the components when a tear or a cut happened (solver.js:238-241).
+// mark the components for relabelling when a tear or a cut happened.
 int settle(fp_solver* s) {
   FP_TRY(s, hipStreamSynchronize(s->stream));
   if (s->prof.on) profile_collect(&s->prof);
@@ -260,10 +275,7 @@ int settle(fp_solver* s) {
   s->cut_dirty_at_step = false;
   s->steps_pending = false;
   if (s->groups_dirty) {
-    std::vector<uint8_t> al(s->C);
-    int rc = constraints_to_host(s, nullptr, al.data());
-    if (rc) return rc;
-    recompute_groups_host(s, al);
+    s->groups_pending = true;
     s->groups_dirty = false;
   }
   return FP_OK;
@@ -628,6 +640,7 @@ int fp_reset_solid(fp_solver* s) {
   }
   s->h_group.assign(ns, 0);
   s->groups_dirty = false;
+  s->groups_pending = false;
   s->cut_dirty = false;
   s->grab_round_len.clear();
   return FP_OK;
@@ -727,6 +740,7 @@ int fp_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[
   int rc = set_device(s);
   if (rc) return rc;
   if ((rc = settle(s))) return rc;
+  if ((rc = materialize_groups(s))) return rc;  // labels from the flags before the cut
   FP_TRY(s, hipMemsetAsync(s->dflags + DF_CUT, 0, sizeof(int), s->stream));
   if ((rc = fp_launch_cut(s, o, d1, d2, nx, ny, nz))) return rc;
   int cnt = 0;
@@ -804,7 +818,10 @@ int fp_read_state(fp_solver* s, float* pos, float* prev, float* vel, float* inv_
     }
   }
   if (active) FP_TRY(s, d2h(s, active, s->active, N));
-  if (group) std::memcpy(group, s->h_group.data(), s->ns * sizeof(int32_t));
+  if (group) {
+    if ((rc = materialize_groups(s))) return rc;
+    std::memcpy(group, s->h_group.data(), s->ns * sizeof(int32_t));
+  }
   return FP_OK;
 }
 
@@ -853,6 +870,7 @@ int fp_write_constraints(fp_solver* s, const float* rest, const uint8_t* alive) 
   int rc = set_device(s);
   if (rc) return rc;
   if ((rc = settle(s))) return rc;
+  if ((rc = materialize_groups(s))) return rc;
   return constraints_to_device(s, rest, alive);
 }
 

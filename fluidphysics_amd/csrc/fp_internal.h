@@ -165,6 +165,7 @@ struct fp_solver {
   std::vector<float> h_snap_pos, h_snap_rest;
   std::vector<int32_t> h_group;
   bool groups_dirty = false;
+  bool groups_pending = false;  // a relabel is due; run when group is read or alive is about to change
   bool steps_pending = false;
   bool cut_dirty = false;
   bool cut_dirty_at_step = false;
