@@ -265,6 +265,32 @@ class Solver:
         self._cache.update(out)
         return out
 
+    # ---- checkpoints and state blobs (src/gpu/sim.js:494-547) ------------
+    def checkpoint(self, slot=0):
+        """Device-resident copy of the scene's state (slots 0..7)."""
+        self._flush()
+        self._check(self._lib.fp_checkpoint_save(self._h, int(slot)))
+
+    def restore(self, slot=0):
+        self._flush()
+        self._check(self._lib.fp_checkpoint_restore(self._h, int(slot)))
+        self._invalidate()
+
+    def save_state(self):
+        """The state as a portable blob (bytes): header + device arrays."""
+        self._flush()
+        n = ctypes.c_int64()
+        self._check(self._lib.fp_state_size(self._h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._check(self._lib.fp_state_export(self._h, buf, n.value))
+        return buf.raw
+
+    def load_state(self, blob):
+        self._flush()
+        b = bytes(blob)
+        self._check(self._lib.fp_state_import(self._h, b, len(b)))
+        self._invalidate()
+
     def read_positions(self):
         """pos alone (fp_read_state with the other outputs omitted), (capacity, 3)."""
         self._flush()

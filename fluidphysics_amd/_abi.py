@@ -165,6 +165,11 @@ SIGNATURES = {
     "fp_read_density": (_I, [_P, _FP, _FP]),
     "fp_profile_enable": (_I, [_P, _I]),
     "fp_profile_read": (_I, [_P, ctypes.POINTER(FpPassStat), _I, _IP]),
+    "fp_checkpoint_save": (_I, [_P, _I]),
+    "fp_checkpoint_restore": (_I, [_P, _I]),
+    "fp_state_size": (_I, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "fp_state_export": (_I, [_P, ctypes.c_void_p, ctypes.c_int64]),
+    "fp_state_import": (_I, [_P, ctypes.c_void_p, ctypes.c_int64]),
     "fp_batch_create": (_I, [_I, _PP]),
     "fp_batch_destroy": (_I, [_P]),
     "fp_batch_last_error": (ctypes.c_char_p, [_P]),

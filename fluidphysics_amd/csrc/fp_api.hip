@@ -112,6 +112,9 @@ void free_all(fp_solver* s) {
   scene_buffers(s, none);
   if (s->arena) s->device_bytes -= (int64_t)fp_pool_put(s->device, s->arena);
   s->arena = nullptr;
+  for (auto& cp : s->checkpoints)
+    if (cp.block) s->device_bytes -= (int64_t)fp_pool_put(s->device, cp.block);
+  s->checkpoints.clear();
   dfree(s, s->grab_idx), dfree(s, s->grab_tgt), dfree(s, s->emit_idx), dfree(s, s->emit_p), dfree(s, s->emit_v);
 }
 
@@ -1195,5 +1198,177 @@ int fp_batch_snapshot(fp_batch* b, const float** pos, const int64_t** offsets, i
   *offsets = b->snap_off_out[slot].data();
   *scenes = ns;
   b->snap_slot ^= 1;
+  return FP_OK;
+}
+
+// ---- checkpoints and state blobs -------------------------------------------------
+//
+// The state that decides a scene's future: positions, previous positions,
+// velocities (w: inverse mass), active flags, the last densities, each CSR
+// entry's rest length and alive flag, the hose ring position, the substep
+// count and the component labels with their pending relabel.  The grid, the
+// sorted copies and the pair list are rebuilt every substep.  (The reference's
+// GPU twin snapshots its buffers the same way, src/gpu/sim.js:494-547.)
+
+namespace {
+
+template <class F>
+void state_arrays(fp_solver* s, F&& f) {
+  const size_t N = (size_t)s->n, E = 2 * (size_t)s->C;
+  f((void*)s->pos, N * sizeof(float4));
+  f((void*)s->prev, N * sizeof(float4));
+  f((void*)s->vel, N * sizeof(float4));
+  f((void*)s->rho, N * sizeof(float2));
+  f((void*)s->active, N);
+  f((void*)s->erest, E * sizeof(float));
+  f((void*)s->ealive, E);
+}
+
+size_t state_array_bytes(fp_solver* s) {
+  size_t total = 0;
+  state_arrays(s, [&](void*, size_t b) { total += (b + 255) & ~(size_t)255; });
+  return total;
+}
+
+const int kMaxCheckpoints = 8;
+
+struct StateHeader {
+  char magic[8];  // "FPSTATE"
+  uint32_t version;
+  int32_t capacity, solid_count, max_fluid, num_constraints, fluid_next;
+  int64_t substeps_done;
+  uint32_t flags;  // 1: relabel pending, 2: a cut awaits the next step
+  uint32_t reserved;
+};
+
+}  // namespace
+
+int fp_checkpoint_save(fp_solver* s, int32_t slot) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_checkpoint_save: NULL solver");
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_checkpoint_save before fp_add_solid_cube");
+  if (slot < 0 || slot >= kMaxCheckpoints) return fp_fail(s, FP_EINVAL, "fp_checkpoint_save: slot out of range [0, 8)");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  if ((int)s->checkpoints.size() <= slot) s->checkpoints.resize(slot + 1);
+  fp_solver::Checkpoint& cp = s->checkpoints[slot];
+  const size_t need = state_array_bytes(s);
+  if (!cp.block) {
+    size_t cb = 0;
+    const hipError_t e = fp_pool_get(s->device, need, &cp.block, &cb);
+    if (e != hipSuccess) return fp_fail(s, FP_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    cp.bytes = need;
+    s->device_bytes += (int64_t)cb;
+  }
+  size_t off = 0;
+  hipError_t e = hipSuccess;
+  state_arrays(s, [&](void* p, size_t b) {
+    if (e == hipSuccess && b) e = hipMemcpyAsync((char*)cp.block + off, p, b, hipMemcpyDeviceToDevice, s->stream);
+    off += (b + 255) & ~(size_t)255;
+  });
+  FP_TRY(s, e);
+  cp.fluid_next = s->fluid_next;
+  cp.substeps_done = s->substeps_done;
+  cp.group = s->h_group;
+  cp.groups_pending = s->groups_pending;
+  cp.cut_dirty = s->cut_dirty;
+  FP_TRY(s, hipStreamSynchronize(s->stream));
+  return FP_OK;
+}
+
+int fp_checkpoint_restore(fp_solver* s, int32_t slot) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_checkpoint_restore: NULL solver");
+  if (slot < 0 || slot >= (int)s->checkpoints.size() || !s->checkpoints[slot].block)
+    return fp_fail(s, FP_EINVAL, "fp_checkpoint_restore: no checkpoint in this slot");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  const fp_solver::Checkpoint& cp = s->checkpoints[slot];
+  size_t off = 0;
+  hipError_t e = hipSuccess;
+  state_arrays(s, [&](void* p, size_t b) {
+    if (e == hipSuccess && b) e = hipMemcpyAsync(p, (char*)cp.block + off, b, hipMemcpyDeviceToDevice, s->stream);
+    off += (b + 255) & ~(size_t)255;
+  });
+  FP_TRY(s, e);
+  FP_TRY(s, hipStreamSynchronize(s->stream));
+  s->fluid_next = cp.fluid_next;
+  s->substeps_done = cp.substeps_done;
+  s->h_group = cp.group;
+  s->groups_pending = cp.groups_pending;
+  s->groups_dirty = false;
+  s->cut_dirty = cp.cut_dirty;
+  return FP_OK;
+}
+
+int fp_state_size(fp_solver* s, int64_t* bytes) {
+  if (!s || !bytes) return fp_fail(s, FP_EINVAL, "fp_state_size: NULL argument");
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_state_size before fp_add_solid_cube");
+  int64_t total = (int64_t)sizeof(StateHeader);
+  state_arrays(s, [&](void*, size_t b) { total += (int64_t)b; });
+  *bytes = total + 4 * (int64_t)s->ns;
+  return FP_OK;
+}
+
+int fp_state_export(fp_solver* s, void* buf, int64_t bytes) {
+  int64_t need = 0;
+  int rc = fp_state_size(s, &need);
+  if (rc) return rc;
+  if (!buf || bytes < need) return fp_fail(s, FP_EINVAL, "fp_state_export: buffer smaller than fp_state_size");
+  if ((rc = set_device(s))) return rc;
+  if ((rc = settle(s))) return rc;
+  StateHeader h;
+  std::memset(&h, 0, sizeof(h));
+  std::memcpy(h.magic, "FPSTATE", 8);
+  h.version = 1;
+  h.capacity = s->n;
+  h.solid_count = s->ns;
+  h.max_fluid = s->max_fluid;
+  h.num_constraints = s->C;
+  h.fluid_next = s->fluid_next;
+  h.substeps_done = s->substeps_done;
+  h.flags = (s->groups_pending ? 1u : 0u) | (s->cut_dirty ? 2u : 0u);
+  char* out = (char*)buf;
+  std::memcpy(out, &h, sizeof(h));
+  size_t off = sizeof(h);
+  hipError_t e = hipSuccess;
+  state_arrays(s, [&](void* p, size_t b) {
+    if (e == hipSuccess && b) e = d2h(s, out + off, p, b);
+    off += b;
+  });
+  FP_TRY(s, e);
+  if (s->ns) std::memcpy(out + off, s->h_group.data(), 4 * (size_t)s->ns);
+  return FP_OK;
+}
+
+int fp_state_import(fp_solver* s, const void* buf, int64_t bytes) {
+  int64_t need = 0;
+  int rc = fp_state_size(s, &need);
+  if (rc) return rc;
+  if (!buf || bytes < (int64_t)sizeof(StateHeader)) return fp_fail(s, FP_EINVAL, "fp_state_import: no state blob");
+  StateHeader h;
+  std::memcpy(&h, buf, sizeof(h));
+  if (std::memcmp(h.magic, "FPSTATE", 8) != 0 || h.version != 1)
+    return fp_fail(s, FP_EINVAL, "fp_state_import: not a state blob of version 1");
+  if (h.capacity != s->n || h.solid_count != s->ns || h.max_fluid != s->max_fluid || h.num_constraints != s->C)
+    return fp_fail(s, FP_EINVAL, "fp_state_import: the blob is of a scene of another shape");
+  if (bytes < need) return fp_fail(s, FP_EINVAL, "fp_state_import: blob shorter than its header says");
+  if ((rc = set_device(s))) return rc;
+  if ((rc = settle(s))) return rc;
+  const char* in = (const char*)buf;
+  size_t off = sizeof(h);
+  hipError_t e = hipSuccess;
+  state_arrays(s, [&](void* p, size_t b) {
+    if (e == hipSuccess && b) e = h2d(s, p, in + off, b);
+    off += b;
+  });
+  FP_TRY(s, e);
+  s->h_group.assign(s->ns, 0);
+  if (s->ns) std::memcpy(s->h_group.data(), in + off, 4 * (size_t)s->ns);
+  s->fluid_next = h.fluid_next;
+  s->substeps_done = h.substeps_done;
+  s->groups_pending = (h.flags & 1u) != 0;
+  s->groups_dirty = false;
+  s->cut_dirty = (h.flags & 2u) != 0;
   return FP_OK;
 }

@@ -109,6 +109,16 @@ struct fp_solver {
   std::string err;
   int max_fluid = 0, ns = 0, n = 0, nf = 0, T = 0, C = 0;
   int fluid_next = 0;
+  // device-resident checkpoints (fp_checkpoint_save / _restore)
+  struct Checkpoint {
+    void* block = nullptr;
+    size_t bytes = 0;
+    int fluid_next = 0;
+    int64_t substeps_done = 0;
+    std::vector<int32_t> group;
+    bool groups_pending = false, cut_dirty = false;
+  };
+  std::vector<Checkpoint> checkpoints;
   bool has_cube = false;
   int64_t substeps_done = 0;
   int64_t device_bytes = 0;

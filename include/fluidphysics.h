@@ -227,6 +227,28 @@ int fp_profile_enable(fp_solver* s, int32_t enable);
 /* Accumulated statistics; *n = number of passes written (<= max). */
 int fp_profile_read(fp_solver* s, fp_pass_stat* out, int32_t max, int32_t* n);
 
+/* ---- checkpoints and state blobs ------------------------------------- */
+
+/* The reference's GPU twin snapshots its buffers for exact repro
+ * (src/gpu/sim.js:494-547 snapshot / restore; src/gpu/main.js:308-390
+ * records the inputs from such a snapshot and replays them).  Here the state
+ * that decides a scene's future is positions, previous positions, velocities
+ * and inverse masses, active flags, the last densities, each constraint's
+ * rest length and alive flag, the hose ring position, the substep count and
+ * the component labels.  Stepping on from a restored state gives the same
+ * bits as stepping on from the saved one. */
+
+/* Device-resident copies in one of 8 slots the solver owns (no host
+ * transfer; a slot keeps its device memory until the solver is destroyed). */
+int fp_checkpoint_save(fp_solver* s, int32_t slot);
+int fp_checkpoint_restore(fp_solver* s, int32_t slot);
+/* A portable blob: header ("FPSTATE", version 1, the scene's shape) + the
+ * arrays in device layout.  Import needs a solver of the same shape
+ * (capacity, solids, maxFluid, constraints). */
+int fp_state_size(fp_solver* s, int64_t* bytes);
+int fp_state_export(fp_solver* s, void* buf, int64_t bytes);
+int fp_state_import(fp_solver* s, const void* buf, int64_t bytes);
+
 /* ---- batches: many independent scenes, one launch per pass ----------- */
 
 /* No counterpart in the reference (it steps one scene).  A batch steps
