@@ -252,6 +252,20 @@ class Solver:
         self._check(self._lib.fp_pick_solid(self._h, _dp(o), _dp(d), float(maxPerp), ctypes.byref(i), ctypes.byref(t)))
         return None if i.value < 0 else {"id": i.value, "t": t.value}
 
+    def select_solids(self, center, radius=0.35, same_group_as=-1):
+        """The grab selection of the reference's caller (src/main.js:293-301):
+        ids of the solids in solid same_group_as's component (any when -1)
+        within radius of center, ascending (device pass fp_select_solids)."""
+        self._flush()
+        c = np.asarray(center, np.float64).reshape(3)
+        n = ctypes.c_int32()
+        self._check(self._lib.fp_select_solids(self._h, _dp(c), float(radius), int(same_group_as), None, 0,
+                                               ctypes.byref(n)))
+        ids = np.zeros(max(n.value, 1), np.int32)
+        self._check(self._lib.fp_select_solids(self._h, _dp(c), float(radius), int(same_group_as), _ip(ids), n.value,
+                                               ctypes.byref(n)))
+        return ids[: n.value]
+
     # ---- state arrays (src/solver.js:42-48), fetched on access -----------
     def read_state(self):
         self._flush()

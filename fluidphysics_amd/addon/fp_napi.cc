@@ -351,6 +351,24 @@ napi_value PickSolid(napi_env env, napi_callback_info info) {
   return r;
 }
 
+// selectSolids(h, cx, cy, cz, radius, sameGroupAs) -> Int32Array of ids
+napi_value SelectSolids(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_solver* s = handle(env, a.v[0]);
+  const double c[3] = {num(env, a.v[1]), num(env, a.v[2]), num(env, a.v[3])};
+  const double r = num(env, a.v[4], 0.35);
+  const int g = (int)num(env, a.v[5], -1);
+  int32_t n = 0;
+  FP_CALL(env, s, fp_select_solids(s, c, r, g, nullptr, 0, &n), "fp_select_solids");
+  void* data = nullptr;
+  napi_value ab, arr;
+  NAPI_CALL(env, napi_create_arraybuffer(env, (size_t)n * 4, &data, &ab));
+  if (n > 0) FP_CALL(env, s, fp_select_solids(s, c, r, g, (int32_t*)data, n, &n), "fp_select_solids");
+  NAPI_CALL(env, napi_create_typedarray(env, napi_int32_array, (size_t)n, ab, 0, &arr));
+  return arr;
+}
+
 napi_value FluidCount(napi_env env, napi_callback_info info) {
   Args a;
   NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
@@ -640,6 +658,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"addSolidCube", AddSolidCube}, {"emit", Emit},           {"step", Step},
       {"sync", Sync},               {"setGrabs", SetGrabs},     {"cut", Cut},
       {"pickSolid", PickSolid},     {"fluidCount", FluidCount}, {"resetSolid", ResetSolid},
+      {"selectSolids", SelectSolids},
       {"clearFluid", ClearFluid},   {"readState", ReadState},   {"writeState", WriteState},
       {"readConstraints", ReadConstraints}, {"readGrid", ReadGrid}, {"buildGrid", BuildGrid},
       {"readDensity", ReadDensity},

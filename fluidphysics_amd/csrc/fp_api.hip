@@ -91,6 +91,7 @@ void scene_buffers(fp_solver* s, F& take) {
   take(s->pslot, N), take(s->count, T), take(s->cell_start, T + 1), take(s->entries_tmp, N);
   take(s->cell_entries, N), take(s->block_sums, (size_t)s->scan_blocks);
   take(s->cons_ab, C), take(s->cent, C), take(s->erest, 2 * C), take(s->ealive, 2 * C);
+  take(s->dgroup, std::max<size_t>(ns, 1));
   take(s->adj_off, ns + 1), take(s->adj, 2 * C), take(s->spos_c[0], std::max<size_t>(ns, 1));
   take(s->spos_c[1], std::max<size_t>(ns, 1)), take(s->dflags, (size_t)DF_WORDS), take(s->dpick_t, 1);
   take(s->dpick_i, 1);
@@ -771,6 +772,42 @@ int fp_pick_solid(fp_solver* s, const double o[3], const double d[3], double max
   if ((rc = settle(s))) return rc;
   *id = hi;
   if (t) *t = hi >= 0 ? ht : INFINITY;
+  return FP_OK;
+}
+
+int fp_select_solids(fp_solver* s, const double center[3], double radius, int32_t same_group_as, int32_t* ids,
+                     int32_t max_ids, int32_t* n) {
+  if (!s || !center || !n || max_ids < 0 || (max_ids > 0 && !ids))
+    return fp_fail(s, FP_EINVAL, "fp_select_solids: bad arguments");
+  *n = 0;
+  if (!s->has_cube || s->ns == 0) return FP_OK;
+  if (same_group_as >= s->ns) return fp_fail(s, FP_EINVAL, "fp_select_solids: same_group_as is not a solid");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  int g = -1;
+  if (same_group_as >= 0) {
+    if ((rc = materialize_groups(s))) return rc;
+    FP_TRY(s, h2d(s, s->dgroup, s->h_group.data(), s->ns * sizeof(int32_t)));
+    g = s->h_group[same_group_as];
+  }
+  // the ids and the count land in the solver's emit staging (grown if needed)
+  const int cap = std::max(max_ids, 1) + 1;
+  if (cap > s->emit_cap) {
+    FP_TRY(s, hipStreamSynchronize(s->stream));
+    dfree(s, s->emit_idx);
+    dfree(s, s->emit_p);
+    dfree(s, s->emit_v);
+    if ((rc = dalloc(s, &s->emit_idx, cap)) || (rc = dalloc(s, &s->emit_p, cap)) || (rc = dalloc(s, &s->emit_v, cap)))
+      return rc;
+    s->emit_cap = cap;
+  }
+  if ((rc = fp_launch_select(s, center, radius * radius, g, s->emit_idx + 1, max_ids, s->emit_idx))) return rc;
+  int total = 0;
+  FP_TRY(s, d2h(s, &total, s->emit_idx, sizeof(int)));
+  *n = total;
+  const int k = std::min(total, max_ids);
+  if (k > 0) FP_TRY(s, d2h(s, ids, s->emit_idx + 1, k * sizeof(int32_t)));
   return FP_OK;
 }
 

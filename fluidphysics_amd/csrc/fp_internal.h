@@ -155,6 +155,7 @@ struct fp_solver {
   // constraints
   int2* cons_ab = nullptr;
   int2* cent = nullptr;       // CSR entries (of a, of b) of each constraint
+  int* dgroup = nullptr;      // component labels, uploaded for a selection
   float* erest = nullptr;     // per CSR entry
   uint8_t* ealive = nullptr;  // per CSR entry
   int* adj_off = nullptr;
@@ -442,5 +443,6 @@ int fp_launch_ref_grid(fp_solver* s, bool rehash);
 int fp_launch_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[3], double nx, double ny,
                   double nz);
 int fp_launch_pick(fp_solver* s, const double o[3], const double d[3], double max_perp2);
+int fp_launch_select(fp_solver* s, const double c[3], double r2, int g, int* ids, int max_ids, int* n);
 int fp_launch_emit(fp_solver* s, int m);
 int fp_launch_count_active(fp_solver* s);

@@ -149,6 +149,51 @@ __global__ void __launch_bounds__(BLOCK) k_pick(int ns, const float4* __restrict
   }
 }
 
+// The grab selection of the reference's caller (src/main.js:293-301): solids
+// of component g (any when g < 0) within radius of c, in ascending index
+// order; the distance is decided in double from the float32 positions, as
+// the reference's JS does.  One workgroup walks the solids in chunks of
+// BLOCK: a wave ballot and the waves' counts give each match its place.
+// *n receives the total number of matches (ids past max are not written).
+__global__ void __launch_bounds__(BLOCK) k_select(int ns, const float4* __restrict__ pos,
+                                                  const int* __restrict__ group, int g, double cx, double cy,
+                                                  double cz, double r2, int* __restrict__ ids, int max_ids,
+                                                  int* __restrict__ n) {
+  __shared__ int wave_count[BLOCK / 64];
+  __shared__ int base;
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  for (int c0 = 0; c0 < ns; c0 += BLOCK) {
+    const int i = c0 + (int)threadIdx.x;
+    bool hit = false;
+    if (i < ns && (g < 0 || group[i] == g)) {
+      const float4 p = pos[i];
+      const double dx = (double)p.x - cx, dy = (double)p.y - cy, dz = (double)p.z - cz;
+      hit = dx * dx + dy * dy + dz * dz < r2;
+    }
+    const uint64_t m = __ballot(hit);
+    if (lane == 0) wave_count[wave] = __popcll(m);
+    __syncthreads();
+    int before = base;
+    for (int w = 0; w < wave; w++) before += wave_count[w];
+    before += __popcll(m & ((1ull << lane) - 1ull));
+    if (hit && before < max_ids) ids[before] = i;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int w = 0; w < BLOCK / 64; w++) base += wave_count[w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *n = base;
+}
+
+int fp_launch_select(fp_solver* s, const double c[3], double r2, int g, int* ids, int max_ids, int* n) {
+  hipLaunchKernelGGL(k_select, dim3(1), dim3(BLOCK), 0, s->stream, s->ns, s->pos, s->dgroup, g, c[0], c[1], c[2], r2,
+                     ids, max_ids, n);
+  FP_TRY(s, hipGetLastError());
+  return FP_OK;
+}
+
 // iters Jacobi iterations for every scene of the group (the solids' final
 // positions end in spos_c[(iters - 1) & 1]; see fp_group_frame).
 int fp_group_constraints(fp_group* g, int iters) {

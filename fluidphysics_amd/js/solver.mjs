@@ -153,6 +153,14 @@ export class Solver {
     return addon.pickSolid(this._h, ox, oy, oz, dx, dy, dz, maxPerp);
   }
 
+  // the grab selection of src/main.js:293-301 as a device pass: ids of the
+  // solids in solid sameGroupAs's component (any when -1) within radius of
+  // (cx, cy, cz), ascending
+  selectSolids(cx, cy, cz, radius = 0.35, sameGroupAs = -1) {
+    this._flush();
+    return addon.selectSolids(this._h, cx, cy, cz, radius, sameGroupAs);
+  }
+
   _state() {
     if (!this._views) {
       this._flush();

@@ -173,6 +173,13 @@ int fp_set_grabs(fp_solver* s, const int32_t* idx, const double* xyz, int32_t n)
  * (0 for a degenerate wedge, |d1 x d2| < 1e-8). */
 int fp_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[3], int32_t* ncut);
 
+/* The grab selection of the reference's caller (src/main.js:293-301, after
+ * pickSolid): the solids in the component of solid same_group_as (any
+ * component when it is -1) whose position lies within radius of center,
+ * tested in double, in ascending index order.  Writes at most max_ids ids;
+ * *n receives the number of matches. */
+int fp_select_solids(fp_solver* s, const double center[3], double radius, int32_t same_group_as, int32_t* ids,
+                     int32_t max_ids, int32_t* n);
 /* pickSolid(ox, oy, oz, dx, dy, dz, maxPerp) (solver.js:502-515): nearest
  * solid along the ray with perpendicular distance <= max_perp; ties go to the
  * lowest index.  *id = -1 (and *t = +inf) when nothing is hit. */

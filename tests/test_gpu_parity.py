@@ -382,3 +382,45 @@ def test_pass_profile_names():
                  "xsph"):
         assert name in stats and stats[name][0] == d["substeps"], name
     g.close()
+
+
+def test_grab_selection_matches_the_callers_rule():
+    """fp_select_solids against src/main.js:293-301 evaluated on the host:
+    solids of the picked solid's component within 0.35 of the hit point, the
+    distance in double from the float32 positions."""
+    from fluidphysics_amd import from_scene, scenes
+    sc = scenes.single_gpu_scene(frames=40)
+    log("case grab selection %s" % sc.name)
+    s = from_scene(sc)
+    cx, cy, cz = sc.cube[:3]
+    for f in range(30):
+        if f == 5:
+            # split the cube in two: the plane x = cx + 0.013 (between two
+            # lattice layers), a wedge wider than the cube
+            d1 = np.array([0.0, -1.0, -0.9]) / np.linalg.norm([0.0, -1.0, -0.9])
+            d2 = np.array([0.0, -1.0, 0.9]) / np.linalg.norm([0.0, -1.0, 0.9])
+            assert s.cut(cx + 0.013, cy + 5.0, cz, *d1, *d2) > 0
+        s.step(1.0 / 60.0, 5)
+    st = s.read_state()
+    pos = st["pos"].reshape(-1, 3)[: s.solidCount].astype(np.float64)
+    group = st["group"]
+    assert len(np.unique(group)) > 1
+    rng = np.random.default_rng(3)
+    checked = 0
+    for trial in range(12):
+        o = np.array([cx, cy, cz]) + rng.uniform(-2, 2, 3) + np.array([0.0, 2.0, 3.0])
+        target = np.array([cx, cy, cz]) + rng.uniform(-0.4, 0.4, 3)
+        d = (target - o) / np.linalg.norm(target - o)
+        hit = s.pickSolid(*o, *d, 0.25)
+        if hit is None:
+            continue
+        hp = o + d * hit["t"]
+        ids = s.select_solids(hp, 0.35, hit["id"])
+        dd = pos - hp
+        want = np.flatnonzero((group == group[hit["id"]]) & ((dd * dd).sum(axis=1) < 0.35 * 0.35))
+        assert np.array_equal(ids, want), (trial, len(ids), len(want))
+        every = s.select_solids(hp, 0.35, -1)
+        assert np.array_equal(every, np.flatnonzero((dd * dd).sum(axis=1) < 0.35 * 0.35))
+        checked += 1
+    assert checked >= 6
+    s.close()
