@@ -43,12 +43,12 @@ __device__ __forceinline__ void scan_reduce_body(int t, const int* __restrict__ 
     const int4* c4 = reinterpret_cast<const int4*>(count + base);
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS / 4; k++) {
-      const int4 v = c4[k];
+      const int4 v = gld(c4 + k);
       s += v.x + v.y + v.z + v.w;
     }
   } else {
     for (int k = 0; k < SCAN_ITEMS; k++)
-      if (base + k < t) s += count[base + k];
+      if (base + k < t) s += gld(count + base + k);
   }
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
@@ -57,7 +57,7 @@ __device__ __forceinline__ void scan_reduce_body(int t, const int* __restrict__ 
     int tot = 0;
 #pragma unroll
     for (int w = 0; w < BLOCK / 64; w++) tot += wsum[w];
-    block_sums[tile] = tot;
+    gst(block_sums + tile, tot);
   }
   __syncthreads();
 }
@@ -71,7 +71,7 @@ __device__ __forceinline__ void scan_apply_body(int t, int* __restrict__ count, 
   __shared__ int carry_s;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int c = 0;
-  for (int k = threadIdx.x; k < tile; k += BLOCK) c += block_sums[k];
+  for (int k = threadIdx.x; k < tile; k += BLOCK) c += gld(block_sums + k);
   c = wave_sum(c);
   if (lane == 0) wsum[wid] = c;
   __syncthreads();
@@ -92,18 +92,19 @@ __device__ __forceinline__ void scan_apply_body(int t, int* __restrict__ count, 
     int4* c4 = reinterpret_cast<int4*>(count + base);
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS / 4; k++) {
-      const int4 q = c4[k];
+      const int4 q = gld(c4 + k);
       v[4 * k] = q.x;
       v[4 * k + 1] = q.y;
       v[4 * k + 2] = q.z;
       v[4 * k + 3] = q.w;
-      c4[k] = make_int4(0, 0, 0, 0);
     }
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS / 4; k++) gst(c4 + k, make_int4(0, 0, 0, 0));
   } else {
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) {
-      v[k] = (base + k < t) ? count[base + k] : 0;
-      if (base + k < t) count[base + k] = 0;
+      v[k] = (base + k < t) ? gld(count + base + k) : 0;
+      if (base + k < t) gst(count + base + k, 0);
     }
   }
   int local = 0;
@@ -129,15 +130,15 @@ __device__ __forceinline__ void scan_apply_body(int t, int* __restrict__ count, 
       run += v[4 * k + 2];
       o.w = run;
       run += v[4 * k + 3];
-      s4[k] = o;
+      gst(s4 + k, o);
     }
   } else {
     for (int k = 0; k < SCAN_ITEMS; k++) {
-      if (base + k < t) start[base + k] = run;
+      if (base + k < t) gst(start + base + k, run);
       run += v[k];
     }
   }
-  if (base <= t - 1 && t - 1 < base + SCAN_ITEMS) start[t] = run;
+  if (base <= t - 1 && t - 1 < base + SCAN_ITEMS) gst(start + t, run);
   __syncthreads();
 }
 
@@ -192,8 +193,10 @@ __global__ void __launch_bounds__(BLOCK) kg_cell_count(const SceneArgs* __restri
 
 __device__ __forceinline__ void cell_scatter_body(const SceneArgs& A, int lb) {
   const int i = lb * BLOCK + threadIdx.x;
-  if (i >= A.n || A.phash[i] == 0xffffffffu) return;
-  A.ltmp[A.lstart[A.lkey[i]] + (int)A.lslot[i]] = i;
+  if (i >= A.n) return;
+  const uint32_t h = gld(A.phash + i), k = gld(A.lkey + i), sl = gld(A.lslot + i);
+  if (h == 0xffffffffu) return;
+  gst(A.ltmp + gld(A.lstart + k) + (int)sl, i);
 }
 
 __global__ void __launch_bounds__(BLOCK) kg_cell_scatter(const SceneArgs* __restrict__ S,
@@ -207,21 +210,22 @@ __global__ void __launch_bounds__(BLOCK) kg_cell_scatter(const SceneArgs* __rest
 __device__ __forceinline__ void cell_rank_body(const SceneArgs& A, int lb) {
   const int i = lb * BLOCK + threadIdx.x;
   if (i >= A.n) return;
-  const uint32_t h = A.phash[i];
+  // the bucket, the key and the position load together
+  const uint32_t h = gld(A.phash + i), k = gld(A.lkey + i);
+  const float4 p = gld(A.pos + i);
   if (h == 0xffffffffu) return;
-  const uint32_t k = A.lkey[i];
-  const int s = A.lstart[k], e = A.lstart[k + 1];
+  const int s = gld(A.lstart + k), e = gld(A.lstart + k + 1);
   int r = 0, nsol = 0;
   for (int q = s; q < e; q++) {
-    const int j = A.ltmp[q];
+    const int j = gld(A.ltmp + q);
     r += j < i ? 1 : 0;
     nsol += j < A.ns ? 1 : 0;
   }
   const int d = s + r;
-  A.sid[d] = i;
-  A.spos[d] = A.pos[i];
-  A.sphash[d] = h;
-  if (r == 0) A.lnsol[k] = nsol;
+  gst(A.sid + d, i);
+  gst(A.spos + d, p);
+  gst(A.sphash + d, h);
+  if (r == 0) gst(A.lnsol + k, nsol);
 }
 
 __global__ void __launch_bounds__(BLOCK) kg_cell_rank(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
@@ -266,9 +270,10 @@ int fp_group_cell_grid(fp_group* g, bool count_done) {
 __device__ __forceinline__ void fluid_scatter_body(const SceneArgs& A, int lb) {
   const int e = lb * BLOCK + threadIdx.x;
   if (e >= A.n) return;
-  const uint32_t k = A.tkey[e];
+  const uint32_t k = gld(A.tkey + e), sl = gld(A.tslot + e);
+  const int x = gld(A.sid + e);
   if (k == NO_KEY) return;
-  A.ltmp[A.lstart[k] + (int)A.tslot[e]] = A.sid[e];
+  gst(A.ltmp + gld(A.lstart + k) + (int)sl, x);
 }
 
 __global__ void __launch_bounds__(BLOCK) kg_fluid_scatter(const SceneArgs* __restrict__ S,
@@ -279,17 +284,19 @@ __global__ void __launch_bounds__(BLOCK) kg_fluid_scatter(const SceneArgs* __res
 __device__ __forceinline__ void fluid_rank_body(const SceneArgs& A, int lb) {
   const int e = lb * BLOCK + threadIdx.x;
   if (e >= A.n) return;
-  const uint32_t k = A.tkey[e];
+  // everything indexed by the grid slot loads together
+  const uint32_t k = gld(A.tkey + e), hb = gld(A.sphash + e);
+  const int i = gld(A.sid + e);
+  const float4 tp = gld(A.tpos + e);
   if (k == NO_KEY) return;
-  const int i = A.sid[e];
-  const int s = A.lstart[k], t = A.lstart[k + 1];
+  const int s = gld(A.lstart + k), t = gld(A.lstart + k + 1);
   int r = 0;
-  for (int q = s; q < t; q++) r += A.ltmp[q] < i ? 1 : 0;
+  for (int q = s; q < t; q++) r += gld(A.ltmp + q) < i ? 1 : 0;
   const int d = s + r;
-  A.fsid[d] = i;
-  A.fslot[i] = d;
-  A.spos2[d] = A.tpos[e];
-  A.sphash2[d] = A.sphash[e];
+  gst(A.fsid + d, i);
+  gst(A.fslot + i, d);
+  gst(A.spos2 + d, tp);
+  gst(A.sphash2 + d, hb);
 }
 
 __global__ void __launch_bounds__(BLOCK) kg_fluid_rank(const SceneArgs* __restrict__ S, const int2* __restrict__ W,

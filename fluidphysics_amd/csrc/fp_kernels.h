@@ -226,6 +226,10 @@ __device__ __forceinline__ void gst(float4* p, const float4& v) {
   fp_f4v w = {v.x, v.y, v.z, v.w};
   *(FP_GLOBAL fp_f4v*)p = w;
 }
+__device__ __forceinline__ void gst(int4* p, const int4& v) {
+  fp_i4v w = {v.x, v.y, v.z, v.w};
+  *(FP_GLOBAL fp_i4v*)p = w;
+}
 __device__ __forceinline__ void gst(float2* p, const float2& v) {
   fp_f2v w = {v.x, v.y};
   *(FP_GLOBAL fp_f2v*)p = w;
@@ -256,12 +260,13 @@ __device__ __forceinline__ void box_update(bool has, float px, float py, float p
     mxz = fmaxf(mxz, __shfl_xor(mxz, off));
   }
   if ((threadIdx.x & 63) == 0) {
-    atomicMin(&box[0], order_float(mnx));
-    atomicMin(&box[1], order_float(mny));
-    atomicMin(&box[2], order_float(mnz));
-    atomicMax(&box[3], order_float(mxx));
-    atomicMax(&box[4], order_float(mxy));
-    atomicMax(&box[5], order_float(mxz));
+    FP_GLOBAL int* gb = (FP_GLOBAL int*)box;
+    __hip_atomic_fetch_min(gb + 0, order_float(mnx), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_min(gb + 1, order_float(mny), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_min(gb + 2, order_float(mnz), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_max(gb + 3, order_float(mxx), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_max(gb + 4, order_float(mxy), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_max(gb + 5, order_float(mxz), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -300,7 +305,9 @@ __device__ __forceinline__ uint32_t wave_bucket_slot(int* __restrict__ count, ui
   const uint64_t above = lane == 63 ? 0ull : (stops & (~0ull << (lane + 1)));
   const int next = above ? (int)__builtin_ctzll(above) : 64;
   int base = 0;
-  if (head) base = atomicAdd(&count[key], next - lane);
+  if (head)
+    base = __hip_atomic_fetch_add((FP_GLOBAL int*)&count[key], next - lane, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
   const uint64_t upto = heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
   const int hl = upto ? 63 - (int)__builtin_clzll(upto) : 0;
   const int b = __shfl(base, hl);

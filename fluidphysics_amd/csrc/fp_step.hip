@@ -32,34 +32,40 @@ template <bool COUNT>
 __device__ __forceinline__ void integrate_body(const SceneArgs& A, int lb) {
   const int i = lb * BLOCK + threadIdx.x;
   const bool in = i < A.n;
-  const bool act = in && A.active[i];
-  float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+  // the flag, the velocity and the position load together (lanes past the
+  // end read slot 0)
+  const int j = in ? i : 0;
+  const uint8_t flag = gld(A.active + j);
+  float4 v = gld(A.vel + j);
+  float4 p = gld(A.pos + j);
+  const bool act = in && flag;
   uint32_t k = 0;
   if (act) {
-    float4 v = A.vel[i];
-    p = A.pos[i];
     v.y += A.gvy;
     if (A.gvx != 0.f) v.x += A.gvx;
     if (A.gvz != 0.f) v.z += A.gvz;
-    A.prev[i] = make_float4(p.x, p.y, p.z, 0.f);
+    gst(A.prev + i, make_float4(p.x, p.y, p.z, 0.f));
     p.x = fmaf(v.x, A.h, p.x);
     p.y = fmaf(v.y, A.h, p.y);
     p.z = fmaf(v.z, A.h, p.z);
-    A.pos[i] = p;
-    A.vel[i] = v;
+    gst(A.pos + i, p);
+    gst(A.vel + i, v);
     if (COUNT) {
-      A.phash[i] = cell_hash(cell_of(p.x, p.y, p.z, A.fp.cg.ref_inv), A.fp.cg.hmask);
+      gst(A.phash + i, cell_hash(cell_of(p.x, p.y, p.z, A.fp.cg.ref_inv), A.fp.cg.hmask));
       k = cell_key(A.fp.cg, cell_of(p.x, p.y, p.z, A.fp.cg.inv_cell));
-      A.lkey[i] = k;
+      gst(A.lkey + i, k);
     }
-  } else if (in) {
-    if (COUNT) A.phash[i] = 0xffffffffu;
-    A.rho[i] = make_float2(0.f, 0.f);
+  } else {
+    p = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (in) {
+      if (COUNT) gst(A.phash + i, 0xffffffffu);
+      gst(A.rho + i, make_float2(0.f, 0.f));
+    }
   }
   if (COUNT) {
     // neighbouring indices are often in one cell: one atomic per run
     const uint32_t sl = wave_bucket_slot(A.lcount, k, act);
-    if (act) A.lslot[i] = sl;
+    if (act) gst(A.lslot + i, sl);
     box_update(act && i < A.ns, p.x, p.y, p.z, A.sbox);
   }
 }
@@ -99,12 +105,17 @@ __device__ __forceinline__ void bounds_velocity_body(const SceneArgs& A, int lb)
   const int i = lb * BLOCK + threadIdx.x;
   if (i >= bp.n) return;
   const bool solid = i < bp.ns;
-  if (!A.active[i]) {
-    if (solid) pos[i] = solid_src[i];  // the reference's constraint sweep moves inactive solids too
+  // a solid's position loads with its flag (inactive solids move too); an
+  // inactive fluid slot, most of a fluid range, loads nothing more
+  const uint8_t flag = gld(A.active + i);
+  float4 p = solid ? gld(solid_src + i) : make_float4(0.f, 0.f, 0.f, 0.f);
+  if (!flag) {
+    if (solid) gst(pos + i, p);  // the reference's constraint sweep moves inactive solids too
     return;
   }
-  float4 p = solid ? solid_src[i] : pos[i];
-  const float4 pv = A.prev[i];
+  if (!solid) p = gld(pos + i);
+  const float4 pv = gld(A.prev + i);
+  const int fs = solid ? 0 : gld(A.fslot + i);
   const float r = solid ? bp.rs : bp.rf;
   if (p.y < r) {
     p.y = r;
@@ -123,10 +134,10 @@ __device__ __forceinline__ void bounds_velocity_body(const SceneArgs& A, int lb)
       p.z = fabsf(p.z) < bp.wall_z ? side * (bp.wall_z - lim) : side * (bp.wall_z + lim);
     }
   }
-  pos[i] = p;
+  gst(pos + i, p);
   if (!solid && (p.y < -3.f || p.x * p.x + p.z * p.z > bp.cull_r2)) {
-    A.active[i] = 0;
-    A.svtmp[A.fslot[i]] = A.vel[i];
+    gst(A.active + i, (uint8_t)0);
+    gst(A.svtmp + fs, gld(A.vel + i));
     return;
   }
   float vx = (p.x - pv.x) * bp.inv_h;
@@ -145,9 +156,9 @@ __device__ __forceinline__ void bounds_velocity_body(const SceneArgs& A, int lb)
   }
   const float4 v = make_float4(vx, vy, vz, 0.f);
   if (solid)
-    A.vel[i] = v;
+    gst(A.vel + i, v);
   else
-    A.svtmp[A.fslot[i]] = v;
+    gst(A.svtmp + fs, v);
 }
 
 __global__ void __launch_bounds__(BLOCK) kg_bounds_velocity(const SceneArgs* __restrict__ S,
