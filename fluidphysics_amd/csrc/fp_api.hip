@@ -87,7 +87,8 @@ void scene_buffers(fp_solver* s, F& take) {
   take(s->ltmp, N), take(s->lkey, N), take(s->lslot, N), take(s->sid, N), take(s->spos, N), take(s->spos2, N);
   take(s->svtmp, N), take(s->sphash, N), take(s->sP, N), take(s->tpos, N), take(s->tkey, N), take(s->tslot, N);
   take(s->fsid, N), take(s->fslot, N), take(s->sphash2, N), take(s->sbox, 6);
-  take(s->nbr, (size_t)(s->pair_slots + s->fluid_lanes) * N), take(s->ncount, N * (size_t)s->fluid_lanes);
+  const size_t lanes = N * (size_t)s->fluid_lanes, lanes64 = (lanes + 63) / 64 * 64;
+  take(s->nbr, (size_t)(s->pair_slots / s->fluid_lanes + 1) * lanes64), take(s->ncount, lanes);
   take(s->pslot, N), take(s->count, T), take(s->cell_start, T + 1), take(s->entries_tmp, N);
   take(s->cell_entries, N), take(s->block_sums, (size_t)s->scan_blocks);
   take(s->cons_ab, C), take(s->cent, C), take(s->erest, 2 * C), take(s->ealive, 2 * C);
@@ -1122,7 +1123,7 @@ int fp_batch_set_contact_lanes(fp_batch* b, int32_t lanes) {
   b->g.clanes_pinned = lanes != 0;
   b->g.clanes_best = lanes;
   b->g.tune_k = 0;
-  b->g.tune_pending = -1;
+  b->g.tune_pending = false;
   return FP_OK;
 }
 

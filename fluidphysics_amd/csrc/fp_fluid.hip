@@ -470,16 +470,28 @@ __device__ __forceinline__ RowRange row_range(const FluidParams& fp, const int* 
 //
 // The density pass records the pairs it finds for the displacement and XSPH
 // passes (the reference freezes its pair list at density time too,
-// solver.js:311, :431-445): entry k of fluid slot e lives at
-// nbr[k * stride + e], so a wave reading its lanes' k-th pairs reads one
-// contiguous run.  An entry packs the neighbour's fluid slot (27 bits) and
-// the weight - 1 (5 bits).  A particle with more than pair_slots pairs gets
-// the count pair_slots + 1 and is gathered again from the grid by the later
-// passes (same pairs, same order); its extra entries go to a sink row
-// (row pair_slots) so the recording loop needs no bounds branch.
+// solver.js:311, :431-445).  The lanes of the fluid order come in blocks of
+// 64, one wave of the fluid passes; block b holds cap + 1 rows of 64
+// entries, entry k of lane v at nbr[(v / 64) * (cap + 1) * 64 + k * 64 +
+// v % 64].  A wave reading its lanes' k-th pairs reads one contiguous run,
+// and every access is the wave's block (scalar registers) plus a 32-bit
+// lane offset.  An entry packs the neighbour's fluid slot (27 bits) and the
+// weight - 1 (5 bits).  A lane with more than cap pairs gets the count
+// cap + 1 and its slot is gathered again from the grid by the later passes
+// (same pairs, same order); its extra entries go to the block's sink row
+// (row cap) so the recording loop needs no bounds branch.
 
 #define NB_QBITS 27
 #define NB_QMASK ((1u << NB_QBITS) - 1u)
+#define NB_ROWB 256u  // bytes per row of a block
+
+// The block of the wave that lane v belongs to (the same in every lane of a
+// wave: the fluid passes' lanes are v = block * FLUID_BLOCK + threadIdx.x).
+__device__ __forceinline__ uint32_t* pair_block(const PairList& pl, int v) {
+  const int b = __builtin_amdgcn_readfirstlane(v >> 6);
+  return pl.nbr + (size_t)b * ((size_t)(pl.cap + 1) * 64);
+}
+__device__ __forceinline__ uint32_t pair_lane_off(int v) { return (uint32_t)(v & 63) * 4u; }
 
 
 // v_sqrt_f32 (within 1 ulp); the reference's own square root is a double
@@ -500,8 +512,9 @@ __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sq
 struct DensityGather {
   float rho = 0.f, rhon = 0.f;
   int cnt = 0;
-  uint32_t* wp;  // where entry cnt goes while cnt < cap
-  int cap, stride;
+  uint32_t* base;  // the wave's block of the pair list
+  uint32_t wofs;   // byte offset of entry cnt (the sink entry once cnt >= cap)
+  uint32_t wend;   // byte offset of this lane's sink entry
 };
 
 __device__ __forceinline__ void density_take(const FluidParams& fp, const PairList& pl, int q, int w, float d2,
@@ -512,8 +525,8 @@ __device__ __forceinline__ void density_take(const FluidParams& fp, const PairLi
   g.rhon += w ? u2 * u : 0.f;
   if (w) {
     // entries past pl.cap all land in the sink row pl.cap, which nothing reads
-    gst(g.wp, (uint32_t)q | ((uint32_t)(w - 1) << NB_QBITS));
-    g.wp += g.cnt < g.cap ? g.stride : 0;
+    gst_at(g.base, g.wofs, (uint32_t)q | ((uint32_t)(w - 1) << NB_QBITS));
+    g.wofs = min(g.wofs + NB_ROWB, g.wend);
     g.cnt++;
   }
 }
@@ -521,8 +534,10 @@ __device__ __forceinline__ void density_take(const FluidParams& fp, const PairLi
 // One batch of four candidates in slots q .. q+3 (those past last are not
 // candidates).  Weight 1 for d^2 < H^2 (1 - 1e-5) unless a flag is set on
 // either particle; the band and flagged cases go to the one rare branch.
+// flag_min: the least w bits of a candidate that make the pair flagged (0
+// when me itself is flagged, so every candidate is).
 __device__ __forceinline__ void density_batch(const FluidParams& fp, const PairSource& src, const PairList& pl,
-                                              const PairSelf& me, bool me_flagged, int q, int last, const float4* c,
+                                              const PairSelf& me, uint32_t flag_min, int q, int last, const float4* c,
                                               DensityGather& g) {
   float d2[DENSITY_BATCH];
   int w[DENSITY_BATCH];
@@ -532,9 +547,10 @@ __device__ __forceinline__ void density_batch(const FluidParams& fp, const PairS
     const float dx = me.p.x - c[i].x, dy = me.p.y - c[i].y, dz = me.p.z - c[i].z;
     d2[i] = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const bool cand = (q + i <= last) & (q + i != me.e);
-    const bool inside = cand & (d2[i] < fp.H2_lo);
-    const bool band = cand & (d2[i] >= fp.H2_lo) & (d2[i] < fp.H2_hi);
-    const bool flagged = me_flagged | (fbits(c[i].w) >= (1u << PF_SHIFT));
+    const bool near = d2[i] < fp.H2_lo;
+    const bool inside = cand & near;
+    const bool band = cand & !near & (d2[i] < fp.H2_hi);  // NaN: neither
+    const bool flagged = fbits(c[i].w) >= flag_min;
     w[i] = inside ? 1 : 0;
     rare |= band | (inside & flagged);
   }
@@ -549,7 +565,7 @@ __device__ __forceinline__ void density_batch(const FluidParams& fp, const PairS
 
 __device__ __forceinline__ void load_batch(const float4* __restrict__ row, int q, int last, float4* b) {
 #pragma unroll
-  for (int i = 0; i < DENSITY_BATCH; i++) b[i] = gld(row + min(q + i, last));
+  for (int i = 0; i < DENSITY_BATCH; i++) b[i] = gld_at(row, (uint32_t)min(q + i, last) << 4);
 }
 
 __device__ __forceinline__ void density_gather(const FluidParams& fp, const PairSource& src, const PairList& pl,
@@ -557,7 +573,7 @@ __device__ __forceinline__ void density_gather(const FluidParams& fp, const Pair
   const Cell c = cell_of(me.p.x, me.p.y, me.p.z, fp.cg.inv_cell);
   const uint32_t kx = cell_key(fp.cg, c);
   const float4* __restrict__ row = src.spos2;
-  const bool me_flagged = me.moved() | me.dup();
+  const uint32_t flag_min = (me.moved() | me.dup()) ? 0u : (1u << PF_SHIFT);
   const FaceDist f = face_dist(me.p.x, me.p.y, me.p.z, c, fp.cs);
   RowRange nx = row_range(fp, src.lstart, kx, 0, f);
 #pragma unroll 1
@@ -579,11 +595,11 @@ __device__ __forceinline__ void density_gather(const FluidParams& fp, const Pair
       while (q <= last) {
         const int q1 = q + step;
         load_batch(row, q1, last, b1);
-        density_batch(fp, src, pl, me, me_flagged, q, last, b0, g);
+        density_batch(fp, src, pl, me, flag_min, q, last, b0, g);
         if (q1 > last) break;
         const int q2 = q1 + step;
         load_batch(row, q2, last, b0);
-        density_batch(fp, src, pl, me, me_flagged, q1, last, b1, g);
+        density_batch(fp, src, pl, me, flag_min, q1, last, b1, g);
         q = q2;
       }
     }
@@ -629,12 +645,14 @@ __device__ __forceinline__ void visit_rows_simple(const FluidParams& fp, const P
 template <class F>
 __device__ __forceinline__ void replay_pairs(const PairSource& src, const PairList& pl, const PairSelf& me, int cnt,
                                              F& f) {
-  const uint32_t* __restrict__ row = pl.nbr + me.v;
-  const size_t st = (size_t)pl.stride;
+  const uint32_t* base = pair_block(pl, me.v);
+  const uint32_t lo = pair_lane_off(me.v);
   const int last = cnt - 1;
+  // entry k of this lane at byte lo + k * NB_ROWB of the wave's block
+  const uint32_t olast = lo + (uint32_t)max(last, 0) * NB_ROWB;
   uint32_t v[4];
 #pragma unroll
-  for (int i = 0; i < 4; i++) v[i] = cnt > 0 ? gld(row + (size_t)min(i, last) * st) : 0u;
+  for (int i = 0; i < 4; i++) v[i] = cnt > 0 ? gld_at(base, min(lo + (uint32_t)i * NB_ROWB, olast)) : 0u;
   for (int k = 0; k < cnt; k += 4) {
     // this batch's gathers first, then the next batch's entries: the loads
     // complete in issue order, so waiting for the gathers leaves the
@@ -644,7 +662,7 @@ __device__ __forceinline__ void replay_pairs(const PairSource& src, const PairLi
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int q = (int)(v[i] & NB_QMASK);
-      if (F::kNeedPos) py[i] = gld_xyz(src.spos2 + q);
+      if (F::kNeedPos) py[i] = gld_xyz_at(src.spos2, (uint32_t)q << 4);
       ex[i] = f.fetch(q);
     }
     uint32_t u[4];
@@ -652,7 +670,7 @@ __device__ __forceinline__ void replay_pairs(const PairSource& src, const PairLi
     for (int i = 0; i < 4; i++) u[i] = v[i];
     if (k + 4 < cnt) {
 #pragma unroll
-      for (int i = 0; i < 4; i++) v[i] = gld(row + (size_t)min(k + 4 + i, last) * st);
+      for (int i = 0; i < 4; i++) v[i] = gld_at(base, min(lo + (uint32_t)(k + 4 + i) * NB_ROWB, olast));
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -719,9 +737,9 @@ __device__ __forceinline__ void density_body(const SceneArgs& A, int lb) {
   PairSelf me;
   if (!pair_self(fp, src, fluid_lane(lb), me)) return;
   DensityGather g;
-  g.wp = pl.nbr + me.v;
-  g.cap = pl.cap;
-  g.stride = pl.stride;
+  g.base = pair_block(pl, me.v);
+  g.wofs = pair_lane_off(me.v);
+  g.wend = g.wofs + (uint32_t)pl.cap * NB_ROWB;
   density_gather(fp, src, pl, me, g);
   density_store(fp, pl, me, g, A.sP, A.rho, A.dflags);
 }
@@ -760,7 +778,7 @@ struct DisplaceAcc {
   float2 Px;
   float inv_H, max_pair;
   float sx = 0.f, sy = 0.f, sz = 0.f;
-  __device__ __forceinline__ float2 fetch(int q) const { return gld(sP + q); }
+  __device__ __forceinline__ float2 fetch(int q) const { return gld_at(sP, (uint32_t)q << 3); }
   __device__ __forceinline__ void use(int, int w, float dx, float dy, float dz, float d2, float2 Py) {
     const float d = fast_sqrt(d2);
     if (d < 1e-9f) return;
@@ -818,7 +836,7 @@ struct XsphAcc {
   const float4* __restrict__ svtmp;
   float sx = 0.f, sy = 0.f, sz = 0.f;
   int m = 0;
-  __device__ __forceinline__ float4 fetch(int q) const { return gld_xyz(svtmp + q); }
+  __device__ __forceinline__ float4 fetch(int q) const { return gld_xyz_at(svtmp, (uint32_t)q << 4); }
   __device__ __forceinline__ void use(int, int w, float, float, float, float, float4 v) {
     const float fw = (float)w;
     sx += fw * v.x;
@@ -915,33 +933,43 @@ int fp_scene_blocks(const fp_solver* s, int kind, int contact_lanes) {
   return 0;
 }
 
+static void launch_contacts_solid(fp_group* g, int lanes, const int2* items, int n) {
+  if (n <= 0) return;
+  switch (lanes) {
+    case 32:
+      hipLaunchKernelGGL(kg_contacts_solid<32>, dim3(n), dim3(BLOCK), 0, g->stream, g->d_args, items, n);
+      break;
+    case 16:
+      hipLaunchKernelGGL(kg_contacts_solid<16>, dim3(n), dim3(BLOCK), 0, g->stream, g->d_args, items, n);
+      break;
+    case 8:
+      hipLaunchKernelGGL(kg_contacts_solid<8>, dim3(n), dim3(BLOCK), 0, g->stream, g->d_args, items, n);
+      break;
+    default:
+      hipLaunchKernelGGL(kg_contacts_solid<4>, dim3(n), dim3(BLOCK), 0, g->stream, g->d_args, items, n);
+  }
+}
+
 int fp_group_contacts(fp_group* g) {
   PassScope ps(g->prof, g->stream, P_CONTACTS);
-  hipEvent_t* ev = nullptr;
   if (g->tune_record) {
+    // a tuning frame: this substep's candidate, from its own work list,
+    // between events
+    const int c = g->tune_cand;
     const size_t need = 2 * (size_t)(g->tune_used + 1);
     while (g->tune_ev.size() < need) {
       hipEvent_t e;
       FP_GTRY(g, hipEventCreate(&e));
       g->tune_ev.push_back(e);
     }
-    ev = &g->tune_ev[2 * (size_t)g->tune_used++];
+    hipEvent_t* ev = &g->tune_ev[2 * (size_t)g->tune_used++];
+    g->tune_of.push_back(c);
     FP_GTRY(g, hipEventRecord(ev[0], g->stream));
+    launch_contacts_solid(g, kTuneLanes[c], g->d_tune_items + g->tune_off[c], g->tune_n[c]);
+    FP_GTRY(g, hipEventRecord(ev[1], g->stream));
+  } else {
+    launch_contacts_solid(g, g->clanes, g->d_items[W_SOLID], g->n_items[W_SOLID]);
   }
-  switch (g->clanes) {
-    case 32:
-      GROUP_LAUNCH(g, kg_contacts_solid<32>, W_SOLID, BLOCK);
-      break;
-    case 16:
-      GROUP_LAUNCH(g, kg_contacts_solid<16>, W_SOLID, BLOCK);
-      break;
-    case 8:
-      GROUP_LAUNCH(g, kg_contacts_solid<8>, W_SOLID, BLOCK);
-      break;
-    default:
-      GROUP_LAUNCH(g, kg_contacts_solid<4>, W_SOLID, BLOCK);
-  }
-  if (ev) FP_GTRY(g, hipEventRecord(ev[1], g->stream));
   GROUP_LAUNCH(g, kg_contacts_fluid, W_PARTICLE, BLOCK);
   FP_GTRY(g, hipGetLastError());
   return FP_OK;

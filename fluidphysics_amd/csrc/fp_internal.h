@@ -142,7 +142,7 @@ struct fp_solver {
   float4 *spos2 = nullptr, *svtmp = nullptr;
   uint32_t* sphash2 = nullptr;
   float2* sP = nullptr;
-  uint32_t* nbr = nullptr;  // pair list, (pair_slots + 1) x capacity, entry-major; the last row is a sink
+  uint32_t* nbr = nullptr;  // pair list: per 64 lanes of the fluid order, cap + 1 rows of 64 (the last a sink)
   int* ncount = nullptr;
   int pair_slots = 0;
   int* sbox = nullptr;  // solids' bounding box at grid time (6 ordered ints)
@@ -229,11 +229,12 @@ struct PairSource {
   const int* __restrict__ lstart;       // fluid-order cell starts
 };
 
-// The pair list the density pass records (see fp_fluid.hip).
+// The pair list the density pass records (layout in fp_fluid.hip): blocks
+// of cap + 1 rows of 64 entries, one block per 64 lanes of the fluid order.
 struct PairList {
   uint32_t* __restrict__ nbr;
   int* __restrict__ ncount;
-  int stride, cap;
+  int cap;  // pairs per lane; row cap of a block is its sink
 };
 
 // The grid order as the contact pass reads it.
@@ -297,6 +298,9 @@ enum WorkKind {
   W_NUM
 };
 
+// lanes per solid of the solid contact pass that a group's tuner tries
+constexpr int kTuneLanes[4] = {4, 8, 16, 32};
+
 // A set of scenes stepped together: one launch per pass over all of them
 // (a solver alone is a group of one).  Owns the device scene table, the work
 // lists and the captured frame graph.
@@ -325,20 +329,32 @@ struct fp_group {
   bool use_graph = true;
   int64_t graph_launches = 0;
   // autotuned launch shape: lanes per solid of the solid contact pass (the
-  // result does not depend on it).  The first frames try 4, 8, 16 and 32
-  // lanes, timed with events, and the fastest is kept; a group whose solid
-  // count changes by more than 2x tunes again.
-  int clanes = 4;                 // lanes of the current frame
+  // result does not depend on it).  The first TUNE_FRAMES frames time 4, 8,
+  // 16 and 32 lanes with events, rotating the candidate from substep to
+  // substep (and the starting candidate from frame to frame) so that every
+  // candidate is timed on the same states; the fastest mean is kept.  A group
+  // whose solid count changes by more than 2x tunes again.
+  static constexpr int TUNE_FRAMES = 4;
+  int clanes = 4;                 // lanes of the work lists' W_SOLID entries
   int items_clanes = 0;           // lanes the work lists were built for
   int clanes_best = 0;            // 0 while tuning
   bool clanes_pinned = false;     // set by the caller: no tuning
-  int tune_k = 0;                 // candidate being timed
-  int tune_pending = -1;          // candidate whose events await reading
-  bool tune_record = false;       // record events around the contact launches
+  int tune_k = 0;                 // tuning frames done
+  bool tune_pending = false;      // a tuning frame's events await reading
+  bool tune_record = false;       // this frame tunes
+  int tune_cand = 0;              // candidate of the current substep
   int tune_used = 0;              // event pairs recorded in the pending frame
-  float tune_ms[4] = {0.f, 0.f, 0.f, 0.f};
+  std::vector<int> tune_of;       // candidate of each event pair
+  double tune_sum[4] = {0.0, 0.0, 0.0, 0.0};
+  int tune_cnt[4] = {0, 0, 0, 0};
   int64_t tuned_solids = 0;
   std::vector<hipEvent_t> tune_ev;
+  // W_SOLID work lists of the four candidates, back to back, while tuning
+  int2* d_tune_items = nullptr;
+  int tune_items_cap = 0;
+  int tune_off[4] = {0, 0, 0, 0};
+  int tune_n[4] = {0, 0, 0, 0};
+  bool tune_items_ok = false;
   std::string err;
 };
 

@@ -235,6 +235,28 @@ __device__ __forceinline__ void gst(float2* p, const float2& v) {
   *(FP_GLOBAL fp_f2v*)p = w;
 }
 
+// base + a 32-bit byte offset.  With the base the same in every lane (an
+// array of the scene table, a wave's block of the pair list) the access is one
+// instruction with the base in scalar registers and the offset in one vector
+// register: no 64-bit address arithmetic per access.
+#define FP_AT(T, base, off) ((const FP_GLOBAL T*)((const FP_GLOBAL char*)(base) + (uint32_t)(off)))
+__device__ __forceinline__ float4 gld_at(const float4* base, uint32_t off) {
+  const fp_f4v v = *FP_AT(fp_f4v, base, off);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 gld_xyz_at(const float4* base, uint32_t off) {
+  const fp_f3v v = *FP_AT(fp_f3v, base, off);
+  return make_float4(v.x, v.y, v.z, 0.f);
+}
+__device__ __forceinline__ float2 gld_at(const float2* base, uint32_t off) {
+  const fp_f2v v = *FP_AT(fp_f2v, base, off);
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ uint32_t gld_at(const uint32_t* base, uint32_t off) { return *FP_AT(uint32_t, base, off); }
+__device__ __forceinline__ void gst_at(uint32_t* base, uint32_t off, uint32_t v) {
+  *(FP_GLOBAL uint32_t*)((FP_GLOBAL char*)base + off) = v;
+}
+
 // ---- solids' bounding box ------------------------------------------------------
 
 // Floats as ints whose signed order is the floats' order.

@@ -222,7 +222,6 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
   a.src.lstart = s->lstart;
   a.pl.nbr = s->nbr;
   a.pl.ncount = s->ncount;
-  a.pl.stride = s->n * s->fluid_lanes;
   a.pl.cap = s->pair_slots / s->fluid_lanes;
   a.cc = ContactCtx{s->spos, s->sid, s->lstart, s->lnsol, s->adj_off, s->adj, s->ealive, s->sbox};
   a.bp = bounds_params(s, h);
@@ -281,6 +280,10 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
 void fp_group_free(fp_group* g) {
   for (hipEvent_t e : g->tune_ev) (void)hipEventDestroy(e);
   g->tune_ev.clear();
+  if (g->d_tune_items) fp_pool_put(g->device, g->d_tune_items);
+  g->d_tune_items = nullptr;
+  g->tune_items_cap = 0;
+  g->tune_items_ok = false;
   if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   g->gexec = nullptr;
@@ -308,7 +311,7 @@ static void drop_graph(fp_group* g) {
 // parameters or h changed.  A changed membership also drops the graph.
 int fp_group_prepare(fp_group* g, double h) {
   bool changed = g->dirty || g->built_h != h || g->versions.size() != g->scenes.size() ||
-                 g->items_clanes != g->clanes;
+                 g->items_clanes != g->clanes || (g->tune_record && !g->tune_items_ok);
   if (!changed)
     for (size_t k = 0; k < g->scenes.size(); k++)
       if (g->versions[k] != g->scenes[k]->version) changed = true;
@@ -378,6 +381,33 @@ int fp_group_prepare(fp_group* g, double h) {
       g->n_items[kind] = n;
     }
     g->items_clanes = g->clanes;
+    g->tune_items_ok = false;
+  }
+  // a tuning frame launches the solid contacts from each candidate's list
+  std::vector<int2> titems;
+  if (g->tune_record && !g->tune_items_ok) {
+    for (int c = 0; c < 4; c++) {
+      g->tune_off[c] = (int)titems.size();
+      for (int k = 0; k < ns; k++) {
+        const int nb = fp_scene_blocks(g->scenes[k], W_SOLID, kTuneLanes[c]);
+        for (int b = 0; b < nb; b++) titems.push_back(make_int2(k, b));
+      }
+      g->tune_n[c] = (int)titems.size() - g->tune_off[c];
+    }
+    const int n = (int)titems.size();
+    if (n > g->tune_items_cap) {
+      FP_GTRY(g, hipStreamSynchronize(g->stream));
+      if (g->d_tune_items) fp_pool_put(g->device, g->d_tune_items);
+      g->d_tune_items = nullptr;
+      g->tune_items_cap = 0;
+      FP_GTRY(g, fp_pool_get(g->device, sizeof(int2) * std::max(n, 1), (void**)&g->d_tune_items, nullptr));
+      g->tune_items_cap = n;
+    }
+    if (n) {
+      FP_GTRY(g, hipMemcpyAsync(g->d_tune_items, titems.data(), sizeof(int2) * n, hipMemcpyHostToDevice,
+                                g->stream));
+    }
+    g->tune_items_ok = true;
   }
   // the host vectors are read by the copies above before they go out of scope
   FP_GTRY(g, hipStreamSynchronize(g->stream));
@@ -429,10 +459,8 @@ static int group_substep(fp_group* g) {
 // into a HIP graph once and replayed.
 // ---- contact-lane autotuning ---------------------------------------------------
 
-static const int kTuneLanes[4] = {4, 8, 16, 32};
-
-// Before a frame: read the timing of the last tuning frame, pick the lane
-// count for this one.
+// Before a frame: decide whether it tunes (the candidates' lane counts are
+// then picked per substep), else use the tuned lane count.
 void fp_group_tune_begin(fp_group* g) {
   int64_t solids = 0;
   for (fp_solver* s : g->scenes) solids += s->ns;
@@ -449,33 +477,43 @@ void fp_group_tune_begin(fp_group* g) {
     g->clanes = g->clanes_best;
     return;
   }
-  g->clanes = kTuneLanes[g->tune_k];
+  if (g->tune_k == 0)
+    for (int c = 0; c < 4; c++) {
+      g->tune_sum[c] = 0.0;
+      g->tune_cnt[c] = 0;
+    }
   g->tune_record = true;
   g->tune_used = 0;
+  g->tune_of.clear();
   g->tuned_solids = solids;
 }
 
 // After a tuning frame has been queued: its events are read at the next
 // frame (or sync), once they have completed.
 int fp_group_tune_end(fp_group* g) {
-  if (g->tune_pending < 0) return FP_OK;
-  const int k = g->tune_pending;
-  g->tune_pending = -1;
-  float total = 0.f;
+  if (!g->tune_pending) return FP_OK;
+  g->tune_pending = false;
   for (int i = 0; i < g->tune_used; i++) {
     float ms = 0.f;
     FP_GTRY(g, hipEventSynchronize(g->tune_ev[2 * i + 1]));
     FP_GTRY(g, hipEventElapsedTime(&ms, g->tune_ev[2 * i], g->tune_ev[2 * i + 1]));
-    total += ms;
+    g->tune_sum[g->tune_of[i]] += ms;
+    g->tune_cnt[g->tune_of[i]]++;
   }
-  g->tune_ms[k] = total / std::max(1, g->tune_used);
-  g->tune_k = k + 1;
-  if (g->tune_k == 4) {
-    int best = 0;
-    for (int i = 1; i < 4; i++)
-      if (g->tune_ms[i] < g->tune_ms[best]) best = i;
-    g->clanes_best = kTuneLanes[best];
+  if (++g->tune_k == fp_group::TUNE_FRAMES) {
+    int best = -1;
+    for (int c = 0; c < 4; c++)
+      if (g->tune_cnt[c] &&
+          (best < 0 || g->tune_sum[c] / g->tune_cnt[c] < g->tune_sum[best] / g->tune_cnt[best]))
+        best = c;
+    g->clanes_best = kTuneLanes[best < 0 ? 0 : best];
     g->tune_k = 0;
+    // the candidates' lists are idle: every launch that read them has been
+    // waited for above
+    if (g->d_tune_items) fp_pool_put(g->device, g->d_tune_items);
+    g->d_tune_items = nullptr;
+    g->tune_items_cap = 0;
+    g->tune_items_ok = false;
   }
   return FP_OK;
 }
@@ -510,11 +548,13 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
     FP_GTRY(g, hipGraphLaunch(g->gexec, g->stream));
     g->graph_launches++;
   } else {
-    for (int k = 0; k < substeps; k++)
+    for (int k = 0; k < substeps; k++) {
+      g->tune_cand = (k + g->tune_k) & 3;
       if ((rc = group_substep(g))) return rc;
+    }
     if (g->tune_record) {
       g->tune_record = false;
-      g->tune_pending = g->clanes == kTuneLanes[g->tune_k] ? g->tune_k : -1;
+      g->tune_pending = true;
     }
   }
   for (fp_solver* s : g->scenes) s->substeps_done += substeps;
