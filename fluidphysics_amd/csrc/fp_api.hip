@@ -400,6 +400,8 @@ int fp_create(const fp_params* params, int32_t max_fluid, int32_t device, fp_sol
   return FP_OK;
 }
 
+static void upload_free(fp_solver* s);
+
 int fp_destroy(fp_solver* s) {
   if (!s) return FP_OK;
   (void)hipSetDevice(s->device);
@@ -410,6 +412,7 @@ int fp_destroy(fp_solver* s) {
     delete s->group;
   }
   free_all(s);
+  upload_free(s);
   prof_free(&s->prof);
   if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
   if (s->tear_host) (void)hipHostFree(s->tear_host);
@@ -580,6 +583,44 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
   return FP_OK;
 }
 
+// A pinned host buffer of at least `bytes` for an upload on the solver's
+// stream; upload_end() once the copies that read it are queued.
+static int upload_begin(fp_solver* s, size_t bytes, char** host) {
+  fp_solver::Upload& u = s->upload[s->upload_next];
+  if (u.pending) {
+    FP_TRY(s, hipEventSynchronize(u.ev));
+    u.pending = false;
+  }
+  if (!u.ev) FP_TRY(s, hipEventCreateWithFlags(&u.ev, hipEventDisableTiming));
+  if (bytes > u.cap) {
+    if (u.host) (void)hipHostFree(u.host);
+    u.host = nullptr;
+    u.cap = 0;
+    const size_t cap = std::max<size_t>(bytes, (size_t)64 << 10);
+    FP_TRY(s, hipHostMalloc(&u.host, cap, 0));
+    u.cap = cap;
+  }
+  *host = (char*)u.host;
+  return FP_OK;
+}
+
+static int upload_end(fp_solver* s) {
+  fp_solver::Upload& u = s->upload[s->upload_next];
+  FP_TRY(s, hipEventRecord(u.ev, s->stream));
+  u.pending = true;
+  s->upload_next ^= 1;
+  return FP_OK;
+}
+
+static void upload_free(fp_solver* s) {
+  for (fp_solver::Upload& u : s->upload) {
+    if (u.ev) (void)hipEventSynchronize(u.ev);
+    if (u.host) (void)hipHostFree(u.host);
+    if (u.ev) (void)hipEventDestroy(u.ev);
+    u = fp_solver::Upload();
+  }
+}
+
 int fp_emit(fp_solver* s, const double* pos, const double* vel, const double* mass, int32_t n) {
   if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_emit: NULL solver");
   if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_emit before fp_add_solid_cube");
@@ -601,8 +642,13 @@ int fp_emit(fp_solver* s, const double* pos, const double* vel, const double* ma
       return rc;
     s->emit_cap = cap;
   }
-  std::vector<int> idx(m);
-  std::vector<float4> hp(m), hv(m);
+  // staged in pinned memory: the copies run behind the host, which goes on
+  // to queue the next step
+  char* h = nullptr;
+  if ((rc = upload_begin(s, (size_t)m * (2 * sizeof(float4) + sizeof(int)), &h))) return rc;
+  float4* hp = (float4*)h;
+  float4* hv = hp + m;
+  int* idx = (int*)(hv + m);
   int next = (int)(((int64_t)s->fluid_next + first) % s->max_fluid);
   for (int k = 0; k < m; k++) {
     const int j = first + k;
@@ -613,13 +659,13 @@ int fp_emit(fp_solver* s, const double* pos, const double* vel, const double* ma
     next = (next + 1) % s->max_fluid;
   }
   s->fluid_next = (int)(((int64_t)s->fluid_next + n) % s->max_fluid);
-  FP_TRY(s, hipMemcpyAsync(s->emit_idx, idx.data(), m * sizeof(int), hipMemcpyHostToDevice, s->stream));
-  FP_TRY(s, hipMemcpyAsync(s->emit_p, hp.data(), m * sizeof(float4), hipMemcpyHostToDevice, s->stream));
-  FP_TRY(s, hipMemcpyAsync(s->emit_v, hv.data(), m * sizeof(float4), hipMemcpyHostToDevice, s->stream));
-  if ((rc = fp_launch_emit(s, m))) return rc;
-  // the host vectors and the staging buffers are reused: wait for the copies
-  FP_TRY(s, hipStreamSynchronize(s->stream));
-  return FP_OK;
+  FP_TRY(s, hipMemcpyAsync(s->emit_idx, idx, m * sizeof(int), hipMemcpyHostToDevice, s->stream));
+  FP_TRY(s, hipMemcpyAsync(s->emit_p, hp, m * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+  FP_TRY(s, hipMemcpyAsync(s->emit_v, hv, m * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+  if ((rc = upload_end(s))) return rc;
+  // the device staging is reused in stream order: the next emit's copies
+  // queue behind this launch
+  return fp_launch_emit(s, m);
 }
 
 int fp_reset_solid(fp_solver* s) {
@@ -727,10 +773,13 @@ int fp_set_grabs(fp_solver* s, const int32_t* idx, const double* xyz, int32_t n)
     if ((rc = dalloc(s, &s->grab_idx, cap)) || (rc = dalloc(s, &s->grab_tgt, cap))) return rc;
     s->grab_cap = cap;
   }
-  FP_TRY(s, hipMemcpyAsync(s->grab_idx, hidx.data(), n * sizeof(int), hipMemcpyHostToDevice, s->stream));
-  FP_TRY(s, hipMemcpyAsync(s->grab_tgt, htgt.data(), n * sizeof(float4), hipMemcpyHostToDevice, s->stream));
-  FP_TRY(s, hipStreamSynchronize(s->stream));
-  return FP_OK;
+  char* h = nullptr;
+  if ((rc = upload_begin(s, (size_t)n * (sizeof(float4) + sizeof(int)), &h))) return rc;
+  std::memcpy(h, htgt.data(), n * sizeof(float4));
+  std::memcpy(h + n * sizeof(float4), hidx.data(), n * sizeof(int));
+  FP_TRY(s, hipMemcpyAsync(s->grab_tgt, h, n * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+  FP_TRY(s, hipMemcpyAsync(s->grab_idx, h + n * sizeof(float4), n * sizeof(int), hipMemcpyHostToDevice, s->stream));
+  return upload_end(s);
 }
 
 int fp_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[3], int32_t* ncut) {

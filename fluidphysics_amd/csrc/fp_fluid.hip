@@ -744,6 +744,33 @@ __device__ __forceinline__ void density_body(const SceneArgs& A, int lb) {
   density_store(fp, pl, me, g, A.sP, A.rho, A.dflags);
 }
 
+// Work placement of the fluid passes.  A scene's W_FLUID items are one run
+// of the work list, item j at list position o + j, and list position b runs
+// on XCD b % 8 (one workgroup per item).  The scene's first `a` items hold
+// its fluid (the fluid order's occupied slots; the rest is idle capacity).
+// Each XCD gets one contiguous run of those a items: the neighbour rows a
+// particle gathers (the next z and y rows are one row and one layer of the
+// fluid order away) are then mostly in the L2 of the XCD that reads them,
+// and every XCD still holds an eighth of each scene's fluid.  A bijection on
+// [0, a) for every o; items past a keep their place.
+__device__ __forceinline__ int fluid_block(int j, int o, int a) {
+  if (j >= a) return j;
+  const int c = o & 7, q = a >> 3, r = a & 7;
+  const int x = (j + c) & 7;  // the XCD of item j's workgroup
+  int first = x * q;          // items of the XCDs before x
+  for (int y = 0; y < x; y++) first += ((y - c) & 7) < r ? 1 : 0;
+  return first + ((j - ((x - c) & 7)) >> 3);
+}
+
+#define FLUID_ITEMS(S, W, nw)                                                                        \
+  for (int w_ = (int)blockIdx.x; w_ < (nw); w_ += (int)gridDim.x)                                    \
+    if (const int2 it_ = (W)[w_]; true)                                                              \
+      if (const SceneArgs& A = (S)[it_.x]; true)                                                     \
+        if (const int lb = fluid_block(it_.y, w_ - it_.y,                                            \
+                                       (int)((((int64_t)gld(A.src.lstart + A.fp.cg.M)) << A.fp.fl_log2) + \
+                                             FLUID_BLOCK - 1) / FLUID_BLOCK);                        \
+            true)
+
 // Minimum waves per SIMD asked of the register allocator for the three
 // gather passes (0: the allocator's choice).  These loops are bound by
 // memory latency and issue, so resident waves matter more than registers.
@@ -768,7 +795,7 @@ __device__ __forceinline__ void density_body(const SceneArgs& A, int lb) {
 
 __global__ void __launch_bounds__(FLUID_BLOCK) DENSITY_ATTR kg_density(const SceneArgs* __restrict__ S,
                                                           const int2* __restrict__ W, int nw) {
-  FOR_ITEMS(S, W, nw) density_body(A, lb);
+  FLUID_ITEMS(S, W, nw) density_body(A, lb);
 }
 
 struct DisplaceAcc {
@@ -827,7 +854,7 @@ __device__ __forceinline__ void displace_body(const SceneArgs& A, int lb) {
 
 __global__ void __launch_bounds__(FLUID_BLOCK) WAVES_ATTR(DISPLACE_WAVES) kg_displace(const SceneArgs* __restrict__ S,
                                                            const int2* __restrict__ W, int nw) {
-  FOR_ITEMS(S, W, nw) displace_body(A, lb);
+  FLUID_ITEMS(S, W, nw) displace_body(A, lb);
 }
 
 struct XsphAcc {
@@ -880,7 +907,7 @@ __device__ __forceinline__ void xsph_body(const SceneArgs& A, int lb) {
 
 __global__ void __launch_bounds__(FLUID_BLOCK) WAVES_ATTR(XSPH_WAVES) kg_xsph(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
                                                        int nw) {
-  FOR_ITEMS(S, W, nw) xsph_body(A, lb);
+  FLUID_ITEMS(S, W, nw) xsph_body(A, lb);
 }
 
 // ---- launchers ------------------------------------------------------------------

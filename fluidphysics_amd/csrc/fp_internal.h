@@ -171,6 +171,17 @@ struct fp_solver {
   int* emit_idx = nullptr;
   float4 *emit_p = nullptr, *emit_v = nullptr;
   int emit_cap = 0;
+  // pinned host staging of the small uploads (emits, grab lists): two
+  // buffers used in turn, each with an event recorded after the copies that
+  // read it, so an upload waits only for the copies of the one before last
+  struct Upload {
+    void* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+  };
+  Upload upload[2];
+  int upload_next = 0;
   std::vector<int> h_cons_a, h_cons_b;
   std::vector<int2> h_cent;  // CSR entries of each constraint (host copy of cent)
   std::vector<float> h_snap_pos, h_snap_rest;
@@ -421,9 +432,27 @@ static inline int fp_blocks(int n) { return (n + BLOCK - 1) / BLOCK; }
   } while (0)
 
 // The (scene, local block) items of a group kernel: for (w ...) { SceneArgs& A; int lb }
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup b to XCD
+// b % 8), each XCD with its own L2.  With one workgroup per item, the work
+// list is dealt to the XCDs in runs of FP_XCD_CHUNK consecutive items, so
+// items next to each other (neighbouring particles, whose gathers read the
+// same rows) share an L2 while every XCD still gets a share of each part of
+// the list (idle capacity at a fluid order's end, scenes of unequal size).
+// The items past the last whole round of 8 runs keep their own workgroup.
+#ifndef FP_XCD_CHUNK
+#define FP_XCD_CHUNK 16
+#endif
+#define FP_XCDS 8
+__device__ __forceinline__ int xcd_item(int b, int n) {
+  constexpr int K = FP_XCD_CHUNK, R = FP_XCDS * FP_XCD_CHUNK;
+  if (K <= 1 || (int)gridDim.x != n || b >= n / R * R) return b;
+  const int x = b % FP_XCDS, l = b / FP_XCDS;  // XCD, and the workgroup's place in its turn
+  return (l / K * FP_XCDS + x) * K + l % K;
+}
+
 #define FOR_ITEMS(S, W, nw)                                                         \
   for (int w_ = (int)blockIdx.x; w_ < (nw); w_ += (int)gridDim.x)                   \
-    if (const int2 it_ = (W)[w_]; true)                                             \
+    if (const int2 it_ = (W)[xcd_item(w_, (nw))]; true)                             \
       if (const SceneArgs& A = (S)[it_.x]; true)                                    \
         if (const int lb = it_.y; true)
 
