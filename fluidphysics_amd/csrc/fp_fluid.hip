@@ -732,8 +732,6 @@ __device__ __forceinline__ void density_body(const SceneArgs& A, int lb) {
   const FluidParams& fp = A.fp;
   const PairSource& src = A.src;
   const PairList& pl = A.pl;
-  // the solids' bounding box is rebuilt by the next integrate
-  if (lb == 0 && threadIdx.x < 6) A.sbox[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : INT_MIN;
   PairSelf me;
   if (!pair_self(fp, src, fluid_lane(lb), me)) return;
   DensityGather g;
@@ -960,28 +958,28 @@ int fp_scene_blocks(const fp_solver* s, int kind, int contact_lanes) {
   return 0;
 }
 
-static void launch_contacts_solid(fp_group* g, int lanes, const int2* items, int n) {
+static void launch_contacts_solid(fp_group* g, hipStream_t st, int lanes, const int2* items, int n) {
   if (n <= 0) return;
   switch (lanes) {
     case 32:
-      hipLaunchKernelGGL(kg_contacts_solid<32>, dim3(n), dim3(BLOCK), 0, g->stream, g->d_args, items, n);
+      hipLaunchKernelGGL(kg_contacts_solid<32>, dim3(n), dim3(BLOCK), 0, st, g->d_args, items, n);
       break;
     case 16:
-      hipLaunchKernelGGL(kg_contacts_solid<16>, dim3(n), dim3(BLOCK), 0, g->stream, g->d_args, items, n);
+      hipLaunchKernelGGL(kg_contacts_solid<16>, dim3(n), dim3(BLOCK), 0, st, g->d_args, items, n);
       break;
     case 8:
-      hipLaunchKernelGGL(kg_contacts_solid<8>, dim3(n), dim3(BLOCK), 0, g->stream, g->d_args, items, n);
+      hipLaunchKernelGGL(kg_contacts_solid<8>, dim3(n), dim3(BLOCK), 0, st, g->d_args, items, n);
       break;
     default:
-      hipLaunchKernelGGL(kg_contacts_solid<4>, dim3(n), dim3(BLOCK), 0, g->stream, g->d_args, items, n);
+      hipLaunchKernelGGL(kg_contacts_solid<4>, dim3(n), dim3(BLOCK), 0, st, g->d_args, items, n);
   }
 }
 
-int fp_group_contacts(fp_group* g) {
-  PassScope ps(g->prof, g->stream, P_CONTACTS);
+// Solid contacts on stream st (the group's stream, or the side stream of a
+// captured frame).  A tuning frame launches this substep's candidate from its
+// own work list, between events (tuning frames are never captured).
+int fp_group_contacts_solid(fp_group* g, hipStream_t st) {
   if (g->tune_record) {
-    // a tuning frame: this substep's candidate, from its own work list,
-    // between events
     const int c = g->tune_cand;
     const size_t need = 2 * (size_t)(g->tune_used + 1);
     while (g->tune_ev.size() < need) {
@@ -991,15 +989,28 @@ int fp_group_contacts(fp_group* g) {
     }
     hipEvent_t* ev = &g->tune_ev[2 * (size_t)g->tune_used++];
     g->tune_of.push_back(c);
-    FP_GTRY(g, hipEventRecord(ev[0], g->stream));
-    launch_contacts_solid(g, kTuneLanes[c], g->d_tune_items + g->tune_off[c], g->tune_n[c]);
-    FP_GTRY(g, hipEventRecord(ev[1], g->stream));
+    FP_GTRY(g, hipEventRecord(ev[0], st));
+    launch_contacts_solid(g, st, kTuneLanes[c], g->d_tune_items + g->tune_off[c], g->tune_n[c]);
+    FP_GTRY(g, hipEventRecord(ev[1], st));
   } else {
-    launch_contacts_solid(g, g->clanes, g->d_items[W_SOLID], g->n_items[W_SOLID]);
+    launch_contacts_solid(g, st, g->clanes, g->d_items[W_SOLID], g->n_items[W_SOLID]);
   }
+  FP_GTRY(g, hipGetLastError());
+  return FP_OK;
+}
+
+int fp_group_contacts_fluid(fp_group* g) {
   GROUP_LAUNCH(g, kg_contacts_fluid, W_PARTICLE, BLOCK);
   FP_GTRY(g, hipGetLastError());
   return FP_OK;
+}
+
+// Both contact passes, one after the other (timed as one pass).
+int fp_group_contacts(fp_group* g) {
+  PassScope ps(g->prof, g->stream, P_CONTACTS);
+  int rc = fp_group_contacts_solid(g, g->stream);
+  if (rc) return rc;
+  return fp_group_contacts_fluid(g);
 }
 
 int fp_group_density(fp_group* g) {

@@ -157,9 +157,10 @@ __global__ void __launch_bounds__(BLOCK) kg_scan_reduce(const SceneArgs* __restr
   FOR_ITEMS(S, W, nw) scan_reduce_body((int)A.fp.cg.M, A.lcount, A.lblock_sums, lb);
 }
 
+// fluid: the fluid order's scan (starts in lstart_f), else the grid's (lstart)
 __global__ void __launch_bounds__(BLOCK) kg_scan_apply(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
-                                                       int nw) {
-  FOR_ITEMS(S, W, nw) scan_apply_body((int)A.fp.cg.M, A.lcount, A.lblock_sums, A.lstart, lb);
+                                                       int nw, int fluid) {
+  FOR_ITEMS(S, W, nw) scan_apply_body((int)A.fp.cg.M, A.lcount, A.lblock_sums, fluid ? A.lstart_f : A.lstart, lb);
 }
 
 // ---- cell grid -----------------------------------------------------------------
@@ -244,7 +245,7 @@ int fp_group_cell_grid(fp_group* g, bool count_done) {
   }
   {
     PassScope ps(g->prof, g->stream, P_SCAN_APPLY);
-    GROUP_LAUNCH(g, kg_scan_apply, W_SCAN, BLOCK);
+    GROUP_LAUNCH(g, kg_scan_apply, W_SCAN, BLOCK, 0);
   }
   {
     PassScope ps(g->prof, g->stream, P_SCATTER);
@@ -273,7 +274,7 @@ __device__ __forceinline__ void fluid_scatter_body(const SceneArgs& A, int lb) {
   const uint32_t k = gld(A.tkey + e), sl = gld(A.tslot + e);
   const int x = gld(A.sid + e);
   if (k == NO_KEY) return;
-  gst(A.ltmp + gld(A.lstart + k) + (int)sl, x);
+  gst(A.ltmp + gld(A.lstart_f + k) + (int)sl, x);
 }
 
 __global__ void __launch_bounds__(BLOCK) kg_fluid_scatter(const SceneArgs* __restrict__ S,
@@ -289,7 +290,7 @@ __device__ __forceinline__ void fluid_rank_body(const SceneArgs& A, int lb) {
   const int i = gld(A.sid + e);
   const float4 tp = gld(A.tpos + e);
   if (k == NO_KEY) return;
-  const int s = gld(A.lstart + k), t = gld(A.lstart + k + 1);
+  const int s = gld(A.lstart_f + k), t = gld(A.lstart_f + k + 1);
   int r = 0;
   for (int q = s; q < t; q++) r += gld(A.ltmp + q) < i ? 1 : 0;
   const int d = s + r;
@@ -307,7 +308,7 @@ __global__ void __launch_bounds__(BLOCK) kg_fluid_rank(const SceneArgs* __restri
 int fp_group_resort(fp_group* g) {
   PassScope ps(g->prof, g->stream, P_RESORT);
   GROUP_LAUNCH(g, kg_scan_reduce, W_SCAN, BLOCK);
-  GROUP_LAUNCH(g, kg_scan_apply, W_SCAN, BLOCK);
+  GROUP_LAUNCH(g, kg_scan_apply, W_SCAN, BLOCK, 1);
   GROUP_LAUNCH(g, kg_fluid_scatter, W_PARTICLE, BLOCK);
   GROUP_LAUNCH(g, kg_fluid_rank, W_PARTICLE, BLOCK);
   FP_GTRY(g, hipGetLastError());

@@ -131,6 +131,7 @@ struct fp_solver {
   CellGrid cg;
   int lscan_blocks = 0;
   int *lcount = nullptr, *lstart = nullptr, *lblock_sums = nullptr, *ltmp = nullptr;
+  int* lstart_f = nullptr;  // cell starts of the fluid order (lstart: of the grid order)
   int* lnsol = nullptr;  // solids per non-empty cell of the grid order
   uint32_t *lkey = nullptr, *lslot = nullptr;
   int* sid = nullptr;
@@ -278,6 +279,7 @@ struct SceneArgs {
   float2* rho;
   uint32_t* phash;
   int *lcount, *lstart, *lblock_sums, *ltmp, *lnsol;
+  int* lstart_f;  // the fluid order's cell starts
   uint32_t *lkey, *lslot;
   int* sid;
   float4* spos;
@@ -339,6 +341,14 @@ struct fp_group {
   int graph_iters = -1;
   bool use_graph = true;
   int64_t graph_launches = 0;
+  // A captured frame runs each substep's solid chain (solid contacts, the
+  // constraint iterations) on a second stream, beside the fluid chain (fluid
+  // contacts, the fluid order, density, displacement); they share no buffer
+  // they write, and join before the bounds pass.  Eager frames (tuning,
+  // grabs, per-pass timing) run the two chains one after the other.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool capturing = false;
   // autotuned launch shape: lanes per solid of the solid contact pass (the
   // result does not depend on it).  The first TUNE_FRAMES frames time 4, 8,
   // 16 and 32 lanes with events, rotating the candidate from substep to
@@ -421,12 +431,13 @@ static inline int fp_blocks(int n) { return (n + BLOCK - 1) / BLOCK; }
 
 // Launch of a group kernel over work kind k: grid = the items (or the
 // persistent cap), the kernel strides over them.
-#define GROUP_LAUNCH(g, kernel, kind, block, ...)                                                   \
+#define GROUP_LAUNCH(g, kernel, kind, block, ...) GROUP_LAUNCH_ON(g, (g)->stream, kernel, kind, block, ##__VA_ARGS__)
+#define GROUP_LAUNCH_ON(g, st, kernel, kind, block, ...)                                            \
   do {                                                                                               \
     const int n_ = (g)->n_items[kind];                                                               \
     if (n_ > 0) {                                                                                    \
       const int grid_ = (g)->grid_cap[kind] > 0 ? std::min(n_, (g)->grid_cap[kind]) : n_;          \
-      hipLaunchKernelGGL(kernel, dim3(grid_), dim3(block), 0, (g)->stream, (g)->d_args, (g)->d_items[kind], n_, \
+      hipLaunchKernelGGL(kernel, dim3(grid_), dim3(block), 0, (st), (g)->d_args, (g)->d_items[kind], n_, \
                          ##__VA_ARGS__);                                                             \
     }                                                                                                \
   } while (0)
@@ -469,11 +480,13 @@ int fp_group_cell_grid(fp_group* g, bool count_done);
 int fp_group_resort(fp_group* g);
 // fp_fluid.hip
 int fp_group_contacts(fp_group* g);
+int fp_group_contacts_solid(fp_group* g, hipStream_t st);
+int fp_group_contacts_fluid(fp_group* g);
 int fp_group_density(fp_group* g);
 int fp_group_displace(fp_group* g);
 int fp_group_xsph(fp_group* g);
 // fp_solid.hip
-int fp_group_constraints(fp_group* g, int iters);
+int fp_group_constraints(fp_group* g, int iters, hipStream_t st);
 // fp_step.hip
 int fp_group_frame(fp_group* g, double dt, int substeps);
 int fp_group_prepare(fp_group* g, double h);

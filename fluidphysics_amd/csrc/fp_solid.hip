@@ -196,10 +196,10 @@ int fp_launch_select(fp_solver* s, const double c[3], double r2, int g, int* ids
 
 // iters Jacobi iterations for every scene of the group (the solids' final
 // positions end in spos_c[(iters - 1) & 1]; see fp_group_frame).
-int fp_group_constraints(fp_group* g, int iters) {
+int fp_group_constraints(fp_group* g, int iters, hipStream_t st) {
   if (iters < 1) return FP_OK;
-  PassScope ps(g->prof, g->stream, P_CONSTRAINTS);
-  for (int it = 0; it < iters; it++) GROUP_LAUNCH(g, kg_constraints, W_CONS, BLOCK, it);
+  PassScope ps(g->prof, st, P_CONSTRAINTS);
+  for (int it = 0; it < iters; it++) GROUP_LAUNCH_ON(g, st, kg_constraints, W_CONS, BLOCK, it);
   FP_GTRY(g, hipGetLastError());
   return FP_OK;
 }

@@ -102,6 +102,9 @@ __device__ __forceinline__ void bounds_velocity_body(const SceneArgs& A, int lb)
   const BoundsParams& bp = A.bp;
   const float4* __restrict__ solid_src = A.cp.iters > 0 ? A.spos_c[(A.cp.iters - 1) & 1] : A.pos;
   float4* __restrict__ pos = A.pos;
+  // the solids' bounding box is rebuilt by the next integrate; both contact
+  // passes, its last readers, are done
+  if (lb == 0 && threadIdx.x < 6) gst(A.sbox + threadIdx.x, threadIdx.x < 3 ? INT_MAX : INT_MIN);
   const int i = lb * BLOCK + threadIdx.x;
   if (i >= bp.n) return;
   const bool solid = i < bp.ns;
@@ -219,7 +222,7 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
   a.fp = fp_fluid_params(s);
   a.src.spos2 = s->spos2;
   a.src.sphash = s->sphash2;
-  a.src.lstart = s->lstart;
+  a.src.lstart = s->lstart_f;
   a.pl.nbr = s->nbr;
   a.pl.ncount = s->ncount;
   a.pl.cap = s->pair_slots / s->fluid_lanes;
@@ -247,6 +250,7 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
   a.phash = s->phash;
   a.lcount = s->lcount;
   a.lstart = s->lstart;
+  a.lstart_f = s->lstart_f;
   a.lblock_sums = s->lblock_sums;
   a.ltmp = s->ltmp;
   a.lnsol = s->lnsol;
@@ -280,6 +284,12 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
 void fp_group_free(fp_group* g) {
   for (hipEvent_t e : g->tune_ev) (void)hipEventDestroy(e);
   g->tune_ev.clear();
+  if (g->side) (void)hipStreamSynchronize(g->side);
+  if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
+  if (g->ev_join) (void)hipEventDestroy(g->ev_join);
+  if (g->side) (void)hipStreamDestroy(g->side);
+  g->side = nullptr;
+  g->ev_fork = g->ev_join = nullptr;
   if (g->d_tune_items) fp_pool_put(g->device, g->d_tune_items);
   g->d_tune_items = nullptr;
   g->tune_items_cap = 0;
@@ -418,6 +428,11 @@ int fp_group_prepare(fp_group* g, double h) {
   return FP_OK;
 }
 
+// 1: captured frames run the solid chain on the group's side stream.
+#ifndef FP_SIDE_STREAM
+#define FP_SIDE_STREAM 1
+#endif
+
 // One substep of every scene of the group.
 static int group_substep(fp_group* g) {
   int rc;
@@ -440,11 +455,26 @@ static int group_substep(fp_group* g) {
     }
   }
   if ((rc = fp_group_cell_grid(g, !g->any_grabs))) return rc;
-  if ((rc = fp_group_contacts(g))) return rc;
-  if ((rc = fp_group_resort(g))) return rc;
-  if ((rc = fp_group_density(g))) return rc;
-  if ((rc = fp_group_displace(g))) return rc;
-  if ((rc = fp_group_constraints(g, g->iters))) return rc;
+  if (g->capturing && g->side) {
+    // the solid chain on the side stream, the fluid chain here; they join
+    // before the bounds pass
+    FP_GTRY(g, hipEventRecord(g->ev_fork, g->stream));
+    FP_GTRY(g, hipStreamWaitEvent(g->side, g->ev_fork, 0));
+    if ((rc = fp_group_contacts_solid(g, g->side))) return rc;
+    if ((rc = fp_group_constraints(g, g->iters, g->side))) return rc;
+    FP_GTRY(g, hipEventRecord(g->ev_join, g->side));
+    if ((rc = fp_group_contacts_fluid(g))) return rc;
+    if ((rc = fp_group_resort(g))) return rc;
+    if ((rc = fp_group_density(g))) return rc;
+    if ((rc = fp_group_displace(g))) return rc;
+    FP_GTRY(g, hipStreamWaitEvent(g->stream, g->ev_join, 0));
+  } else {
+    if ((rc = fp_group_contacts(g))) return rc;
+    if ((rc = fp_group_resort(g))) return rc;
+    if ((rc = fp_group_density(g))) return rc;
+    if ((rc = fp_group_displace(g))) return rc;
+    if ((rc = fp_group_constraints(g, g->iters, g->stream))) return rc;
+  }
   {
     PassScope ps(g->prof, g->stream, P_BOUNDS_VELOCITY);
     GROUP_LAUNCH(g, kg_bounds_velocity, W_PARTICLE, BLOCK);
@@ -530,9 +560,16 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
   if (graphable) {
     if (!g->gexec || g->graph_sub != substeps || g->graph_iters != g->iters) {
       drop_graph(g);
+      if (FP_SIDE_STREAM && !g->side) {
+        FP_GTRY(g, hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking));
+        FP_GTRY(g, hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
+        FP_GTRY(g, hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming));
+      }
       FP_GTRY(g, hipStreamBeginCapture(g->stream, hipStreamCaptureModeRelaxed));
       int crc = FP_OK;
+      g->capturing = true;
       for (int k = 0; k < substeps && crc == FP_OK; k++) crc = group_substep(g);
+      g->capturing = false;
       hipGraph_t graph = nullptr;
       const hipError_t e = hipStreamEndCapture(g->stream, &graph);
       if (crc) {
