@@ -292,6 +292,41 @@ __device__ __forceinline__ void box_update(bool has, float px, float py, float p
   }
 }
 
+// The same over a block of NT threads: wave min/max, then the waves' results
+// meet in LDS and one thread per box word sends the block's atomic.  Every
+// thread of the block must call it (it synchronizes the block).
+template <int NT>
+__device__ __forceinline__ void box_update_block(bool has, float px, float py, float pz, int* __restrict__ box) {
+  __shared__ float red[NT / 64][6];
+  if (!__syncthreads_or(has ? 1 : 0)) return;
+  float v[6] = {has ? px : INFINITY, has ? py : INFINITY, has ? pz : INFINITY,
+                has ? px : -INFINITY, has ? py : -INFINITY, has ? pz : -INFINITY};
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      const float o = __shfl_xor(v[c], off);
+      v[c] = c < 3 ? fminf(v[c], o) : fmaxf(v[c], o);
+    }
+  }
+  const int wid = (int)threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < 6; c++) red[wid][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int c = (int)threadIdx.x;
+    float m = red[0][c];
+#pragma unroll
+    for (int w = 1; w < NT / 64; w++) m = c < 3 ? fminf(m, red[w][c]) : fmaxf(m, red[w][c]);
+    FP_GLOBAL int* gb = (FP_GLOBAL int*)box;
+    if (c < 3)
+      __hip_atomic_fetch_min(gb + c, order_float(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      __hip_atomic_fetch_max(gb + c, order_float(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // ---- wave64 scan primitives ------------------------------------------------
 
 // Inclusive prefix sum over the 64 lanes of a wave: DPP row_shr:1,2,4,8 with

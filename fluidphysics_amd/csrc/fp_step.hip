@@ -66,7 +66,7 @@ __device__ __forceinline__ void integrate_body(const SceneArgs& A, int lb) {
     // neighbouring indices are often in one cell: one atomic per run
     const uint32_t sl = wave_bucket_slot(A.lcount, k, act);
     if (act) gst(A.lslot + i, sl);
-    box_update(act && i < A.ns, p.x, p.y, p.z, A.sbox);
+    box_update_block<BLOCK>(act && i < A.ns, p.x, p.y, p.z, A.sbox);
   }
 }
 
