@@ -84,7 +84,7 @@ void scene_buffers(fp_solver* s, F& take) {
   const size_t C = (size_t)std::max(s->C, 1);
   take(s->pos, N), take(s->prev, N), take(s->vel, N), take(s->active, N), take(s->rho, N), take(s->phash, N);
   take(s->lcount, M), take(s->lstart, M + 1), take(s->lstart_f, M + 1), take(s->lnsol, M);
-  take(s->lblock_sums, (size_t)s->lscan_blocks);
+  take(s->lblock_sums, (size_t)s->lscan_blocks), take(s->lblock_f, (size_t)s->lscan_blocks);
   take(s->ltmp, N), take(s->lkey, N), take(s->lslot, N), take(s->sid, N), take(s->spos, N), take(s->spos2, N);
   take(s->svtmp, N), take(s->sphash, N), take(s->sP, N), take(s->tpos, N), take(s->tkey, N), take(s->tslot, N);
   take(s->fsid, N), take(s->fslot, N), take(s->sphash2, N), take(s->sbox, 6);

@@ -366,6 +366,9 @@ __device__ __forceinline__ void contacts_fluid_body(const SceneArgs& A, int lb) 
   }
   // consecutive grid slots mostly share a cell: one atomic per run
   const uint32_t slot = wave_bucket_slot(lcount, k1, has);
+  // the fluid order's scan tiles take their sums here (the lanes walk the
+  // grid order, so a wave's keys mostly share one tile)
+  wave_bucket_add(A.lblock_f, has ? k1 / SCAN_TILE : 0u, has);
   if (has) {
     gst(tkey + e, k1);
     gst(tslot + e, slot);

@@ -160,7 +160,8 @@ __global__ void __launch_bounds__(BLOCK) kg_scan_reduce(const SceneArgs* __restr
 // fluid: the fluid order's scan (starts in lstart_f), else the grid's (lstart)
 __global__ void __launch_bounds__(BLOCK) kg_scan_apply(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
                                                        int nw, int fluid) {
-  FOR_ITEMS(S, W, nw) scan_apply_body((int)A.fp.cg.M, A.lcount, A.lblock_sums, fluid ? A.lstart_f : A.lstart, lb);
+  FOR_ITEMS(S, W, nw)
+  scan_apply_body((int)A.fp.cg.M, A.lcount, fluid ? A.lblock_f : A.lblock_sums, fluid ? A.lstart_f : A.lstart, lb);
 }
 
 // ---- cell grid -----------------------------------------------------------------
@@ -270,6 +271,13 @@ int fp_group_cell_grid(fp_group* g, bool count_done) {
 
 __device__ __forceinline__ void fluid_scatter_body(const SceneArgs& A, int lb) {
   const int e = lb * BLOCK + threadIdx.x;
+  {
+    // the tile sums were read by the scan: clear them for the next contact
+    // pass (the scene's blocks stride over its tiles)
+    const int tiles = (int)((A.fp.cg.M + SCAN_TILE - 1) / SCAN_TILE);
+    const int stride = (A.n + BLOCK - 1) / BLOCK * BLOCK;
+    for (int t = e; t < tiles; t += stride) gst(A.lblock_f + t, 0);
+  }
   if (e >= A.n) return;
   const uint32_t k = gld(A.tkey + e), sl = gld(A.tslot + e);
   const int x = gld(A.sid + e);
@@ -307,7 +315,7 @@ __global__ void __launch_bounds__(BLOCK) kg_fluid_rank(const SceneArgs* __restri
 
 int fp_group_resort(fp_group* g) {
   PassScope ps(g->prof, g->stream, P_RESORT);
-  GROUP_LAUNCH(g, kg_scan_reduce, W_SCAN, BLOCK);
+  // the tile sums come from the contact pass (lblock_f): no reduce pass
   GROUP_LAUNCH(g, kg_scan_apply, W_SCAN, BLOCK, 1);
   GROUP_LAUNCH(g, kg_fluid_scatter, W_PARTICLE, BLOCK);
   GROUP_LAUNCH(g, kg_fluid_rank, W_PARTICLE, BLOCK);

@@ -43,7 +43,9 @@
 #include "fp_kernels.h"
 
 #define BLOCK 256
-#define SCAN_ITEMS 16
+#ifndef SCAN_ITEMS
+#define SCAN_ITEMS 4  // cells per thread of a scan tile
+#endif
 #define SCAN_TILE (BLOCK * SCAN_ITEMS)
 #define PF_MOVED 1u  // fluid particle changed cell in the contact pass
 #define PF_DUP 2u    // its current cell's 27 reference probes repeat a bucket
@@ -132,6 +134,7 @@ struct fp_solver {
   int lscan_blocks = 0;
   int *lcount = nullptr, *lstart = nullptr, *lblock_sums = nullptr, *ltmp = nullptr;
   int* lstart_f = nullptr;  // cell starts of the fluid order (lstart: of the grid order)
+  int* lblock_f = nullptr;  // per scan tile: fluid counted into it by the contact pass
   int* lnsol = nullptr;  // solids per non-empty cell of the grid order
   uint32_t *lkey = nullptr, *lslot = nullptr;
   int* sid = nullptr;
@@ -280,6 +283,7 @@ struct SceneArgs {
   uint32_t* phash;
   int *lcount, *lstart, *lblock_sums, *ltmp, *lnsol;
   int* lstart_f;  // the fluid order's cell starts
+  int* lblock_f;  // the fluid order's tile sums (added by the contact pass)
   uint32_t *lkey, *lslot;
   int* sid;
   float4* spos;

@@ -351,6 +351,22 @@ __device__ __forceinline__ int wave_sum(int v) { return __builtin_amdgcn_readlan
 // one atomic per run of equal keys in consecutive lanes (lanes with
 // has == false take part in the vote and get nothing).  Every lane of the
 // wave must call it.  Slots inside a bucket come out in no fixed order.
+// count[key] += 1 for every lane with has, one atomic per run of equal keys
+// in consecutive lanes (no result).  Every lane of the wave must call it.
+__device__ __forceinline__ void wave_bucket_add(int* __restrict__ count, uint32_t key, bool has) {
+  const int lane = (int)(threadIdx.x & 63);
+  const uint32_t kprev = (uint32_t)__shfl_up((int)key, 1);
+  const bool hprev = __shfl_up(has ? 1 : 0, 1) != 0;
+  const bool head = has && (lane == 0 || !hprev || kprev != key);
+  const uint64_t heads = __ballot(head);
+  const uint64_t stops = heads | ~__ballot(has);
+  const uint64_t above = lane == 63 ? 0ull : (stops & (~0ull << (lane + 1)));
+  const int next = above ? (int)__builtin_ctzll(above) : 64;
+  if (head)
+    (void)__hip_atomic_fetch_add((FP_GLOBAL int*)&count[key], next - lane, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ uint32_t wave_bucket_slot(int* __restrict__ count, uint32_t key, bool has) {
   const int lane = (int)(threadIdx.x & 63);
   const uint32_t kprev = (uint32_t)__shfl_up((int)key, 1);

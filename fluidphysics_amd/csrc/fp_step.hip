@@ -251,6 +251,7 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
   a.lcount = s->lcount;
   a.lstart = s->lstart;
   a.lstart_f = s->lstart_f;
+  a.lblock_f = s->lblock_f;
   a.lblock_sums = s->lblock_sums;
   a.ltmp = s->ltmp;
   a.lnsol = s->lnsol;
