@@ -345,14 +345,21 @@ struct fp_group {
   int graph_iters = -1;
   bool use_graph = true;
   int64_t graph_launches = 0;
-  // A captured frame runs each substep's solid chain (solid contacts, the
-  // constraint iterations) on a second stream, beside the fluid chain (fluid
-  // contacts, the fluid order, density, displacement); they share no buffer
-  // they write, and join before the bounds pass.  Eager frames (tuning,
-  // grabs, per-pass timing) run the two chains one after the other.
+  // A captured frame of a group with at least FP_SIDE_MIN particle slots runs
+  // each substep's solid chain (solid contacts, the constraint iterations) on
+  // a second stream, beside the fluid chain (fluid contacts, the fluid order,
+  // density, displacement); they share no buffer they write, and join before
+  // the bounds pass.  Smaller groups (whose solid chain is too short to hide
+  // anything), and eager frames (tuning, grabs, per-pass timing), run the two
+  // chains one after the other.
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool capturing = false;
+  bool fork = false;  // the frame being captured forks
+  // the members or their parameters changed for this frame: it runs eagerly
+  // (a graph captured now might be replayed once; continuous batching
+  // changes members every few frames)
+  bool members_changed = true;
   // autotuned launch shape: lanes per solid of the solid contact pass (the
   // result does not depend on it).  The first TUNE_FRAMES frames time 4, 8,
   // 16 and 32 lanes with events, rotating the candidate from substep to

@@ -321,11 +321,13 @@ static void drop_graph(fp_group* g) {
 // Scene table and work lists for substep h: rebuilt when the members, their
 // parameters or h changed.  A changed membership also drops the graph.
 int fp_group_prepare(fp_group* g, double h) {
-  bool changed = g->dirty || g->built_h != h || g->versions.size() != g->scenes.size() ||
-                 g->items_clanes != g->clanes || (g->tune_record && !g->tune_items_ok);
-  if (!changed)
+  bool members = g->dirty || g->versions.size() != g->scenes.size();
+  if (!members)
     for (size_t k = 0; k < g->scenes.size(); k++)
-      if (g->versions[k] != g->scenes[k]->version) changed = true;
+      if (g->versions[k] != g->scenes[k]->version) members = true;
+  g->members_changed = members;
+  const bool changed = members || g->built_h != h || g->items_clanes != g->clanes ||
+                       (g->tune_record && !g->tune_items_ok);
   g->any_grabs = false;
   for (fp_solver* s : g->scenes) g->any_grabs |= !s->grab_round_len.empty();
   if (!changed) return FP_OK;
@@ -429,9 +431,18 @@ int fp_group_prepare(fp_group* g, double h) {
   return FP_OK;
 }
 
-// 1: captured frames run the solid chain on the group's side stream.
+// 1: captured frames of groups with FP_SIDE_MIN particle slots or more run
+// the solid chain on the group's side stream.
 #ifndef FP_SIDE_STREAM
 #define FP_SIDE_STREAM 1
+#endif
+#ifndef FP_SIDE_MIN
+#define FP_SIDE_MIN 131072
+#endif
+// 1: the first frame after the members (or their parameters) changed runs
+// eagerly; capture waits for a frame whose members match the last one's.
+#ifndef FP_GRAPH_DEFER
+#define FP_GRAPH_DEFER 1
 #endif
 
 // One substep of every scene of the group.
@@ -456,7 +467,7 @@ static int group_substep(fp_group* g) {
     }
   }
   if ((rc = fp_group_cell_grid(g, !g->any_grabs))) return rc;
-  if (g->capturing && g->side) {
+  if (g->capturing && g->fork) {
     // the solid chain on the side stream, the fluid chain here; they join
     // before the bounds pass
     FP_GTRY(g, hipEventRecord(g->ev_fork, g->stream));
@@ -557,11 +568,17 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
   rc = fp_group_prepare(g, h);
   if (rc) return rc;
   const bool profiling = g->prof && g->prof->on;
-  const bool graphable = g->use_graph && !g->any_grabs && !profiling && !g->tune_record;
+  // a frame whose members just changed and that would have to capture a new
+  // graph runs eagerly instead
+  const bool graphable = g->use_graph && !g->any_grabs && !profiling && !g->tune_record &&
+                         !(FP_GRAPH_DEFER && g->members_changed && !g->gexec);
   if (graphable) {
     if (!g->gexec || g->graph_sub != substeps || g->graph_iters != g->iters) {
       drop_graph(g);
-      if (FP_SIDE_STREAM && !g->side) {
+      int64_t slots = 0;
+      for (fp_solver* s : g->scenes) slots += s->n;
+      g->fork = FP_SIDE_STREAM && slots >= FP_SIDE_MIN;
+      if (g->fork && !g->side) {
         FP_GTRY(g, hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking));
         FP_GTRY(g, hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
         FP_GTRY(g, hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming));
