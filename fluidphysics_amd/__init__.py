@@ -266,6 +266,17 @@ class Solver:
                                                ctypes.byref(n)))
         return ids[: n.value]
 
+    def components(self):
+        """_recomputeGroups (src/solver.js:488-499) on the device over the
+        constraints alive now (fp_components): (labels, count), labels[i] the
+        smallest solid index of solid i's component.  The partition is the
+        reference's; `group` keeps the reference's own root labels."""
+        self._flush()
+        labels = np.zeros(max(self.solidCount, 1), np.int32)
+        n = ctypes.c_int32()
+        self._check(self._lib.fp_components(self._h, _ip(labels), ctypes.byref(n)))
+        return labels[: self.solidCount], n.value
+
     # ---- state arrays (src/solver.js:42-48), fetched on access -----------
     def read_state(self):
         self._flush()

@@ -156,6 +156,7 @@ SIGNATURES = {
     "fp_cut": (_I, [_P, _DP, _DP, _DP, _IP]),
     "fp_pick_solid": (_I, [_P, _DP, _DP, _D, _IP, _DP]),
     "fp_select_solids": (_I, [_P, _DP, _D, _I, _IP, _I, _IP]),
+    "fp_components": (_I, [_P, _IP, _IP]),
     "fp_fluid_count": (_I, [_P, _IP]),
     "fp_read_state": (_I, [_P, _FP, _FP, _FP, _FP, _U8P, _IP]),
     "fp_write_state": (_I, [_P, _FP, _FP, _FP, _FP, _U8P]),

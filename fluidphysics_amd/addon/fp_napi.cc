@@ -369,6 +369,28 @@ napi_value SelectSolids(napi_env env, napi_callback_info info) {
   return arr;
 }
 
+// components(h) -> {labels: Int32Array (smallest solid index of each
+// solid's component), count}
+napi_value Components(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_solver* s = handle(env, a.v[0]);
+  fp_info inf;
+  FP_CALL(env, s, fp_get_info(s, &inf), "fp_get_info");
+  const size_t ns = inf.solid_count > 0 ? (size_t)inf.solid_count : 0;
+  void* data = nullptr;
+  napi_value ab, arr, out, cnt;
+  NAPI_CALL(env, napi_create_arraybuffer(env, ns * 4, &data, &ab));
+  int32_t n = 0;
+  FP_CALL(env, s, fp_components(s, ns ? (int32_t*)data : nullptr, &n), "fp_components");
+  NAPI_CALL(env, napi_create_typedarray(env, napi_int32_array, ns, ab, 0, &arr));
+  NAPI_CALL(env, napi_create_object(env, &out));
+  NAPI_CALL(env, napi_create_int32(env, n, &cnt));
+  NAPI_CALL(env, napi_set_named_property(env, out, "labels", arr));
+  NAPI_CALL(env, napi_set_named_property(env, out, "count", cnt));
+  return out;
+}
+
 napi_value FluidCount(napi_env env, napi_callback_info info) {
   Args a;
   NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
@@ -658,7 +680,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"addSolidCube", AddSolidCube}, {"emit", Emit},           {"step", Step},
       {"sync", Sync},               {"setGrabs", SetGrabs},     {"cut", Cut},
       {"pickSolid", PickSolid},     {"fluidCount", FluidCount}, {"resetSolid", ResetSolid},
-      {"selectSolids", SelectSolids},
+      {"selectSolids", SelectSolids}, {"components", Components},
       {"clearFluid", ClearFluid},   {"readState", ReadState},   {"writeState", WriteState},
       {"readConstraints", ReadConstraints}, {"readGrid", ReadGrid}, {"buildGrid", BuildGrid},
       {"readDensity", ReadDensity},

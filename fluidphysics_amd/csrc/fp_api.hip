@@ -262,6 +262,7 @@ int materialize_groups(fp_solver* s) {
   if (rc) return rc;
   recompute_groups_host(s, al);
   s->groups_pending = false;
+  s->alive_changed = false;
   return FP_OK;
 }
 
@@ -283,6 +284,7 @@ int settle(fp_solver* s) {
   if (s->groups_dirty) {
     s->groups_pending = true;
     s->groups_dirty = false;
+    s->alive_changed = false;  // the due labels are the current flags' partition
   }
   return FP_OK;
 }
@@ -804,7 +806,10 @@ int fp_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[
   if (ncut) *ncut = cnt;
   // the reference marks components dirty; they are relabelled at the end
   // of the next step (solver.js:238-241, :386)
-  if (cnt) s->cut_dirty = true;
+  if (cnt) {
+    s->cut_dirty = true;
+    s->alive_changed = true;
+  }
   return FP_OK;
 }
 
@@ -826,6 +831,21 @@ int fp_pick_solid(fp_solver* s, const double o[3], const double d[3], double max
   return FP_OK;
 }
 
+int fp_components(fp_solver* s, int32_t* labels, int32_t* count) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_components: NULL solver");
+  if (count) *count = 0;
+  if (!s->has_cube || s->ns == 0) return FP_OK;
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  if ((rc = fp_launch_components(s))) return rc;
+  int nc = 0;
+  FP_TRY(s, d2h(s, &nc, s->dflags + DF_NCOMP, sizeof(int)));
+  if (count) *count = nc;
+  if (labels) FP_TRY(s, d2h(s, labels, s->dgroup, (size_t)s->ns * sizeof(int32_t)));
+  return FP_OK;
+}
+
 int fp_select_solids(fp_solver* s, const double center[3], double radius, int32_t same_group_as, int32_t* ids,
                      int32_t max_ids, int32_t* n) {
   if (!s || !center || !n || max_ids < 0 || (max_ids > 0 && !ids))
@@ -836,11 +856,17 @@ int fp_select_solids(fp_solver* s, const double center[3], double radius, int32_
   int rc = set_device(s);
   if (rc) return rc;
   if ((rc = settle(s))) return rc;
-  int g = -1;
   if (same_group_as >= 0) {
-    if ((rc = materialize_groups(s))) return rc;
-    FP_TRY(s, h2d(s, s->dgroup, s->h_group.data(), s->ns * sizeof(int32_t)));
-    g = s->h_group[same_group_as];
+    if (!s->alive_changed) {
+      // the labels the caller would read are the partition of the current
+      // alive flags: the device computes it, nothing crosses to the host
+      if ((rc = fp_launch_components(s))) return rc;
+    } else {
+      // a cut or a write since the labels were computed: the caller still
+      // sees the labels of before it (solver.js relabels at a step's end)
+      if ((rc = materialize_groups(s))) return rc;
+      FP_TRY(s, h2d(s, s->dgroup, s->h_group.data(), s->ns * sizeof(int32_t)));
+    }
   }
   // the ids and the count land in the solver's emit staging (grown if needed)
   const int cap = std::max(max_ids, 1) + 1;
@@ -853,7 +879,8 @@ int fp_select_solids(fp_solver* s, const double center[3], double radius, int32_
       return rc;
     s->emit_cap = cap;
   }
-  if ((rc = fp_launch_select(s, center, radius * radius, g, s->emit_idx + 1, max_ids, s->emit_idx))) return rc;
+  if ((rc = fp_launch_select(s, center, radius * radius, same_group_as, s->emit_idx + 1, max_ids, s->emit_idx)))
+    return rc;
   int total = 0;
   FP_TRY(s, d2h(s, &total, s->emit_idx, sizeof(int)));
   *n = total;
@@ -962,6 +989,7 @@ int fp_write_constraints(fp_solver* s, const float* rest, const uint8_t* alive) 
   if (rc) return rc;
   if ((rc = settle(s))) return rc;
   if ((rc = materialize_groups(s))) return rc;
+  if (alive) s->alive_changed = true;
   return constraints_to_device(s, rest, alive);
 }
 
@@ -1386,6 +1414,7 @@ int fp_checkpoint_restore(fp_solver* s, int32_t slot) {
   s->groups_pending = cp.groups_pending;
   s->groups_dirty = false;
   s->cut_dirty = cp.cut_dirty;
+  s->alive_changed = true;  // conservative: the host labels decide selection
   return FP_OK;
 }
 
@@ -1458,5 +1487,6 @@ int fp_state_import(fp_solver* s, const void* buf, int64_t bytes) {
   s->groups_pending = (h.flags & 1u) != 0;
   s->groups_dirty = false;
   s->cut_dirty = (h.flags & 2u) != 0;
+  s->alive_changed = true;  // conservative: the host labels decide selection
   return FP_OK;
 }

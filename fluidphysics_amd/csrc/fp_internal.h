@@ -65,6 +65,7 @@
 #define DF_CUT 1         // knife count
 #define DF_SCRATCH 2     // fluid count
 #define DF_PAIR_OVF 3    // particle-substeps whose pair list overflowed
+#define DF_NCOMP 4       // components counted by fp_launch_components
 #define DF_WORDS 8
 
 enum PassId {
@@ -195,6 +196,10 @@ struct fp_solver {
   bool steps_pending = false;
   bool cut_dirty = false;
   bool cut_dirty_at_step = false;
+  // alive flags changed (a cut, a constraint write, a restore) since the
+  // group labels were last computed or marked due: until then the labels are
+  // not the partition of the current flags
+  bool alive_changed = false;
   Prof prof;
   hipStream_t own_stream = nullptr;  // the stream fp_create made (stream may be a batch's)
   // "a constraint tore" flag in pinned host memory, set by the constraint
@@ -512,6 +517,7 @@ int fp_launch_ref_grid(fp_solver* s, bool rehash);
 int fp_launch_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[3], double nx, double ny,
                   double nz);
 int fp_launch_pick(fp_solver* s, const double o[3], const double d[3], double max_perp2);
-int fp_launch_select(fp_solver* s, const double c[3], double r2, int g, int* ids, int max_ids, int* n);
+int fp_launch_select(fp_solver* s, const double c[3], double r2, int same, int* ids, int max_ids, int* n);
+int fp_launch_components(fp_solver* s);
 int fp_launch_emit(fp_solver* s, int m);
 int fp_launch_count_active(fp_solver* s);

@@ -155,14 +155,16 @@ __global__ void __launch_bounds__(BLOCK) k_pick(int ns, const float4* __restrict
 // the reference's JS does.  One workgroup walks the solids in chunks of
 // BLOCK: a wave ballot and the waves' counts give each match its place.
 // *n receives the total number of matches (ids past max are not written).
+// same >= 0: only the solids whose label equals solid same's.
 __global__ void __launch_bounds__(BLOCK) k_select(int ns, const float4* __restrict__ pos,
-                                                  const int* __restrict__ group, int g, double cx, double cy,
+                                                  const int* __restrict__ group, int same, double cx, double cy,
                                                   double cz, double r2, int* __restrict__ ids, int max_ids,
                                                   int* __restrict__ n) {
   __shared__ int wave_count[BLOCK / 64];
   __shared__ int base;
   if (threadIdx.x == 0) base = 0;
   __syncthreads();
+  const int g = same >= 0 ? group[same] : -1;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   for (int c0 = 0; c0 < ns; c0 += BLOCK) {
     const int i = c0 + (int)threadIdx.x;
@@ -187,9 +189,89 @@ __global__ void __launch_bounds__(BLOCK) k_select(int ns, const float4* __restri
   if (threadIdx.x == 0) *n = base;
 }
 
-int fp_launch_select(fp_solver* s, const double c[3], double r2, int g, int* ids, int max_ids, int* n) {
-  hipLaunchKernelGGL(k_select, dim3(1), dim3(BLOCK), 0, s->stream, s->ns, s->pos, s->dgroup, g, c[0], c[1], c[2], r2,
-                     ids, max_ids, n);
+int fp_launch_select(fp_solver* s, const double c[3], double r2, int same, int* ids, int max_ids, int* n) {
+  hipLaunchKernelGGL(k_select, dim3(1), dim3(BLOCK), 0, s->stream, s->ns, s->pos, s->dgroup, same, c[0], c[1], c[2],
+                     r2, ids, max_ids, n);
+  FP_TRY(s, hipGetLastError());
+  return FP_OK;
+}
+
+// ---- connected components of the solids (solver.js:488-499) -------------------
+//
+// The partition _recomputeGroups computes, on the device: a lock-free
+// union-find over the alive constraints.  parent[x] <= x always; a hook
+// links the larger of two roots under the smaller with a compare-and-swap,
+// and a failed swap means that root gained a parent meanwhile, so the loop
+// continues from a strictly smaller root and ends.  Finding halves the path
+// (each write stores an ancestor, so the forest stays valid whatever the
+// interleaving).  The root of a component is its smallest solid index, which
+// becomes every member's label.  (The reference labels a component by the
+// root its sequential union order leaves, which the host relabel keeps for
+// the group array; the partition is the same.)
+
+__device__ __forceinline__ int cc_load(const int* p) {
+  return __hip_atomic_load((const FP_GLOBAL int*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void cc_store(int* p, int v) {
+  __hip_atomic_store((FP_GLOBAL int*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int cc_find(int* parent, int x) {
+  int cur = cc_load(parent + x);
+  if (cur == x) return x;
+  int prev = x, next;
+  while (cur > (next = cc_load(parent + cur))) {
+    cc_store(parent + prev, next);
+    prev = cur;
+    cur = next;
+  }
+  return cur;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cc_init(int ns, int* __restrict__ parent) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < ns) parent[i] = i;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cc_hook(int C, const int2* __restrict__ cons_ab,
+                                                   const int2* __restrict__ cent,
+                                                   const uint8_t* __restrict__ ealive, int* __restrict__ parent) {
+  const int c = blockIdx.x * BLOCK + threadIdx.x;
+  if (c >= C) return;
+  if (!ealive[cent[c].x]) return;
+  const int2 ab = cons_ab[c];
+  int ra = cc_find(parent, ab.x), rb = cc_find(parent, ab.y);
+  while (ra != rb) {
+    const int hi = max(ra, rb), lo = min(ra, rb);
+    int expect = hi;
+    if (__hip_atomic_compare_exchange_strong((FP_GLOBAL int*)(parent + hi), &expect, lo, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      break;
+    // hi gained the parent `expect` (< hi) meanwhile: continue from the roots
+    ra = cc_find(parent, expect);
+    rb = cc_find(parent, lo);
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cc_final(int ns, int* __restrict__ parent, int* __restrict__ ncomp) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  const int r = i < ns ? cc_find(parent, i) : -1;
+  if (i < ns) cc_store(parent + i, r);
+  const int roots = (int)__popcll(__ballot(i < ns && r == i));
+  if ((threadIdx.x & 63) == 0 && roots) atomicAdd(ncomp, roots);
+}
+
+// Component labels of the solids into dgroup (and their count into
+// dflags[DF_NCOMP]), queued on the solver's stream.
+int fp_launch_components(fp_solver* s) {
+  if (s->ns <= 0) return FP_OK;
+  const int nb = fp_blocks(s->ns);
+  hipLaunchKernelGGL(k_cc_init, dim3(nb), dim3(BLOCK), 0, s->stream, s->ns, s->dgroup);
+  if (s->C > 0)
+    hipLaunchKernelGGL(k_cc_hook, dim3(fp_blocks(s->C)), dim3(BLOCK), 0, s->stream, s->C, s->cons_ab, s->cent,
+                       s->ealive, s->dgroup);
+  FP_TRY(s, hipMemsetAsync(s->dflags + DF_NCOMP, 0, sizeof(int), s->stream));
+  hipLaunchKernelGGL(k_cc_final, dim3(nb), dim3(BLOCK), 0, s->stream, s->ns, s->dgroup, s->dflags + DF_NCOMP);
   FP_TRY(s, hipGetLastError());
   return FP_OK;
 }

@@ -161,6 +161,15 @@ export class Solver {
     return addon.selectSolids(this._h, cx, cy, cz, radius, sameGroupAs);
   }
 
+  // _recomputeGroups (solver.js:488-499) on the device over the constraints
+  // alive now: {labels, count}, labels[i] the smallest solid index of solid
+  // i's component (the partition of `group`, whose labels are the
+  // reference's union-find roots).
+  components() {
+    this._flush();
+    return addon.components(this._h);
+  }
+
   _state() {
     if (!this._views) {
       this._flush();

@@ -177,9 +177,20 @@ int fp_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[
  * pickSolid): the solids in the component of solid same_group_as (any
  * component when it is -1) whose position lies within radius of center,
  * tested in double, in ascending index order.  Writes at most max_ids ids;
- * *n receives the number of matches. */
+ * *n receives the number of matches.  The component is the one the caller
+ * would read in group (relabelled at a step's end, solver.js:238-241): when
+ * no cut or constraint write happened since, it is computed on the device
+ * (fp_components), else from the host labels. */
 int fp_select_solids(fp_solver* s, const double center[3], double radius, int32_t same_group_as, int32_t* ids,
                      int32_t max_ids, int32_t* n);
+/* _recomputeGroups (solver.js:488-499) on the device: the connected
+ * components of the solids over the constraints alive now, from a lock-free
+ * union-find (the larger of two roots hooked under the smaller).  labels[i]
+ * (ns entries, may be NULL) receives the smallest solid index of i's
+ * component and *count (may be NULL) the number of components.  The
+ * partition is the reference's; its labels are the roots its sequential
+ * union order leaves, which fp_read_state's group keeps. */
+int fp_components(fp_solver* s, int32_t* labels, int32_t* count);
 /* pickSolid(ox, oy, oz, dx, dy, dz, maxPerp) (solver.js:502-515): nearest
  * solid along the ray with perpendicular distance <= max_perp; ties go to the
  * lowest index.  *id = -1 (and *t = +inf) when nothing is hit. */

@@ -5,7 +5,9 @@
 //   settle 3.5 s (210 frames) -> two vertical knife strokes -> grab + drag +
 //   release -> hose held 6 s (360 frames, src/main.js:224-251 pattern with a
 //   seeded jitter).
-// Prints one JSON line per phase: {phase, frames, fluid, alive, ke, minY, maxY, nan}.
+// Prints one JSON line per phase: {phase, frames, fluid, alive, ke, minY, maxY, nan,
+// comps, same}: comps = the device's component count (components()), same =
+// its partition equals the one `group` labels.
 import { Solver } from '../../fluidphysics_amd/js/solver.mjs';
 
 const s = new Solver({ maxFluid: 2600, params: { constraintIters: 3, jacobiOmega: 8 } });
@@ -39,7 +41,14 @@ function report(phase, frames) {
   }
   let alive = 0;
   for (const x of s.consAlive) alive += x;
-  console.log(JSON.stringify({ phase, frames, fluid: s.fluidCount(), alive, ke, minY, maxY, nan }));
+  const c = s.components(), g = s.group, byLabel = new Map(), byGroup = new Map();
+  let same = true;
+  for (let i = 0; i < s.solidCount; i++) {
+    const l = c.labels[i], q = g[i];
+    if (!byLabel.has(l)) byLabel.set(l, q); else if (byLabel.get(l) !== q) same = false;
+    if (!byGroup.has(q)) byGroup.set(q, l); else if (byGroup.get(q) !== l) same = false;
+  }
+  console.log(JSON.stringify({ phase, frames, fluid: s.fluidCount(), alive, ke, minY, maxY, nan, comps: c.count, same }));
 }
 
 function run(n, each) { for (let k = 0; k < n; k++) { if (each) each(k); s.step(1 / 60, 5); frame++; } }

@@ -92,6 +92,9 @@ def test_headless_drive_timeline():
     assert by["slice"]["alive"] < by["settle"]["alive"]      # the strokes cut constraints
     assert by["hose"]["fluid"] > 0                            # the hose emitted water
     assert by["settle"]["minY"] >= 0.09 - 1e-3                # the cube rests on the floor
+    # the device components (components()) partition the solids as `group` does
+    assert all(p["same"] for p in phases), [(p["phase"], p["comps"]) for p in phases]
+    assert by["settle"]["comps"] == 1
 
 
 def to_js_params(p):

@@ -424,3 +424,95 @@ def test_grab_selection_matches_the_callers_rule():
         checked += 1
     assert checked >= 6
     s.close()
+
+
+def _split_scene_solver(frames_before_cut=5):
+    """A single-GPU scene whose cube a planar wedge splits in two (the plane
+    x = cx + 0.013, between two lattice layers), stepped to the cut."""
+    from fluidphysics_amd import from_scene, scenes
+    sc = scenes.single_gpu_scene(frames=40)
+    s = from_scene(sc)
+    for _ in range(frames_before_cut):
+        s.step(1.0 / 60.0, 5)
+    cx, cy, cz = sc.cube[:3]
+    d1 = np.array([0.0, -1.0, -0.9]) / np.linalg.norm([0.0, -1.0, -0.9])
+    d2 = np.array([0.0, -1.0, 0.9]) / np.linalg.norm([0.0, -1.0, 0.9])
+    return sc, s, (cx + 0.013, cy + 5.0, cz, *d1, *d2)
+
+
+def _csgraph_partition(ns, c):
+    """Connected components of the solids over the alive constraints, by
+    scipy's textbook routine (the check is independent of both union-finds)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    keep = c["alive"] != 0
+    g = coo_matrix((np.ones(int(keep.sum())), (c["a"][keep], c["b"][keep])), shape=(ns, ns))
+    return connected_components(g, directed=False)
+
+
+def _same_partition(x, y):
+    """x and y label the same partition: the pairs (x_i, y_i) form a bijection."""
+    pairs = np.unique(np.stack([x, y], axis=1), axis=0)
+    return len(pairs) == len(np.unique(x)) == len(np.unique(y))
+
+
+def test_components_on_device_match_the_partition():
+    """fp_components (the device form of solver.js:488-499) after a cut that
+    splits the cube and a grab that tears it: the partition equals scipy's
+    connected components of the alive constraints and the host union-find's
+    (whose roots are the reference's labels); every label is its component's
+    smallest solid index."""
+    sc, s, knife = _split_scene_solver()
+    log("case components: %s" % sc.name)
+    labels0, n0 = s.components()
+    assert n0 == 1 and not labels0.any()  # the lattice is one body
+    assert s.cut(*knife) > 0
+    cx, cy, cz = sc.cube[:3]
+    s.grabs = [{"i": i, "x": cx - 3.0, "y": cy + 3.0, "z": cz} for i in range(24)]
+    for _ in range(12):
+        s.step(1.0 / 60.0, 5)
+    s.grabs = []
+    s.step(1.0 / 60.0, 5)
+    ns = s.solidCount
+    labels, n = s.components()
+    c = s.read_constraints()
+    ncc, ref = _csgraph_partition(ns, c)
+    group = s.read_state()["group"][:ns]
+    log("components %d (scipy %d), constraints alive %d of %d" % (n, ncc, int(c["alive"].sum()), len(c["alive"])))
+    assert n >= 2 and n == ncc == len(np.unique(labels))
+    assert _same_partition(labels, ref) and _same_partition(labels, group)
+    for lab in np.unique(labels):
+        assert lab == np.flatnonzero(labels == lab)[0]
+    s.close()
+
+
+def test_selection_after_a_cut_sees_the_components_of_before_it():
+    """Between a cut and the next step the caller's group still holds the
+    labels of before the cut (solver.js relabels at a step's end), so the
+    grab selection takes the whole uncut body; after the step, one side."""
+    sc, s, knife = _split_scene_solver()
+    log("case selection across a cut: %s" % sc.name)
+    cx, cy, cz = sc.cube[:3]
+    assert s.cut(*knife) > 0
+    st = s.read_state()
+    ns = s.solidCount
+    pos = st["pos"].reshape(-1, 3)[:ns].astype(np.float64)
+    group = st["group"][:ns]
+    assert len(np.unique(group)) == 1  # not relabelled yet
+    hp = pos.mean(axis=0)
+    hp[0] = cx + 0.013  # on the cut plane, at the body's centre
+    d2 = ((pos - hp) ** 2).sum(axis=1)
+    near = np.flatnonzero(d2 < 0.35 * 0.35)
+    assert (pos[near, 0] < hp[0]).any() and (pos[near, 0] > hp[0]).any()
+    pick = int(near[0])
+    assert np.array_equal(s.select_solids(hp, 0.35, pick), near)
+    s.step(1.0 / 60.0, 5)
+    st = s.read_state()
+    pos = st["pos"].reshape(-1, 3)[:ns].astype(np.float64)
+    group = st["group"][:ns]
+    assert len(np.unique(group)) >= 2
+    d2 = ((pos - hp) ** 2).sum(axis=1)
+    want = np.flatnonzero((group == group[pick]) & (d2 < 0.35 * 0.35))
+    got = s.select_solids(hp, 0.35, pick)
+    assert np.array_equal(got, want) and len(got) < len(np.flatnonzero(d2 < 0.35 * 0.35))
+    s.close()
