@@ -118,7 +118,7 @@ void free_all(fp_solver* s) {
   for (auto& cp : s->checkpoints)
     if (cp.block) s->device_bytes -= (int64_t)fp_pool_put(s->device, cp.block);
   s->checkpoints.clear();
-  dfree(s, s->grab_idx), dfree(s, s->grab_tgt), dfree(s, s->emit_idx), dfree(s, s->emit_p), dfree(s, s->emit_v);
+  dfree(s, s->grab_idx), dfree(s, s->grab_tgt), dfree(s, s->emit_buf);
 }
 
 uint32_t next_pow2(double v, uint32_t lo) {
@@ -624,6 +624,19 @@ static void upload_free(fp_solver* s) {
   }
 }
 
+// Device staging for an emit of m particles (or a selection of m - 1 ids),
+// grown when needed: m entries of 36 bytes.
+static int emit_reserve(fp_solver* s, int m) {
+  if (m <= s->emit_cap) return FP_OK;
+  FP_TRY(s, hipStreamSynchronize(s->stream));
+  dfree(s, s->emit_buf);
+  const int cap = std::max(m, 1024);
+  const int rc = dalloc(s, &s->emit_buf, ((size_t)cap * 9 + 3) / 4);
+  if (rc) return rc;
+  s->emit_cap = cap;
+  return FP_OK;
+}
+
 int fp_emit(fp_solver* s, const double* pos, const double* vel, const double* mass, int32_t n) {
   if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_emit: NULL solver");
   if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_emit before fp_add_solid_cube");
@@ -635,16 +648,7 @@ int fp_emit(fp_solver* s, const double* pos, const double* vel, const double* ma
   // only the last maxFluid emits of a batch survive the ring (solver.js:140-141)
   const int first = n > s->max_fluid ? n - s->max_fluid : 0;
   const int m = n - first;
-  if (m > s->emit_cap) {
-    FP_TRY(s, hipStreamSynchronize(s->stream));
-    dfree(s, s->emit_idx);
-    dfree(s, s->emit_p);
-    dfree(s, s->emit_v);
-    const int cap = std::max(m, 1024);
-    if ((rc = dalloc(s, &s->emit_idx, cap)) || (rc = dalloc(s, &s->emit_p, cap)) || (rc = dalloc(s, &s->emit_v, cap)))
-      return rc;
-    s->emit_cap = cap;
-  }
+  if ((rc = emit_reserve(s, m))) return rc;
   // staged in pinned memory: the copies run behind the host, which goes on
   // to queue the next step
   char* h = nullptr;
@@ -662,9 +666,9 @@ int fp_emit(fp_solver* s, const double* pos, const double* vel, const double* ma
     next = (next + 1) % s->max_fluid;
   }
   s->fluid_next = (int)(((int64_t)s->fluid_next + n) % s->max_fluid);
-  FP_TRY(s, hipMemcpyAsync(s->emit_idx, idx, m * sizeof(int), hipMemcpyHostToDevice, s->stream));
-  FP_TRY(s, hipMemcpyAsync(s->emit_p, hp, m * sizeof(float4), hipMemcpyHostToDevice, s->stream));
-  FP_TRY(s, hipMemcpyAsync(s->emit_v, hv, m * sizeof(float4), hipMemcpyHostToDevice, s->stream));
+  // hp, hv and idx are back to back, as the device staging expects
+  FP_TRY(s, hipMemcpyAsync(s->emit_buf, hp, (size_t)m * (2 * sizeof(float4) + sizeof(int)), hipMemcpyHostToDevice,
+                           s->stream));
   if ((rc = upload_end(s))) return rc;
   // the device staging is reused in stream order: the next emit's copies
   // queue behind this launch
@@ -870,22 +874,14 @@ int fp_select_solids(fp_solver* s, const double center[3], double radius, int32_
   }
   // the ids and the count land in the solver's emit staging (grown if needed)
   const int cap = std::max(max_ids, 1) + 1;
-  if (cap > s->emit_cap) {
-    FP_TRY(s, hipStreamSynchronize(s->stream));
-    dfree(s, s->emit_idx);
-    dfree(s, s->emit_p);
-    dfree(s, s->emit_v);
-    if ((rc = dalloc(s, &s->emit_idx, cap)) || (rc = dalloc(s, &s->emit_p, cap)) || (rc = dalloc(s, &s->emit_v, cap)))
-      return rc;
-    s->emit_cap = cap;
-  }
-  if ((rc = fp_launch_select(s, center, radius * radius, same_group_as, s->emit_idx + 1, max_ids, s->emit_idx)))
-    return rc;
+  if ((rc = emit_reserve(s, cap))) return rc;
+  int* out = (int*)s->emit_buf;
+  if ((rc = fp_launch_select(s, center, radius * radius, same_group_as, out + 1, max_ids, out))) return rc;
   int total = 0;
-  FP_TRY(s, d2h(s, &total, s->emit_idx, sizeof(int)));
+  FP_TRY(s, d2h(s, &total, out, sizeof(int)));
   *n = total;
   const int k = std::min(total, max_ids);
-  if (k > 0) FP_TRY(s, d2h(s, ids, s->emit_idx + 1, k * sizeof(int32_t)));
+  if (k > 0) FP_TRY(s, d2h(s, ids, out + 1, k * sizeof(int32_t)));
   return FP_OK;
 }
 

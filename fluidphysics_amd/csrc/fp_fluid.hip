@@ -315,11 +315,16 @@ __device__ __forceinline__ void contacts_fluid_body(const SceneArgs& A, int lb) 
   uint32_t k1 = 0;
   float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
   if (e < fp.n) {
-    const int x = e < gld(cc.lstart + fp.cg.M) ? gld(cc.sid + e) : -1;
+    // the slot's particle and position load together (the empty asm takes
+    // both here, so the position load is not sunk behind the branch)
+    const int occupied = gld(cc.lstart + fp.cg.M);
+    int xs = gld(cc.sid + e);
+    float4 px = gld(cc.spos + e);
+    asm volatile("" : "+v"(xs), "+v"(px.x), "+v"(px.y), "+v"(px.z), "+v"(px.w));
+    const int x = e < occupied ? xs : -1;
     if (x < fp.ns) {
       gst(tkey + e, (uint32_t)NO_KEY);
     } else {
-      const float4 px = gld(cc.spos + e);
       const Cell c0 = cell_of(px.x, px.y, px.z, fp.cg.inv_cell);  // sort cell
       float ax = 0.f, ay = 0.f, az = 0.f;
       if (fp.ns > 0) {

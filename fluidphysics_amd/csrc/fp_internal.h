@@ -173,8 +173,10 @@ struct fp_solver {
   float4* grab_tgt = nullptr;
   int grab_cap = 0;
   std::vector<int> grab_round_len;
-  int* emit_idx = nullptr;
-  float4 *emit_p = nullptr, *emit_v = nullptr;
+  // device staging of an emit of m particles: m positions, m velocities, m
+  // slot indices, back to back (the host staging's layout, one copy); also
+  // the output of fp_select_solids (count, then ids, as ints)
+  float4* emit_buf = nullptr;
   int emit_cap = 0;
   // pinned host staging of the small uploads (emits, grab lists): two
   // buffers used in turn, each with an event recorded after the copies that

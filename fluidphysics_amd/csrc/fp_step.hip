@@ -639,7 +639,8 @@ int fp_group_snapshot(fp_group* g, float* out) {
 
 int fp_launch_emit(fp_solver* s, int m) {
   PassScope ps(s, P_EMIT);
-  hipLaunchKernelGGL(k_emit, dim3(fp_blocks(m)), dim3(BLOCK), 0, s->stream, m, s->emit_idx, s->emit_p, s->emit_v,
+  hipLaunchKernelGGL(k_emit, dim3(fp_blocks(m)), dim3(BLOCK), 0, s->stream, m, (const int*)(s->emit_buf + 2 * m),
+                     s->emit_buf, s->emit_buf + m,
                      s->pos, s->prev, s->vel, s->active);
   FP_TRY(s, hipGetLastError());
   return FP_OK;
