@@ -1262,8 +1262,9 @@ int fp_scene_bytes(const fp_params* p, int32_t max_fluid, int32_t cube_n, int64_
   scene_buffers(&tmp, size);
   int64_t total = (int64_t)fp_pool_class(size.off);
   if (max_fluid > 0) {
+    // the emit staging, at its largest (one emit of max_fluid; emit_reserve)
     const size_t cap = (size_t)std::max(max_fluid, 1024);
-    total += (int64_t)(fp_pool_class(cap * sizeof(int)) + 2 * fp_pool_class(cap * sizeof(float4)));
+    total += (int64_t)fp_pool_class((cap * 9 + 3) / 4 * sizeof(float4));
   }
   *bytes = total;
   return FP_OK;

@@ -45,6 +45,7 @@
 
 #include "fp_internal.h"
 #include "fp_kernels.h"
+#include "fp_bounds.h"
 
 #ifndef FLUID_BLOCK
 #define FLUID_BLOCK 256  // threads per block of the density, displacement and XSPH passes
@@ -855,7 +856,8 @@ __device__ __forceinline__ void displace_body(const SceneArgs& A, int lb) {
     sz *= g;
   }
   const float w = gld(pos + me.x()).w;
-  gst(pos + me.x(), make_float4(me.p.x + sx, me.p.y + sy, me.p.z + sz, w));
+  // the particle's substep ends here: bounds, then its velocity for XSPH
+  fluid_finish(A, me.x(), me.e, make_float4(me.p.x + sx, me.p.y + sy, me.p.z + sz, w));
 }
 
 __global__ void __launch_bounds__(FLUID_BLOCK) WAVES_ATTR(DISPLACE_WAVES) kg_displace(const SceneArgs* __restrict__ S,
@@ -891,6 +893,9 @@ __device__ __forceinline__ void xsph_body(const SceneArgs& A, int lb) {
   const float4* __restrict__ svtmp = A.svtmp;
   float4* __restrict__ vel = A.vel;
   const uint8_t* __restrict__ active = A.active;
+  // the solids' bounding box is rebuilt by the next integrate; both contact
+  // passes, its readers, are done
+  if (lb == 0 && threadIdx.x < 6) gst(A.sbox + threadIdx.x, threadIdx.x < 3 ? INT_MAX : INT_MIN);
   PairSelf me;
   if (!pair_self(fp, src, fluid_lane(lb), me)) return;
   if (!gld(active + me.x())) return;

@@ -344,6 +344,7 @@ struct fp_group {
   bool dirty = true;
   int iters = 0;
   bool any_grabs = false;
+  bool any_unconstrained = false;  // a member has solids but no constraint iterations
   // captured frame
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;

@@ -18,6 +18,7 @@
 //              minimum of (t, index) over the solids that pass both tests.
 #include "fp_internal.h"
 #include "fp_kernels.h"
+#include "fp_bounds.h"
 
 #define CLANES 32
 
@@ -74,7 +75,14 @@ __device__ __forceinline__ void constraints_body(const SceneArgs& A, int lb, int
     cy += __shfl_xor(cy, off, CLANES);
     cz += __shfl_xor(cz, off, CLANES);
   }
-  if (valid && lane == 0) gst(sout + a, make_float4(pa.x + cx, pa.y + cy, pa.z + cz, pa.w));
+  if (valid && lane == 0) {
+    const float4 p = make_float4(pa.x + cx, pa.y + cy, pa.z + cz, pa.w);
+    // the last iteration finishes the solid's substep (bounds, velocity)
+    if (it == A.cp.iters - 1)
+      solid_finish(A, a, p);
+    else
+      gst(sout + a, p);
+  }
   if (tore) *A.tear = 1;
 }
 

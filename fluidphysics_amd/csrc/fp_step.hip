@@ -22,6 +22,7 @@
 
 #include "fp_internal.h"
 #include "fp_kernels.h"
+#include "fp_bounds.h"
 
 // Integrate (solver.js:194-202); with COUNT the particle is also entered
 // into the cell grid at its integrated position (cell key, arrival slot,
@@ -90,78 +91,13 @@ __global__ void __launch_bounds__(BLOCK) k_grab(int n, const int* __restrict__ i
   pos[i] = p;
 }
 
-// Math.sign(q) || 1 (solver.js:413): +1 for +0, -0 and positive q.
-__device__ __forceinline__ float sign_or_one(float q) { return q < 0.f ? -1.f : 1.f; }
-
-// Bounds (solver.js:389-427) then velocity + damping + clamp (:219-234).
-// Solids read their positions from the last constraint iteration and write
-// vel; fluid velocities go to the fluid-order slot in svtmp, where XSPH
-// reads them (a particle culled here leaves its velocity as it was, in vel
-// and in svtmp).
+// The solids of a scene without constraint iterations (their positions are
+// final after the contacts); every other particle is finished by the last
+// constraint iteration (solids) or by the displacement pass (fluid).
 __device__ __forceinline__ void bounds_velocity_body(const SceneArgs& A, int lb) {
-  const BoundsParams& bp = A.bp;
-  const float4* __restrict__ solid_src = A.cp.iters > 0 ? A.spos_c[(A.cp.iters - 1) & 1] : A.pos;
-  float4* __restrict__ pos = A.pos;
-  // the solids' bounding box is rebuilt by the next integrate; both contact
-  // passes, its last readers, are done
-  if (lb == 0 && threadIdx.x < 6) gst(A.sbox + threadIdx.x, threadIdx.x < 3 ? INT_MAX : INT_MIN);
   const int i = lb * BLOCK + threadIdx.x;
-  if (i >= bp.n) return;
-  const bool solid = i < bp.ns;
-  // a solid's position loads with its flag (inactive solids move too); an
-  // inactive fluid slot, most of a fluid range, loads nothing more
-  const uint8_t flag = gld(A.active + i);
-  float4 p = solid ? gld(solid_src + i) : make_float4(0.f, 0.f, 0.f, 0.f);
-  if (!flag) {
-    if (solid) gst(pos + i, p);  // the reference's constraint sweep moves inactive solids too
-    return;
-  }
-  if (!solid) p = gld(pos + i);
-  const float4 pv = gld(A.prev + i);
-  const int fs = solid ? 0 : gld(A.fslot + i);
-  const float r = solid ? bp.rs : bp.rf;
-  if (p.y < r) {
-    p.y = r;
-    const float mu = solid ? bp.mu_s : bp.mu_f;
-    p.x -= (p.x - pv.x) * mu;
-    p.z -= (p.z - pv.z) * mu;
-  }
-  if (p.y < bp.wall_h + r) {
-    const float lim = r + bp.wall_t;
-    if (fabsf(fabsf(p.x) - bp.wall_x) < lim) {
-      const float side = sign_or_one(p.x);
-      p.x = fabsf(p.x) < bp.wall_x ? side * (bp.wall_x - lim) : side * (bp.wall_x + lim);
-    }
-    if (fabsf(fabsf(p.z) - bp.wall_z) < lim) {
-      const float side = sign_or_one(p.z);
-      p.z = fabsf(p.z) < bp.wall_z ? side * (bp.wall_z - lim) : side * (bp.wall_z + lim);
-    }
-  }
-  gst(pos + i, p);
-  if (!solid && (p.y < -3.f || p.x * p.x + p.z * p.z > bp.cull_r2)) {
-    gst(A.active + i, (uint8_t)0);
-    gst(A.svtmp + fs, gld(A.vel + i));
-    return;
-  }
-  float vx = (p.x - pv.x) * bp.inv_h;
-  float vy = (p.y - pv.y) * bp.inv_h;
-  float vz = (p.z - pv.z) * bp.inv_h;
-  const float damp = solid ? bp.damp_s : bp.damp_f;
-  vx *= damp;
-  vy *= damp;
-  vz *= damp;
-  const float sp2 = vx * vx + vy * vy + vz * vz;
-  if (sp2 > bp.vmax * bp.vmax) {
-    const float g = bp.vmax / sqrtf(sp2);
-    vx *= g;
-    vy *= g;
-    vz *= g;
-  }
-  const float4 v = make_float4(vx, vy, vz, 0.f);
-  if (solid)
-    gst(A.vel + i, v);
-  else
-    gst(A.svtmp + fs, v);
+  if (i >= A.bp.ns || A.cp.iters > 0) return;
+  solid_finish(A, i, gld(A.pos + i));
 }
 
 __global__ void __launch_bounds__(BLOCK) kg_bounds_velocity(const SceneArgs* __restrict__ S,
@@ -347,6 +283,8 @@ int fp_group_prepare(fp_group* g, double h) {
     }
   }
   if (g->iters < 0) g->iters = 0;
+  g->any_unconstrained = false;
+  for (int k = 0; k < ns; k++) g->any_unconstrained |= args[k].cp.ns > 0 && args[k].cp.iters == 0;
   int64_t snap = 0;
   for (int k = 0; k < ns; k++) {
     args[k].snap_off = snap;
@@ -469,7 +407,8 @@ static int group_substep(fp_group* g) {
   if ((rc = fp_group_cell_grid(g, !g->any_grabs))) return rc;
   if (g->capturing && g->fork) {
     // the solid chain on the side stream, the fluid chain here; they join
-    // before the bounds pass
+    // before XSPH (each chain ends its particles' substep: the last
+    // constraint iteration its solids', the displacement its fluid's)
     FP_GTRY(g, hipEventRecord(g->ev_fork, g->stream));
     FP_GTRY(g, hipStreamWaitEvent(g->side, g->ev_fork, 0));
     if ((rc = fp_group_contacts_solid(g, g->side))) return rc;
@@ -487,7 +426,7 @@ static int group_substep(fp_group* g) {
     if ((rc = fp_group_displace(g))) return rc;
     if ((rc = fp_group_constraints(g, g->iters, g->stream))) return rc;
   }
-  {
+  if (g->any_unconstrained) {
     PassScope ps(g->prof, g->stream, P_BOUNDS_VELOCITY);
     GROUP_LAUNCH(g, kg_bounds_velocity, W_PARTICLE, BLOCK);
   }
