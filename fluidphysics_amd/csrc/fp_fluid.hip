@@ -351,10 +351,29 @@ __device__ __forceinline__ void contacts_fluid_body(const SceneArgs& A, int lb) 
               b[j] = gld(cc.lstart + k + 1);
               m[j] = gld(cc.lnsol + k);
             }
+            int total = 0;
 #pragma unroll
             for (uint32_t j = 0; j < NB_RUN; j++) {
-              if (j < lo || j >= hi || a[j] == b[j]) continue;
-              for (int q = a[j]; q < a[j] + m[j]; q++) contact_add(fp, px, fp.rf, gld(cc.spos + q), fp.rs, ax, ay, az);
+              if (j < lo || j >= hi || a[j] == b[j]) m[j] = 0;
+              total += m[j];
+            }
+            // the row's solids as one list (cell by cell, slots ascending:
+            // the order of the sums), loaded four at a time ahead of them
+            for (int t0 = 0; t0 < total; t0 += 4) {
+              float4 py[4];
+#pragma unroll
+              for (int i = 0; i < 4; i++) {
+                int t = min(t0 + i, total - 1), q = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < NB_RUN; j++) {
+                  if (t >= 0 && t < m[j]) q = a[j] + t;
+                  t -= m[j];
+                }
+                py[i] = gld(cc.spos + q);
+              }
+#pragma unroll
+              for (int i = 0; i < 4; i++)
+                if (t0 + i < total) contact_add(fp, px, fp.rf, py[i], fp.rs, ax, ay, az);
             }
           }
         }
