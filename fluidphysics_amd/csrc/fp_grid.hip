@@ -217,11 +217,18 @@ __device__ __forceinline__ void cell_rank_body(const SceneArgs& A, int lb) {
   const float4 p = gld(A.pos + i);
   if (h == 0xffffffffu) return;
   const int s = gld(A.lstart + k), e = gld(A.lstart + k + 1);
+  // the cell's entries four at a time (their loads in flight together)
   int r = 0, nsol = 0;
-  for (int q = s; q < e; q++) {
-    const int j = gld(A.ltmp + q);
-    r += j < i ? 1 : 0;
-    nsol += j < A.ns ? 1 : 0;
+  for (int q0 = s; q0 < e; q0 += 4) {
+    int v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = gld(A.ltmp + min(q0 + u, e - 1));
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const bool in = q0 + u < e;
+      r += (in && v[u] < i) ? 1 : 0;
+      nsol += (in && v[u] < A.ns) ? 1 : 0;
+    }
   }
   const int d = s + r;
   gst(A.sid + d, i);
@@ -299,8 +306,15 @@ __device__ __forceinline__ void fluid_rank_body(const SceneArgs& A, int lb) {
   const float4 tp = gld(A.tpos + e);
   if (k == NO_KEY) return;
   const int s = gld(A.lstart_f + k), t = gld(A.lstart_f + k + 1);
+  // the cell's entries four at a time (their loads in flight together)
   int r = 0;
-  for (int q = s; q < t; q++) r += gld(A.ltmp + q) < i ? 1 : 0;
+  for (int q0 = s; q0 < t; q0 += 4) {
+    int v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = gld(A.ltmp + min(q0 + u, t - 1));
+#pragma unroll
+    for (int u = 0; u < 4; u++) r += (q0 + u < t && v[u] < i) ? 1 : 0;
+  }
   const int d = s + r;
   gst(A.fsid + d, i);
   gst(A.fslot + i, d);
