@@ -236,7 +236,7 @@ def test_batch_profile_reports_every_pass():
     prof = b.profile_read()
     # bounds and velocity end each particle's substep inside the last
     # constraint iteration (solids) and the displacement (fluid)
-    for name in ("integrate", "scan_reduce", "scatter", "rank", "contacts", "resort", "density", "displace",
+    for name in ("integrate", "scan_apply", "scatter", "rank", "contacts", "resort", "density", "displace",
                  "constraints", "xsph"):
         assert name in prof and prof[name][0] >= 2 * SUB, (name, prof.get(name))
     assert b.info().graph_launches == 0
