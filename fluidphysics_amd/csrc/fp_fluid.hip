@@ -51,7 +51,7 @@
 #define FLUID_BLOCK 256  // threads per block of the density, displacement and XSPH passes
 #endif
 #ifndef DENSITY_BATCH
-#define DENSITY_BATCH 2  // candidates per step of the density loop
+#define DENSITY_BATCH 1  // candidates per step of the density loop (1: fastest measured, also in batches)
 #endif
 #define MAXB 32          // bonds per solid staged in LDS for the contact pass
 
@@ -532,11 +532,11 @@ __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sq
 // (cull_r2 carries a margin over the float and double tests).
 //
 // Memory latency and branch overhead bound this loop, so: the next row's
-// bounds load while the current row runs; a row is read in batches of four
-// candidates (the last batch predicated, no serial tail), each batch's
-// loads issued before the previous batch is evaluated; the common case is
-// decided with selects and a single, rarely taken branch per batch covers
-// the d ~ H band and multiplicities other than 1.
+// bounds load while the current row runs; a row is read in batches of
+// DENSITY_BATCH candidates per lane (the last batch predicated, no serial
+// tail), each batch's loads issued before the previous batch is evaluated;
+// the common case is decided with selects and a single, rarely taken branch
+// per batch covers the d ~ H band and multiplicities other than 1.
 struct DensityGather {
   float rho = 0.f, rhon = 0.f;
   int cnt = 0;
@@ -559,8 +559,8 @@ __device__ __forceinline__ void density_take(const FluidParams& fp, const PairLi
   }
 }
 
-// One batch of four candidates in slots q .. q+3 (those past last are not
-// candidates).  Weight 1 for d^2 < H^2 (1 - 1e-5) unless a flag is set on
+// One batch of DENSITY_BATCH candidates in slots q, q+1, ... (those past
+// last are not candidates).  Weight 1 for d^2 < H^2 (1 - 1e-5) unless a flag is set on
 // either particle; the band and flagged cases go to the one rare branch.
 // flag_min: the least w bits of a candidate that make the pair flagged (0
 // when me itself is flagged, so every candidate is).
