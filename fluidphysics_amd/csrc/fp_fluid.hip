@@ -50,6 +50,9 @@
 #ifndef FLUID_BLOCK
 #define FLUID_BLOCK 256  // threads per block of the density, displacement and XSPH passes
 #endif
+#ifndef REPLAY_BATCH
+#define REPLAY_BATCH 4  // pair-list entries per step of the displacement and XSPH replay
+#endif
 #ifndef DENSITY_BATCH
 #define DENSITY_BATCH 1  // candidates per step of the density loop (1: fastest measured, also in batches)
 #endif
@@ -667,7 +670,7 @@ __device__ __forceinline__ void visit_rows_simple(const FluidParams& fp, const P
 }
 
 // Replays the recorded pairs of me into f, in the order they were found.
-// Entries are read one batch of four ahead; a batch's gathers (the
+// Entries are read one batch of REPLAY_BATCH ahead; a batch's gathers (the
 // neighbour's position when F::kNeedPos, and f.fetch(q)) are issued
 // together, before the next batch's entries, and before any of them is used.
 template <class F>
@@ -678,30 +681,30 @@ __device__ __forceinline__ void replay_pairs(const PairSource& src, const PairLi
   const int last = cnt - 1;
   // entry k of this lane at byte lo + k * NB_ROWB of the wave's block
   const uint32_t olast = lo + (uint32_t)max(last, 0) * NB_ROWB;
-  uint32_t v[4];
+  uint32_t v[REPLAY_BATCH];
 #pragma unroll
-  for (int i = 0; i < 4; i++) v[i] = cnt > 0 ? gld_at(base, min(lo + (uint32_t)i * NB_ROWB, olast)) : 0u;
-  for (int k = 0; k < cnt; k += 4) {
+  for (int i = 0; i < REPLAY_BATCH; i++) v[i] = cnt > 0 ? gld_at(base, min(lo + (uint32_t)i * NB_ROWB, olast)) : 0u;
+  for (int k = 0; k < cnt; k += REPLAY_BATCH) {
     // this batch's gathers first, then the next batch's entries: the loads
     // complete in issue order, so waiting for the gathers leaves the
     // entries in flight
-    float4 py[4];
-    typename F::Extra ex[4];
+    float4 py[REPLAY_BATCH];
+    typename F::Extra ex[REPLAY_BATCH];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < REPLAY_BATCH; i++) {
       const int q = (int)(v[i] & NB_QMASK);
       if (F::kNeedPos) py[i] = gld_xyz_at(src.spos2, (uint32_t)q << 4);
       ex[i] = f.fetch(q);
     }
-    uint32_t u[4];
+    uint32_t u[REPLAY_BATCH];
 #pragma unroll
-    for (int i = 0; i < 4; i++) u[i] = v[i];
-    if (k + 4 < cnt) {
+    for (int i = 0; i < REPLAY_BATCH; i++) u[i] = v[i];
+    if (k + REPLAY_BATCH < cnt) {
 #pragma unroll
-      for (int i = 0; i < 4; i++) v[i] = gld_at(base, min(lo + (uint32_t)(k + 4 + i) * NB_ROWB, olast));
+      for (int i = 0; i < REPLAY_BATCH; i++) v[i] = gld_at(base, min(lo + (uint32_t)(k + REPLAY_BATCH + i) * NB_ROWB, olast));
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < REPLAY_BATCH; i++) {
       if (k + i <= last) {
         const int q = (int)(u[i] & NB_QMASK);
         const int w = (int)(u[i] >> NB_QBITS) + 1;

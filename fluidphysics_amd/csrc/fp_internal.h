@@ -56,8 +56,12 @@
 // the scene (fp_params.fluid_lanes; 0 = FLUID_LANES_BIG for scenes of at
 // least FLUID_LANES_AUTO fluid slots, else FLUID_LANES_SMALL).  It orders the
 // pair list and the lane sums, so it is part of the scene's arithmetic.
+#ifndef FLUID_LANES_SMALL
 #define FLUID_LANES_SMALL 2
+#endif
+#ifndef FLUID_LANES_BIG
 #define FLUID_LANES_BIG 4
+#endif
 #define FLUID_LANES_AUTO 131072
 
 // device flag words (dflags)
