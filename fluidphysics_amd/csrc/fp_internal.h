@@ -42,7 +42,9 @@
 #include "../../include/fluidphysics.h"
 #include "fp_kernels.h"
 
+#ifndef BLOCK
 #define BLOCK 256
+#endif
 #ifndef SCAN_ITEMS
 #define SCAN_ITEMS 4  // cells per thread of a scan tile
 #endif
