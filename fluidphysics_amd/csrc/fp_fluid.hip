@@ -802,15 +802,16 @@ __device__ __forceinline__ int fluid_block(int j, int o, int a) {
 
 // Minimum waves per SIMD asked of the register allocator for the three
 // gather passes (0: the allocator's choice).  These loops are bound by
-// memory latency and issue, so resident waves matter more than registers.
+// memory latency and issue, so resident waves matter more than registers:
+// 8 (64 VGPRs, no vector spills) measured 1% faster than the allocator's 7.
 #ifndef DENSITY_WAVES
-#define DENSITY_WAVES 0
+#define DENSITY_WAVES 8
 #endif
 #ifndef DISPLACE_WAVES
-#define DISPLACE_WAVES 0
+#define DISPLACE_WAVES 8
 #endif
 #ifndef XSPH_WAVES
-#define XSPH_WAVES 0
+#define XSPH_WAVES 8
 #endif
 #define WAVES_ATTR_0
 #define WAVES_ATTR_45 __attribute__((amdgpu_waves_per_eu(4, 5)))
