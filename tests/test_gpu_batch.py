@@ -241,3 +241,41 @@ def test_batch_profile_reports_every_pass():
         assert name in prof and prof[name][0] >= 2 * SUB, (name, prof.get(name))
     assert b.info().graph_launches == 0
     b.close()
+
+
+def test_forked_frames_equal_serial_frames():
+    """A captured frame of a group of 131,072 particle slots or more runs the
+    solid chain (solid contacts, constraint iterations) on a second stream
+    beside the fluid chain; eager frames run them one after the other.  The
+    two must give the same bits (the chains share no buffer they write)."""
+    from fluidphysics_amd import Batch, from_scene, scenes
+    frames = 9
+    scs, slots = [], 0
+    for k in range(40, 80):
+        sc = _scenes([k], frames)[0]
+        scs.append(sc)
+        slots += sc.cube[3] ** 3 + sc.max_fluid
+        if slots >= 160000:
+            break
+    assert slots >= 131072  # the side stream is used for this group
+    log("case forked vs serial frames: %d scenes, %d slots" % (len(scs), slots))
+    runs = []
+    for use_graph in (True, False):
+        b = Batch()
+        b.graph(use_graph)
+        ss = [from_scene(sc) for sc in scs]
+        for s in ss:
+            b.add(s)
+        snaps = {}
+        for f in range(frames):
+            for s, sc in zip(ss, scs):
+                _cuts(s, sc, f)
+            b.step(DT, SUB)
+            if f in (TUNE_FRAMES + 1, frames - 1):
+                snaps[f] = [_snap(s) for s in ss]
+        assert b.info().graph_launches == (frames - TUNE_FRAMES if use_graph else 0)
+        runs.append(snaps)
+        b.close()
+    for f in runs[0]:
+        for k in range(len(scs)):
+            _same(runs[0][f][k], runs[1][f][k], "%s frame %d" % (scs[k].name, f))
