@@ -279,3 +279,43 @@ def test_forked_frames_equal_serial_frames():
     for f in runs[0]:
         for k in range(len(scs)):
             _same(runs[0][f][k], runs[1][f][k], "%s frame %d" % (scs[k].name, f))
+
+
+def test_unconstrained_scene_finishes_its_solids():
+    """A scene with solids but no constraints (a one-particle cube) has no
+    constraint iteration to end its solids' substep: kg_bounds_velocity does
+    (solver.js:389-427, :219-234).  Alone and in a batch beside a constrained
+    scene it gives the same bits, and its solid comes to rest on the floor
+    at its radius."""
+    from fluidphysics_amd import Batch, Solver, from_scene
+    frames = 60
+
+    def lone():
+        s = Solver(512)
+        s.addSolidCube(0.3, 1.0, -0.2, 1, 0.18)
+        return s
+
+    log("case unconstrained scene alone and in a batch")
+    a = lone()
+    ref = []
+    for _ in range(frames):
+        a.step(DT, SUB)
+        ref.append(_snap(a))
+    r = a.solidRadius
+    y = ref[-1]["pos"][1]
+    assert abs(y - r) < 1e-6 and np.array_equal(ref[-1]["pos"], ref[-2]["pos"])
+    a.close()
+
+    other = _scenes([9], frames)[0]
+    ref_other = _alone(other, frames)
+    b = Batch()
+    s1, s2 = lone(), from_scene(other)
+    b.add(s1)
+    b.add(s2)
+    for f in range(frames):
+        _cuts(s2, other, f)
+        b.step(DT, SUB)
+        _same(_snap(s1), ref[f], "one-particle cube frame %d" % f)
+        if f % 10 == 9:
+            _same(_snap(s2), ref_other[f], "%s frame %d" % (other.name, f))
+    b.close()
