@@ -50,6 +50,16 @@
 #ifndef FLUID_BLOCK
 #define FLUID_BLOCK 256  // threads per block of the density, displacement and XSPH passes
 #endif
+// amdgpu_waves_per_eu(n) for a kernel (WAVES_ATTR(0): the allocator's choice)
+#define WAVES_ATTR_0
+#define WAVES_ATTR_45 __attribute__((amdgpu_waves_per_eu(4, 5)))
+#define WAVES_ATTR_5 __attribute__((amdgpu_waves_per_eu(5)))
+#define WAVES_ATTR_6 __attribute__((amdgpu_waves_per_eu(6)))
+#define WAVES_ATTR_7 __attribute__((amdgpu_waves_per_eu(7)))
+#define WAVES_ATTR_8 __attribute__((amdgpu_waves_per_eu(8)))
+#define WAVES_ATTR_(n) WAVES_ATTR_##n
+#define WAVES_ATTR(n) WAVES_ATTR_(n)
+
 #ifndef REPLAY_BATCH
 #define REPLAY_BATCH 4  // pair-list entries per step of the displacement and XSPH replay
 #endif
@@ -404,7 +414,10 @@ __device__ __forceinline__ void contacts_fluid_body(const SceneArgs& A, int lb) 
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) kg_contacts_fluid(const SceneArgs* __restrict__ S,
+#ifndef CONTACTS_FLUID_WAVES
+#define CONTACTS_FLUID_WAVES 0
+#endif
+__global__ void __launch_bounds__(BLOCK) WAVES_ATTR(CONTACTS_FLUID_WAVES) kg_contacts_fluid(const SceneArgs* __restrict__ S,
                                                            const int2* __restrict__ W, int nw) {
   FOR_ITEMS(S, W, nw) contacts_fluid_body(A, lb);
 }
@@ -813,14 +826,6 @@ __device__ __forceinline__ int fluid_block(int j, int o, int a) {
 #ifndef XSPH_WAVES
 #define XSPH_WAVES 8
 #endif
-#define WAVES_ATTR_0
-#define WAVES_ATTR_45 __attribute__((amdgpu_waves_per_eu(4, 5)))
-#define WAVES_ATTR_5 __attribute__((amdgpu_waves_per_eu(5)))
-#define WAVES_ATTR_6 __attribute__((amdgpu_waves_per_eu(6)))
-#define WAVES_ATTR_7 __attribute__((amdgpu_waves_per_eu(7)))
-#define WAVES_ATTR_8 __attribute__((amdgpu_waves_per_eu(8)))
-#define WAVES_ATTR_(n) WAVES_ATTR_##n
-#define WAVES_ATTR(n) WAVES_ATTR_(n)
 #define DENSITY_ATTR WAVES_ATTR(DENSITY_WAVES)
 
 __global__ void __launch_bounds__(FLUID_BLOCK) DENSITY_ATTR kg_density(const SceneArgs* __restrict__ S,
