@@ -47,14 +47,17 @@ __device__ __forceinline__ void constraints_body(const SceneArgs& A, int lb, int
   float cx = 0.f, cy = 0.f, cz = 0.f;
   bool tore = false;
   for (int k = k0 + lane; k < k1; k += CLANES) {
-    if (!gld(ealive + k)) continue;
+    // the entry's flag, neighbour and rest length load together, then the
+    // neighbour's position (a dead entry's neighbour index is still valid):
+    // two dependent loads instead of three
+    const bool alive = gld(ealive + k) != 0;
     const int2 e = gld(A.adj + k);
     const float rest = gld(erest + k);
     const float4 pb = gld(sin + e.x);
     const float w = pa.w + pb.w;
     const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
     const float d = sqrtf(dx * dx + dy * dy + dz * dz);
-    if (w != 0.f && d >= 1e-9f) {
+    if (alive && w != 0.f && d >= 1e-9f) {
       if (d > rest * cp.break_strain) {
         gst(ealive + k, (uint8_t)0);
         tore = true;
