@@ -378,8 +378,9 @@ struct fp_group {
   // result does not depend on it).  The first TUNE_FRAMES frames time 4, 8,
   // 16 and 32 lanes with events, rotating the candidate from substep to
   // substep (and the starting candidate from frame to frame) so that every
-  // candidate is timed on the same states; the fastest mean is kept.  A group
-  // whose solid count changes by more than 2x tunes again.
+  // candidate is timed on the same states; the fastest mean is kept, or the
+  // most lanes among the candidates within TUNE_TIE of it.  A group whose
+  // solid count changes by more than 2x tunes again.
   static constexpr int TUNE_FRAMES = 4;
   int clanes = 4;                 // lanes of the work lists' W_SOLID entries
   int items_clanes = 0;           // lanes the work lists were built for
