@@ -473,7 +473,7 @@ void fp_group_tune_begin(fp_group* g) {
 // frame (or sync), once they have completed.
 // A candidate within this factor of the fastest mean counts as tied.
 #ifndef TUNE_TIE
-#define TUNE_TIE 1.10
+#define TUNE_TIE 1.25
 #endif
 
 int fp_group_tune_end(fp_group* g) {
@@ -494,8 +494,9 @@ int fp_group_tune_end(fp_group* g) {
         best = c;
     // near-ties go to more lanes: the pass is timed alone, but a replayed
     // frame runs it beside the fluid chain, where a shorter chain per solid
-    // ends the side stream sooner (batched graph frame p50: 16.3 ms at 4
-    // lanes, 15.7 at 8, timed alone within a few % of each other)
+    // ends the side stream sooner (256 batched scenes: graph frame p50 16.3
+    // ms at 4 lanes, 15.7 at 8, 15.9 at 16; timed alone, 8 lanes are 6-12%
+    // slower than 4, 16 about 30%)
     if (best >= 0) {
       const double lim = TUNE_TIE * g->tune_sum[best] / g->tune_cnt[best];
       for (int c = 0; c < 4; c++)
