@@ -217,6 +217,20 @@ void setup_cell_grid(fp_solver* s) {
   s->lscan_blocks = (int)((s->cg.M + SCAN_TILE - 1) / SCAN_TILE);
 }
 
+// Distance constraints of an n^3 lattice (every neighbour within sqrt(3)
+// spacing, once): axis, face-diagonal and body-diagonal pairs.
+int64_t cube_constraints(int64_t n) {
+  return 3 * n * n * (n - 1) + 6 * n * (n - 1) * (n - 1) + 4 * (n - 1) * (n - 1) * (n - 1);
+}
+
+// A cube the 32-bit CSR can hold: capacity below 2^27 particles and both
+// endpoint entries of every constraint (2C) below INT_MAX (n <= 436).
+bool cube_fits(int64_t n, int64_t max_fluid) {
+  if (n < 1 || n > 1024 || max_fluid < 0) return false;
+  if (n * n * n + max_fluid >= (1ll << 27)) return false;
+  return 2 * cube_constraints(n) < (int64_t)INT_MAX;
+}
+
 // Particle counts and the reference table of a cube of side n.
 void scene_dims(fp_solver* s, int n) {
   s->ns = n * n * n;
@@ -524,9 +538,10 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
                       double particle_mass) {
   if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_add_solid_cube: NULL solver");
   if (s->has_cube) return fp_fail(s, FP_ESTATE, "addSolidCube may be called once");
-  if (n < 1 || n > 1024) return fp_fail(s, FP_EINVAL, "cube dimension out of range");
-  const int64_t cap64 = (int64_t)n * n * n + s->max_fluid;
-  if (cap64 >= (1ll << 27)) return fp_fail(s, FP_EINVAL, "capacity exceeds 2^27 particles");
+  if (!cube_fits(n, s->max_fluid))
+    return fp_fail(s, FP_EINVAL,
+                   "cube dimension out of range (n^3 + maxFluid must stay below 2^27 and 2 x constraints below "
+                   "INT_MAX: n <= 436)");
   int rc = set_device(s);
   if (rc) return rc;
   const int ns = n * n * n;
@@ -1303,15 +1318,12 @@ int fp_batch_profile_read(fp_batch* b, fp_pass_stat* out, int32_t max, int32_t* 
 }  // extern "C"
 
 int fp_scene_bytes(const fp_params* p, int32_t max_fluid, int32_t cube_n, int64_t* bytes) {
-  if (!p || !bytes || cube_n < 1 || cube_n > 1024 || max_fluid < 0) return FP_EINVAL;
-  if ((int64_t)cube_n * cube_n * cube_n + max_fluid >= (1ll << 27)) return FP_EINVAL;
+  if (!p || !bytes || !cube_fits(cube_n, max_fluid)) return FP_EINVAL;
   fp_solver tmp;
   tmp.P = *p;
   tmp.max_fluid = max_fluid;
   scene_dims(&tmp, cube_n);
-  // every lattice neighbour within sqrt(3) spacing (26-neighbourhood), once
-  const int64_t n = cube_n;
-  tmp.C = (int)(3 * n * n * (n - 1) + 6 * n * (n - 1) * (n - 1) + 4 * (n - 1) * (n - 1) * (n - 1));
+  tmp.C = (int)cube_constraints(cube_n);
   scene_tables(&tmp);
   Carve size;
   scene_buffers(&tmp, size);
@@ -1522,13 +1534,26 @@ int fp_state_import(fp_solver* s, const void* buf, int64_t bytes) {
   if (h.capacity != s->n || h.solid_count != s->ns || h.max_fluid != s->max_fluid || h.num_constraints != s->C)
     return fp_fail(s, FP_EINVAL, "fp_state_import: the blob is of a scene of another shape");
   if (bytes < need) return fp_fail(s, FP_EINVAL, "fp_state_import: blob shorter than its header says");
+  // the ring cursor addresses a fluid slot (fp_emit writes at solids + cursor)
+  if (h.fluid_next < 0 || h.fluid_next >= std::max(s->max_fluid, 1) || h.substeps_done < 0)
+    return fp_fail(s, FP_EINVAL, "fp_state_import: ring cursor or substep count out of range");
   if ((rc = set_device(s))) return rc;
   if ((rc = settle(s))) return rc;
   const char* in = (const char*)buf;
   size_t off = sizeof(h);
   hipError_t e = hipSuccess;
+  std::vector<uint8_t> flags;
   state_arrays(s, [&](void* p, size_t b) {
-    if (e == hipSuccess && b) e = h2d(s, p, in + off, b);
+    if (e == hipSuccess && b) {
+      if (p == (void*)s->active || p == (void*)s->ealive) {
+        // flags enter the device as 0 / 1 only
+        flags.assign(in + off, in + off + b);
+        for (uint8_t& f : flags) f = f ? 1 : 0;
+        e = h2d(s, p, flags.data(), b);
+      } else {
+        e = h2d(s, p, in + off, b);
+      }
+    }
     off += b;
   });
   FP_TRY(s, e);

@@ -85,3 +85,21 @@ def test_struct_offsets_match_the_c_compiler(struct, cls, tmp_path):
     assert lay.pop("size") == ctypes.sizeof(c)
     for f in fields:
         assert lay[f] == getattr(c, f).offset, f
+
+
+@pytest.mark.skipif(not os.path.exists(_abi.LIB_PATH), reason="library not built")
+def test_scene_bytes_bounds_the_cube_by_the_csr_width():
+    """2 x constraints of an n^3 lattice must fit an int32 CSR: n = 436 is the
+    largest cube (2C = 2,144,678,760 < INT_MAX); n = 437 gives 2,159,493,176
+    and is refused before anything is sized or allocated."""
+    lib = _abi.load_library()
+    p = _abi.FpParams()
+    lib.fp_params_default(ctypes.byref(p))
+    out = ctypes.c_int64(0)
+    c = lambda n: 3 * n * n * (n - 1) + 6 * n * (n - 1) ** 2 + 4 * (n - 1) ** 3
+    assert 2 * c(436) < 2 ** 31 - 1 < 2 * c(437)
+    assert lib.fp_scene_bytes(ctypes.byref(p), 0, 436, ctypes.byref(out)) == _abi.FP_OK
+    assert out.value > 2 * c(436) * 4  # at least the per-entry rest lengths
+    assert lib.fp_scene_bytes(ctypes.byref(p), 0, 437, ctypes.byref(out)) == _abi.FP_EINVAL
+    # the capacity bound (n^3 + maxFluid < 2^27) still holds
+    assert lib.fp_scene_bytes(ctypes.byref(p), (1 << 27) - 8, 2, ctypes.byref(out)) == _abi.FP_EINVAL

@@ -135,3 +135,37 @@ def test_recording_replays_to_the_same_state():
     _same(_state(r), a, "replay")
     s.close()
     r.close()
+
+
+def test_state_blob_with_bad_cursor_or_flags():
+    """A blob whose ring cursor lies outside [0, maxFluid) or whose substep
+    count is negative is refused before any copy (the cursor addresses the
+    next emit's slot); flag bytes other than 0 / 1 enter the device as 1."""
+    import struct
+    from fluidphysics_amd import FpError, from_scene
+    sc = _scene(63)
+    log("case blob validation: %s" % sc.name)
+    s = from_scene(sc)
+    s.step(DT, SUB)
+    blob = bytearray(s.save_state())
+    n = s.capacity
+    # header: magic[8], version u32, capacity, solids, maxFluid, constraints,
+    # fluid_next (i32 each), substeps_done i64, flags u32, reserved u32
+    assert struct.unpack_from("<iiiii", blob, 12)[:4] == (n, s.solidCount, s.maxFluid, s.info().num_constraints)
+    for cur, sub in ((-1, 0), (s.maxFluid, 0), (1 << 30, 0), (0, -5)):
+        bad = bytearray(blob)
+        struct.pack_into("<i", bad, 28, cur)
+        struct.pack_into("<q", bad, 32, sub)
+        with pytest.raises(FpError, match="out of range"):
+            s.load_state(bytes(bad))
+    # active flags of 7 load as 1
+    act_off = 48 + n * (16 * 3 + 8)
+    want = s.read_state()["active"].copy()
+    bad = bytearray(blob)
+    for i in range(n):
+        if bad[act_off + i]:
+            bad[act_off + i] = 7
+    s.load_state(bytes(bad))
+    assert np.array_equal(s.read_state()["active"], want)
+    assert set(np.unique(want)) <= {0, 1}
+    s.close()
