@@ -164,6 +164,7 @@ SIGNATURES = {
     "fp_write_constraints": (_I, [_P, _FP, _U8P]),
     "fp_build_grid": (_I, [_P]),
     "fp_read_grid": (_I, [_P, _IP, _IP]),
+    "fp_build_grid_noalloc": (_I, [_P, _P, _I, _I, _D, _P, _P, _P]),
     "fp_read_density": (_I, [_P, _FP, _FP]),
     "fp_profile_enable": (_I, [_P, _I]),
     "fp_profile_read": (_I, [_P, ctypes.POINTER(FpPassStat), _I, _IP]),

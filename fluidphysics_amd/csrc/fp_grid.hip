@@ -28,6 +28,10 @@
 // The hot-path kernels (kg_*) are group kernels: they stride over a work list
 // of (scene, block of that scene) items and read the scene's arrays from its
 // SceneArgs entry, so one launch builds the grids of every scene of a batch.
+#include <cmath>
+#include <mutex>
+#include <string>
+
 #include "fp_internal.h"
 #include "fp_kernels.h"
 
@@ -402,5 +406,112 @@ int fp_launch_ref_grid(fp_solver* s, bool rehash) {
                      s->entries_tmp, s->cell_entries);
   FP_TRY(s, hipGetLastError());
   s->ref_grid_valid = true;
+  return FP_OK;
+}
+
+// ---- the reference grid for callers that cannot allocate --------------------------
+//
+// fp_build_grid_noalloc: the same count -> scan -> scatter -> rank as
+// fp_launch_ref_grid, over positions and flags the caller holds in device
+// memory, with every intermediate (bucket and arrival slot per particle, the
+// arrival-order entries, the bucket counts and the scan's tile sums) in one
+// file-scope device array.  Each GPU holds its own copy of the array for the
+// life of the process; its device address is looked up once per device (the
+// array's name in host code is only a stand-in for it).  Calls that overlap
+// on one GPU share the array and race.
+
+namespace {
+constexpr size_t kNaMaxN = FP_GRID_NOALLOC_MAX_N;
+constexpr size_t kNaMaxT = FP_GRID_NOALLOC_MAX_T;
+constexpr size_t kNaTiles = (kNaMaxT + SCAN_TILE - 1) / SCAN_TILE;
+// words: phash[N], pslot[N], tmp[N], count[T], tile sums
+constexpr size_t kNaWords = 3 * kNaMaxN + kNaMaxT + kNaTiles;
+constexpr int kNaDevices = 64;
+std::mutex g_na_mutex;
+uint32_t* g_na_ptr[kNaDevices] = {nullptr};
+}  // namespace
+
+static __device__ __align__(256) uint32_t scratch[kNaWords];
+
+// The device address of this device's scratch array (looked up on the first
+// call per device, with that device current).
+static int noalloc_scratch(int device, uint32_t** out) {
+  std::lock_guard<std::mutex> lk(g_na_mutex);
+  if (!g_na_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fp_fail(nullptr, FP_EDEVICE,
+                     std::string("fp_build_grid_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_na_ptr[device] = (uint32_t*)p;
+  }
+  *out = g_na_ptr[device];
+  return FP_OK;
+}
+
+// Bucket of each particle from reference-layout positions (xyz triples).
+__global__ void __launch_bounds__(BLOCK) k_ref_hash_xyz(int n, double inv_cell, uint32_t mask,
+                                                        const float* __restrict__ pos,
+                                                        const uint8_t* __restrict__ active,
+                                                        uint32_t* __restrict__ phash) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  if (!active[i]) {
+    phash[i] = 0xffffffffu;
+    return;
+  }
+  const float* p = pos + 3 * (size_t)i;
+  phash[i] = cell_hash(cell_of(p[0], p[1], p[2], inv_cell), mask);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_zero_words(int n, int* __restrict__ p) {
+  const int i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+extern "C" int fp_build_grid_noalloc(const float* pos, const uint8_t* active, int32_t n, int32_t table_size,
+                                     double cell_size, int32_t* cell_start, int32_t* cell_entries, void* stream) {
+  if (!pos || !active || !cell_start || !cell_entries)
+    return fp_fail(nullptr, FP_EINVAL, "fp_build_grid_noalloc: NULL argument");
+  if (n < 0 || (size_t)n > kNaMaxN)
+    return fp_fail(nullptr, FP_EINVAL, "fp_build_grid_noalloc: n out of range [0, FP_GRID_NOALLOC_MAX_N = 1048576]");
+  if (table_size < 1 || (size_t)table_size > kNaMaxT || (table_size & (table_size - 1)) != 0)
+    return fp_fail(nullptr, FP_EINVAL,
+                   "fp_build_grid_noalloc: table_size must be a power of two in [1, FP_GRID_NOALLOC_MAX_T = 2097152]");
+  if (!(cell_size > 0.0) || !std::isfinite(cell_size))
+    return fp_fail(nullptr, FP_EINVAL, "fp_build_grid_noalloc: cell_size must be positive and finite");
+  int device = 0;
+  hipError_t e = hipGetDevice(&device);
+  if (e != hipSuccess) return fp_fail(nullptr, FP_EDEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (device < 0 || device >= kNaDevices) return fp_fail(nullptr, FP_EDEVICE, "fp_build_grid_noalloc: device ordinal");
+  uint32_t* base = nullptr;
+  int rc = noalloc_scratch(device, &base);
+  if (rc) return rc;
+  uint32_t* phash = base;
+  uint32_t* pslot = base + kNaMaxN;
+  int* tmp = (int*)(base + 2 * kNaMaxN);
+  int* count = (int*)(base + 3 * kNaMaxN);
+  int* tiles = (int*)(base + 3 * kNaMaxN + kNaMaxT);
+  const hipStream_t st = (hipStream_t)stream;
+  const int T = table_size, tb = (T + SCAN_TILE - 1) / SCAN_TILE;
+  // the counts start from zero whatever the last call left (a fill kernel,
+  // not a memset: the call may be captured into a graph)
+  hipLaunchKernelGGL(k_zero_words, dim3(fp_blocks(T)), dim3(BLOCK), 0, st, T, count);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_ref_hash_xyz, dim3(fp_blocks(n)), dim3(BLOCK), 0, st, n, 1.0 / cell_size,
+                       (uint32_t)(T - 1), pos, active, phash);
+    hipLaunchKernelGGL(k_ref_count, dim3(fp_blocks(n)), dim3(BLOCK), 0, st, n, phash, count, pslot);
+  }
+  hipLaunchKernelGGL(k_scan_reduce, dim3(tb), dim3(BLOCK), 0, st, T, count, tiles);
+  hipLaunchKernelGGL(k_scan_apply, dim3(tb), dim3(BLOCK), 0, st, T, count, tiles, (int*)cell_start);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_ref_scatter, dim3(fp_blocks(n)), dim3(BLOCK), 0, st, n, phash, pslot, (int*)cell_start,
+                       tmp);
+    hipLaunchKernelGGL(k_ref_rank, dim3(fp_blocks(n)), dim3(BLOCK), 0, st, n, phash, (int*)cell_start, tmp,
+                       (int*)cell_entries);
+  }
+  e = hipGetLastError();
+  if (e != hipSuccess)
+    return fp_fail(nullptr, FP_EDEVICE, std::string("fp_build_grid_noalloc: launch: ") + hipGetErrorString(e));
   return FP_OK;
 }

@@ -225,6 +225,30 @@ int fp_build_grid(fp_solver* s);
  * first cellStart[table_size] are defined) of the last grid build. */
 int fp_read_grid(fp_solver* s, int32_t* cell_start, int32_t* cell_entries);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): _buildGrid (solver.js:174-184) -- the same kernels and
+ * bit-exact result as fp_build_grid + fp_read_grid -- over arrays the caller
+ * holds in device memory, allocating nothing.  Every intermediate lives in
+ * one static device array of fixed size (~21 MB per GPU, held until the
+ * process ends), so n <= FP_GRID_NOALLOC_MAX_N and table_size <=
+ * FP_GRID_NOALLOC_MAX_T; larger sizes return FP_EINVAL.
+ *   pos: 3n floats (x, y, z per particle), active: n bytes (0 = skipped),
+ *   cell_start: table_size + 1 int32 (bucket k holds entries
+ *   [cell_start[k], cell_start[k+1]), descending particle index),
+ *   cell_entries: n int32 (the first cell_start[table_size] defined);
+ *   table_size a power of two; cell_size the reference's cellSize.
+ * All pointers are device memory of the current device.  The work is
+ * enqueued on `stream` (a hipStream_t; NULL = the null stream) and the call
+ * returns without waiting.  Calls on one GPU share the static array: calls
+ * that overlap in time on that GPU race (serialise them, e.g. on one stream).
+ * The first call on each device looks up the array's device address (not a
+ * stream operation); nothing else is allocated or freed. */
+#define FP_GRID_NOALLOC_MAX_N (1 << 20)
+#define FP_GRID_NOALLOC_MAX_T (1 << 21)
+int fp_build_grid_noalloc(const float* pos, const uint8_t* active, int32_t n, int32_t table_size, double cell_size,
+                          int32_t* cell_start, int32_t* cell_entries, void* stream);
+
 /* rho / rhoNear (capacity floats each; fluid slots) of the last substep's
  * density pass (solver.js:316-317).  Either pointer may be NULL. */
 int fp_read_density(fp_solver* s, float* rho, float* rho_near);

@@ -103,3 +103,22 @@ def test_scene_bytes_bounds_the_cube_by_the_csr_width():
     assert lib.fp_scene_bytes(ctypes.byref(p), 0, 437, ctypes.byref(out)) == _abi.FP_EINVAL
     # the capacity bound (n^3 + maxFluid < 2^27) still holds
     assert lib.fp_scene_bytes(ctypes.byref(p), (1 << 27) - 8, 2, ctypes.byref(out)) == _abi.FP_EINVAL
+
+
+@pytest.mark.skipif(not os.path.exists(_abi.LIB_PATH), reason="library not built")
+def test_noalloc_grid_refuses_sizes_beyond_its_array_without_a_device():
+    """fp_build_grid_noalloc checks its sizes before it touches a device: n
+    above FP_GRID_NOALLOC_MAX_N, a table above FP_GRID_NOALLOC_MAX_T or not a
+    power of two, a bad cell size or a NULL pointer give FP_EINVAL."""
+    lib = _abi.load_library()
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: every call below is refused first
+    call = lambda n, t, cs=0.22, p=fake: lib.fp_build_grid_noalloc(p, fake, n, t, cs, fake, fake, None)
+    assert call((1 << 20) + 1, 1 << 21) == _abi.FP_EINVAL
+    assert b"n out of range" in lib.fp_last_error(None)
+    assert call(-1, 1024) == _abi.FP_EINVAL
+    assert call(100, (1 << 21) * 2) == _abi.FP_EINVAL
+    assert call(100, 1000) == _abi.FP_EINVAL
+    assert b"power of two" in lib.fp_last_error(None)
+    assert call(100, 1024, cs=0.0) == _abi.FP_EINVAL
+    assert call(100, 1024, cs=float("nan")) == _abi.FP_EINVAL
+    assert call(100, 1024, p=None) == _abi.FP_EINVAL
