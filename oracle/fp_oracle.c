@@ -773,13 +773,20 @@ void or_recompute_groups(Oracle* o) {
 
 /* ---- the step (solver.js:187-242) ------------------------------------ */
 
-void or_substep(Oracle* o, double h) {
-  const double invH = 1 / h;
+/* Integrate (solver.js:194-202): v.y += g h; prev = p; p += v h for every
+ * active particle.  The x and z components of gravity (a tilted basin; the
+ * reference has y only, :15, :197) are added the same way when set -- an
+ * unset component adds nothing, so a zero velocity keeps its sign bit and
+ * the float32 build stays bit-identical to solver.js. */
+static void integrate(Oracle* o, double h) {
   ST *p = o->pos, *pv = o->prev, *v = o->vel;
+  const double gx = o->P.gravity_x, gz = o->P.gravity_z;
   for (int i = 0; i < o->capacity; i++) {
     if (!o->active[i]) continue;
     const int i3 = i * 3;
+    if (gx != 0) v[i3] = (ST)(v[i3] + gx * h);
     v[i3 + 1] = (ST)(v[i3 + 1] + o->P.gravity * h);
+    if (gz != 0) v[i3 + 2] = (ST)(v[i3 + 2] + gz * h);
     pv[i3] = p[i3];
     pv[i3 + 1] = p[i3 + 1];
     pv[i3 + 2] = p[i3 + 2];
@@ -787,6 +794,12 @@ void or_substep(Oracle* o, double h) {
     p[i3 + 1] = (ST)(p[i3 + 1] + (double)v[i3 + 1] * h);
     p[i3 + 2] = (ST)(p[i3 + 2] + (double)v[i3 + 2] * h);
   }
+}
+
+void or_substep(Oracle* o, double h) {
+  const double invH = 1 / h;
+  ST *p = o->pos, *pv = o->prev, *v = o->vel;
+  integrate(o, h);
   for (int g = 0; g < o->nGrabs; g++) {
     const int i3 = o->grabI[g] * 3;
     const double k = o->P.grab_stiffness;
@@ -835,23 +848,32 @@ void or_substep(Oracle* o, double h) {
 /* Test hook: the first stages of a substep (integrate, grab, grid, contacts),
  * leaving the post-contact positions in pos -- what the fluid passes see. */
 void or_substep_until_contacts(Oracle* o, double h) {
-  ST *p = o->pos, *pv = o->prev, *v = o->vel;
-  for (int i = 0; i < o->capacity; i++) {
-    if (!o->active[i]) continue;
-    const int i3 = i * 3;
-    v[i3 + 1] = (ST)(v[i3 + 1] + o->P.gravity * h);
-    pv[i3] = p[i3];
-    pv[i3 + 1] = p[i3 + 1];
-    pv[i3 + 2] = p[i3 + 2];
-    p[i3] = (ST)(p[i3] + (double)v[i3] * h);
-    p[i3 + 1] = (ST)(p[i3 + 1] + (double)v[i3 + 1] * h);
-    p[i3 + 2] = (ST)(p[i3 + 2] + (double)v[i3 + 2] * h);
-  }
+  integrate(o, h);
   or_build_grid(o);
   if (o->mode == OR_MODE_JACOBI)
     solve_contacts_jacobi(o);
   else
     solve_contacts_gs(o);
+}
+
+/* Test hooks: one pass alone on the current state, in the current mode --
+ * the contacts over the grid of the current positions, or `iters`
+ * constraint sweeps / iterations with compliance for substep h. */
+void or_solve_contacts(Oracle* o) {
+  or_build_grid(o);
+  if (o->mode == OR_MODE_JACOBI)
+    solve_contacts_jacobi(o);
+  else
+    solve_contacts_gs(o);
+}
+
+void or_solve_constraints(Oracle* o, double h, int iters) {
+  for (int it = 0; it < iters; it++) {
+    if (o->mode == OR_MODE_JACOBI)
+      solve_constraints_jacobi_iter(o, h);
+    else
+      solve_constraints_gs(o, h);
+  }
 }
 
 void or_step(Oracle* o, double dt, int substeps) {

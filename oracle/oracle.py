@@ -54,6 +54,8 @@ def _lib(storage):
         "or_build_grid": (None, [P]),
         "or_substep": (None, [P, D]),
         "or_substep_until_contacts": (None, [P, D]),
+        "or_solve_contacts": (None, [P]),
+        "or_solve_constraints": (None, [P, D, I]),
         "or_step": (None, [P, D, I]),
         "or_recompute_groups": (None, [P]),
         "or_cut": (I, [P, D, D, D, D, D, D, D, D, D]),
@@ -154,6 +156,16 @@ class OracleSolver:
 
     def substep_until_contacts(self, h):
         self.lib.or_substep_until_contacts(self.h, h)
+
+    def solve_contacts(self):
+        """The contact pass alone (grid of the current positions, then
+        contacts in the current mode)."""
+        self.lib.or_solve_contacts(self.h)
+
+    def solve_constraints(self, h, iters=1):
+        """`iters` constraint sweeps (gs) or iterations (jacobi) alone, with
+        the compliance of substep h."""
+        self.lib.or_solve_constraints(self.h, h, int(iters))
 
     def step(self, dt=1.0 / 60.0, substeps=5):
         self.lib.or_step(self.h, dt, int(substeps))
