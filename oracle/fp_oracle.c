@@ -796,10 +796,9 @@ static void integrate(Oracle* o, double h) {
   }
 }
 
-void or_substep(Oracle* o, double h) {
-  const double invH = 1 / h;
-  ST *p = o->pos, *pv = o->prev, *v = o->vel;
-  integrate(o, h);
+/* Grab pull (solver.js:205-210): every {i, x, y, z} in list order. */
+static void apply_grabs(Oracle* o) {
+  ST* p = o->pos;
   for (int g = 0; g < o->nGrabs; g++) {
     const int i3 = o->grabI[g] * 3;
     const double k = o->P.grab_stiffness;
@@ -807,6 +806,13 @@ void or_substep(Oracle* o, double h) {
     p[i3 + 1] = (ST)(p[i3 + 1] + (o->grabXYZ[3 * g + 1] - p[i3 + 1]) * k);
     p[i3 + 2] = (ST)(p[i3 + 2] + (o->grabXYZ[3 * g + 2] - p[i3 + 2]) * k);
   }
+}
+
+void or_substep(Oracle* o, double h) {
+  const double invH = 1 / h;
+  ST *p = o->pos, *pv = o->prev, *v = o->vel;
+  integrate(o, h);
+  apply_grabs(o);
   or_build_grid(o);
   if (o->mode == OR_MODE_JACOBI)
     solve_contacts_jacobi(o);
@@ -849,6 +855,7 @@ void or_substep(Oracle* o, double h) {
  * leaving the post-contact positions in pos -- what the fluid passes see. */
 void or_substep_until_contacts(Oracle* o, double h) {
   integrate(o, h);
+  apply_grabs(o);
   or_build_grid(o);
   if (o->mode == OR_MODE_JACOBI)
     solve_contacts_jacobi(o);

@@ -81,7 +81,7 @@ def test_grid_matches_oracle_at_scale(scene_name, frames):
 AMBIGUOUS_D = 2e-6  # |d - H| below this: float32 position rounding can flip the pair
 
 
-def _ambiguous(o_state, H, params, max_fluid, cube, substep):
+def _ambiguous(o_state, H, params, max_fluid, cube, substep, grabs=None):
     """Fluid particles with a neighbour at |d - H| < AMBIGUOUS_D after the
     contact pass of the FP64 replay.  XSPH counts every pair inside H
     (solver.js:440-445), so whether such a pair is in the list is decided by
@@ -94,6 +94,8 @@ def _ambiguous(o_state, H, params, max_fluid, cube, substep):
     o.add_solid_cube(*cube)
     o.write_state(*o_state["state"])
     o.write_constraints(*o_state["cons"])
+    if grabs is not None:
+        o.set_grabs(*grabs)
     o.substep_until_contacts(substep)
     st = o.read_state()
     p = o.read_state64()["pos"].reshape(-1, 3)

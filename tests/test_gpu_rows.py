@@ -1,0 +1,324 @@
+"""GPU parity for the rows and configurations the scene-level tests leave
+open: the grab pull (src/solver.js:205-210), resetSolid / clearFluid
+(:125-153), the knife on the 20^3 cube (:458-486), tilted gravity (the
+batched scenes), and the batched (256 scenes) and serving (120 Hz Poisson
+stream) workloads at their stated sizes.
+
+Bars as in test_gpu_parity: one substep within 1e-4 * h of the FP64 oracle
+(Jacobi form, pinned in tests/test_oracle_pins.py); alive flags and cut
+counts bit-exact; a scene in a batch or a serving mix bit-identical to the
+same scene alone.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from tests import _golden as G
+from tests._gpu import log, make_gpu, tol_report
+from tests.test_gpu_parity import SUB, _ambiguous, _compare_substep
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 60.0
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _lib():
+    from fluidphysics_amd import _abi
+    _abi.load_library()
+
+
+def _state_of(g):
+    st = g.read_state()
+    c = g.read_constraints()
+    return dict(state=(st["pos"], st["prev"], st["vel"], st["inv_mass"], st["active"]),
+                cons=(c["rest"].astype(np.float64), c["alive"]))
+
+
+def _oracle(sc, state, storage, mode="jacobi", grabs=None, **overrides):
+    from oracle.oracle import OracleSolver
+    o = OracleSolver(sc.params_with(**overrides), sc.max_fluid, storage, mode)
+    o.add_solid_cube(*sc.cube)
+    o.write_state(*state["state"])
+    o.write_constraints(*state["cons"])
+    if grabs is not None:
+        o.set_grabs(*grabs)
+    return o
+
+
+def _one_substep_vs_oracle(g, sc, label, grabs=None, step=None):
+    """One substep of g (step(): how to run it) against the FP64 and float32
+    oracles from g's current state."""
+    state = _state_of(g)
+    params = sc.params_with()
+    oracles = {}
+    for storage in ("f64", "f32"):
+        o = _oracle(sc, state, storage, grabs=grabs)
+        o.step(SUB, 1)
+        oracles[storage] = o
+    amb = _ambiguous(state, params["fluid_h"], params, sc.max_fluid, sc.cube, SUB, grabs=grabs)
+    if step is None:
+        g.step(SUB, 1)
+    else:
+        step()
+    return _compare_substep(g, oracles["f64"], params["fluid_h"], label, amb=amb, o32=oracles["f32"]), state
+
+
+# ------------------------------------------------------------------ grab ----
+
+def test_grab_substep_vs_fp64_jacobi():
+    """64 solids nearest the grab point of the 20^3 scene pulled to their
+    sinusoidal targets, plus a second target for the first of them later in
+    the list (the reference pulls in list order, :205-210): one substep
+    against the FP64 oracle; the repeated index's order is checked by the
+    swapped list, which must not match."""
+    from fluidphysics_amd import from_scene, scenes
+    sc = scenes.single_gpu_scene(frames=60)
+    g = from_scene(sc)
+    for _ in range(20):
+        g.step(DT, sc.substeps)
+    ns = g.solidCount
+    pos = g.read_state()["pos"].reshape(-1, 3)[:ns]
+    ids = np.argsort(np.linalg.norm(pos - np.asarray(sc.grab["point"]), axis=1), kind="stable")[:64]
+    tg = scenes.grab_targets(sc, sc.grab["frames"][0] + 7, ids, pos[ids].astype(np.float64))
+    idx = np.concatenate([ids, ids[:1]]).astype(np.int32)
+    xyz = np.concatenate([tg, tg[:1] + np.array([0.02, 0.01, -0.015])])
+    log("case grab substep %s: %d grabs, repeated index %d" % (sc.name, len(idx), idx[0]))
+    g.grabs = [{"i": int(i), "x": float(p[0]), "y": float(p[1]), "z": float(p[2])} for i, p in zip(idx, xyz)]
+    _, state = _one_substep_vs_oracle(g, sc, "grab", grabs=(idx, xyz))
+    # the list in the other order (second target first) ends elsewhere
+    sw = np.concatenate([idx[-1:], idx[:-1]])
+    swx = np.concatenate([xyz[-1:], xyz[:-1]])
+    o = _oracle(sc, state, "f64", grabs=(sw, swx))
+    o.step(SUB, 1)
+    got = g.read_state()["pos"].reshape(-1, 3)[idx[0]].astype(np.float64)
+    other = o.read_state64()["pos"].reshape(-1, 3)[idx[0]]
+    assert np.abs(got - other).max() > 100 * 1e-4 * sc.params_with()["fluid_h"]
+    g.close()
+
+
+# --------------------------------------------------------- reset / clear ----
+
+def test_reset_solid_and_clear_fluid_vs_oracle():
+    """After frames with a cut and a tearing grab, resetSolid + clearFluid
+    restore the reference's state (:125-137, :151-153): lattice positions and
+    prev bit-exact, solid velocities zero, rest lengths of the set-up, every
+    constraint alive, groups 0, no fluid, grabs cleared.  A fresh block of
+    fluid then steps one substep within 1e-4 * h of the FP64 replay."""
+    from fluidphysics_amd import scenes
+    log("case resetSolid / clearFluid plain_cpu")
+    d = G.scenario("plain_cpu")
+    g = make_gpu(d, iters=3, omega=8.0)
+    w = scenes.plain_cpu_scene().wedges[0]
+    for f in range(20):
+        if f == 6:
+            assert g.cut(*w.o, *w.d1, *w.d2) > 0
+        if f == 8:
+            g.grabs = [{"i": i, "x": 2.5, "y": 3.5, "z": 0.0} for i in range(7)]
+        g.step(DT, 5)
+    c0 = g.read_constraints()
+    assert c0["alive"].sum() < len(c0["alive"])
+    assert len(np.unique(g.read_state()["group"])) > 1
+    g.resetSolid()
+    g.clearFluid()
+    assert g.grabs == [] and g.fluidCount() == 0
+    o = G.make_oracle(d, storage="f32", mode="gs")
+    o.reset_solid()
+    o.clear_fluid()
+    st, ost = g.read_state(), o.read_state()
+    ns = g.solidCount
+    assert np.array_equal(st["pos"][: 3 * ns], ost["pos"][: 3 * ns])
+    assert np.array_equal(st["prev"][: 3 * ns], ost["prev"][: 3 * ns])
+    assert not st["vel"][: 3 * ns].any()
+    assert not st["active"][ns:].any() and np.array_equal(st["active"], ost["active"])
+    assert not st["group"].any()
+    c, oc = g.read_constraints(), o.read_constraints()
+    assert c["alive"].all() and np.array_equal(c["alive"], oc["alive"])
+    assert np.array_equal(c["rest"], oc["rest"].astype(np.float32))
+    # stepping on: a fresh block of fluid beside the restored cube
+    o64 = G.make_oracle(d, storage="f64", mode="gs")
+    o64.write_state(st["pos"], st["prev"], st["vel"], st["inv_mass"], st["active"])
+    o64.write_constraints(c["rest"].astype(np.float64), c["alive"])
+    rng = np.random.default_rng(17)
+    blk = np.array([1.2, 0.3, -0.6]) + rng.uniform(0.0, 1.0, (200, 3)) * np.array([1.0, 0.8, 1.2])
+    for s in (g, o64):
+        if s is g:
+            s.emit_batch(blk, np.zeros_like(blk), 0.03)
+        else:
+            s.emit(blk, np.zeros_like(blk), 0.03)
+    assert g.fluidCount() == o64.fluid_count() == 200
+    g.step(SUB, 1)
+    o64.step(SUB, 1)
+    _compare_substep(g, o64, 0.19, "after reset")
+    g.close()
+
+
+# ------------------------------------------------------------ the knife ----
+
+def test_config2_wedges_bit_exact():
+    """The 20^3 scene's two seeded wedges (frames 150 and 350) on its 93,556
+    constraints: cut counts and alive flags equal the oracle's double
+    predicate evaluated on the same float32 positions."""
+    from fluidphysics_amd import from_scene, scenes
+    from oracle.oracle import OracleSolver
+    sc = scenes.single_gpu_scene()
+    log("case config-2 wedges %s at frames %s" % (sc.name, [w.frame for w in sc.wedges]))
+    g = from_scene(sc)
+    o = OracleSolver(sc.params_with(), sc.max_fluid, "f32", "jacobi")
+    o.add_solid_cube(*sc.cube)
+    assert g.info().num_constraints == 93556
+    f, total = 0, 0
+    for w in sc.wedges:
+        while f < w.frame:
+            g.step(sc.dt, sc.substeps)
+            f += 1
+        st = _state_of(g)
+        o.write_state(*st["state"])
+        o.write_constraints(*st["cons"])
+        ng = g.cut(*w.o, *w.d1, *w.d2)
+        no = o.cut(w.o, w.d1, w.d2)
+        log("  frame %d: %d cut (oracle %d)" % (f, ng, no))
+        assert ng == no
+        assert np.array_equal(g.read_constraints()["alive"], o.read_constraints()["alive"])
+        total += ng
+    assert total > 0
+    g.close()
+
+
+# -------------------------------------------------------- tilted gravity ----
+
+def _tilt_deg(sc):
+    p = sc.params_with()
+    return np.degrees(np.arctan2(np.hypot(p["gravity_x"], p["gravity_z"]), -p["gravity"]))
+
+
+def test_tilted_batched_scene_one_substep_vs_fp64_jacobi():
+    """A batched scene with gravity tilted by more than 10 degrees, stepped in
+    a batch with two others for 8 frames (a slice among them), then one
+    substep of the batch against the FP64 oracle with the same tilt."""
+    from fluidphysics_amd import Batch, from_scene, scenes
+    k = next(k for k in range(64) if _tilt_deg(scenes.batched_scene(k, frames=12)) > 10.0)
+    scs = [scenes.batched_scene(j, frames=12) for j in (k, k + 1, k + 2)]
+    for j, w in enumerate(scs[0].wedges):
+        w.frame = 3 + 2 * j
+    log("case tilted batch scene %s (tilt %.1f deg)" % (scs[0].name, _tilt_deg(scs[0])))
+    b = Batch()
+    ss = [from_scene(sc) for sc in scs]
+    for s in ss:
+        b.add(s)
+    for f in range(8):
+        for s, sc in zip(ss, scs):
+            for w in sc.wedges:
+                if w.frame == f:
+                    s.cut(*w.o, *w.d1, *w.d2)
+        b.step(DT, 5)
+    _one_substep_vs_oracle(ss[0], scs[0], "tilted", step=lambda: b.step(SUB, 1))
+    b.close()
+    for s in ss:
+        s.close()
+
+
+# ------------------------------------------- the batched workload, full size ----
+
+KEYS = ("pos", "prev", "vel", "active")
+
+
+def _snap(s):
+    st = s.read_state()
+    c = s.read_constraints()
+    return {k: st[k].copy() for k in KEYS} | {"rest": c["rest"], "alive": c["alive"]}
+
+
+def _cuts(s, sc, f):
+    for w in sc.wedges:
+        if w.frame == f:
+            s.cut(*w.o, *w.d1, *w.d2)
+
+
+def test_full_batched_workload_sample_alone_and_vs_oracle():
+    """All 256 scenes of the batched workload (seed 4, 5.06M particle slots)
+    as one batch for 8 frames (tuning, eager and replayed graph frames; every
+    scene's slices moved into them): 8 seeded scenes bit-identical to the
+    same scenes stepped alone, and one further substep of one of them within
+    1e-4 * h of the FP64 oracle."""
+    from fluidphysics_amd import Batch, from_scene, scenes
+    F = 8
+    scs = scenes.batched_scenes(256, 4, frames=600)
+    for k, sc in enumerate(scs):
+        for j, w in enumerate(sc.wedges):
+            w.frame = 2 + 2 * j + k % 3
+    sample = sorted(np.random.default_rng(404).choice(256, 8, replace=False).tolist())
+    log("case full batch: 256 scenes, %d slots; sample %s" % (sum(sc.capacity for sc in scs), sample))
+    b = Batch()
+    ss = [from_scene(sc) for sc in scs]
+    for s in ss:
+        b.add(s)
+    for f in range(F):
+        for s, sc in zip(ss, scs):
+            _cuts(s, sc, f)
+        b.step(DT, 5)
+    inf = b.info()
+    assert inf.graph_launches >= 2, "no replayed frame"
+    got = {k: _snap(ss[k]) for k in sample}
+    cut = sum(int((got[k]["alive"] == 0).sum()) for k in sample)
+    log("  graph frames %d; constraints cut in the sample %d" % (inf.graph_launches, cut))
+    assert cut > 0
+    # one more substep of the whole batch, scene sample[0] against the oracle
+    j = sample[0]
+    _one_substep_vs_oracle(ss[j], scs[j], scs[j].name, step=lambda: b.step(SUB, 1))
+    b.close()
+    for s in ss:
+        s.close()
+    for k in sample:
+        s = from_scene(scs[k])
+        for f in range(F):
+            _cuts(s, scs[k], f)
+            s.step(DT, 5)
+        alone = _snap(s)
+        s.close()
+        for key in got[k]:
+            assert np.array_equal(got[k][key], alone[key]), "%s: %s differs alone" % (scs[k].name, key)
+
+
+# ----------------------------------------------- the serving stream, full rate ----
+
+def test_serving_stream_sample_alone_and_in_the_mix():
+    """The serving workload's open-loop Poisson stream (120 requests/s, seed
+    5) served for 2 s of wall clock with continuous batching; every frame
+    delivered to 6 seeded requests equals, bit for bit, the same request
+    served alone."""
+    from fluidphysics_amd import scenes
+    from fluidphysics_amd.scenes import Request
+    from fluidphysics_amd.serving import Server
+    reqs = scenes.serving_requests(120.0, 2.0, seed=5)
+    early = [r for r in reqs if r.arrival < 1.0]
+    pick = np.random.default_rng(505).choice(len(early), 4, replace=False)
+    sample = {early[i].id: early[i] for i in pick}
+    # and the two longest horizons among the first half second's arrivals
+    # (they run through most of the stream beside many others)
+    for r in sorted([r for r in early if r.arrival < 0.5], key=lambda r: -r.horizon)[:2]:
+        sample[r.id] = r
+    log("case serving stream: %d requests over 2 s; sample %s" % (len(reqs), sorted(sample)))
+    dig = {i: {} for i in sample}
+
+    def on_frame(req, k, pos):
+        if req.id in dig:
+            dig[req.id][k] = hashlib.blake2b(pos.tobytes(), digest_size=16).digest()
+    rep = Server(mode="batch", on_frame=on_frame).run(reqs, duration=2.0)
+    s = rep.summary()
+    log("  served: %s" % {k: s[k] for k in ("frames", "admitted", "completed", "peak_scenes", "frame_ms_p99")})
+    assert s["peak_scenes"] >= 8
+    for rid, r in sample.items():
+        n = len(dig[rid])
+        assert n > 0 and sorted(dig[rid]) == list(range(n))
+        alone = {}
+
+        def cb(req, k, pos, alone=alone):
+            alone[k] = hashlib.blake2b(pos.tobytes(), digest_size=16).digest()
+        solo = Request(id=rid, arrival=0, index=r.index, seed=r.seed, horizon=r.horizon)
+        Server(mode="batch", on_frame=cb).run([solo], by_frame=True, max_frames=n)
+        log("  request %d (%s): %d frames in the mix" % (rid, r.scene().name, n))
+        assert len(alone) == n
+        for k in range(n):
+            assert alone[k] == dig[rid][k], "request %d frame %d differs alone" % (rid, k)
