@@ -24,9 +24,15 @@ sustained throughput beside the per-scene path on the same stream (the
 `serving` object; --no-serving skips it).
 
 Printed: one JSON line with the metric, a `roofline` object for the dominant
-kernel (algorithmic bytes per launch / average launch time from HIP events
-on the solver's stream) and a `cpu_baseline` object (the oracle -- the
-plain-C restatement of src/solver.js -- on one host core, bounded sample).
+kernel and a `cpu_baseline` object.  The roofline's launch time is measured
+over the timed frames themselves: the state at the start of the timed region
+is kept in a device checkpoint, and after the timed run the same frames are
+replayed from it with HIP events around every pass on the solver's stream
+(the replay is bit-identical to the timed run, which the line reports).
+cpu_baseline is the oracle -- the plain-C restatement of src/solver.js,
+Gauss-Seidel mode, float32 stores -- on a bounded sample of the same
+workload: one host core, and one independent copy per core of the box's CPU
+share (aggregate), both run before the GPU is touched.
 """
 import argparse
 import json
@@ -44,6 +50,11 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (scaled by a plain copy kernel of known size); see the file for commands.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "large_hbm_traffic.json")
+# Speed of the oracle port over the reference src/solver.js under node, same
+# frames, one thread each (oracle/time_reference.py in the development
+# container; profiles/cpu_reference_timing.md): a port rate divided by this
+# is the reference's own rate on the same host.
+PORT_OVER_SOLVER_JS = {}
 
 
 def model_bytes_per_substep(n, nf, ns, c, t, iters):
@@ -141,6 +152,37 @@ def cpu_baseline(scene, budget_s=12.0, max_frames=50):
                       % (frames, scene.substeps, scene.name, el, os.cpu_count() or 0)}
 
 
+def _cpu_worker(args):
+    name, budget_s, max_frames = args
+    from fluidphysics_amd import scenes
+    return cpu_baseline(scenes.SCENES[name](), budget_s, max_frames)
+
+
+def host_cores():
+    """The CPU share this process may use (affinity), at most 16 (the share
+    of one GPU on the box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline_all_cores(workload, budget_s=10.0, max_frames=50):
+    """One independent copy of the same sample per host core, run together
+    (the reference solver is single-threaded; this is the host's aggregate
+    rate on independent scenes).  Forked before the GPU is initialised."""
+    import multiprocessing as mp
+    n = host_cores()
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(n) as pool:
+        outs = pool.map(_cpu_worker, [(workload, budget_s, max_frames)] * n)
+    el = time.perf_counter() - t0
+    return {"value": round(sum(o["value"] for o in outs), 1), "unit": "particle-substeps/s", "cores": n,
+            "kind": "port", "processes": n, "wall_s": round(el, 2),
+            "sample": "%d independent copies of the 1-core sample, one per core, run at once (aggregate)" % n}
+
+
 def _pct(xs, q):
     xs = sorted(xs)
     return xs[min(len(xs) - 1, int(math.ceil(q * len(xs))) - 1)]
@@ -177,6 +219,8 @@ def batched_leg(count, frames, warmup, solo_frames, seed=4):
         cuts(f)
         b.step(dt, sub)
     n0 = active()
+    for s in ss:
+        s.checkpoint(0)
     lat = []
     for f in range(warmup, warmup + frames):
         t = time.perf_counter()
@@ -185,11 +229,25 @@ def batched_leg(count, frames, warmup, solo_frames, seed=4):
         lat.append(time.perf_counter() - t)
     n1 = active()
     inf = b.info()
+    n_mean = 0.5 * (n0 + n1)
+    value = n_mean * sub * frames / sum(lat)
+    # the timed frames again from the checkpoints, with events around every pass
+    for s in ss:
+        s.restore(0)
+    b.profile(True)
+    for f in range(warmup, warmup + frames):
+        cuts(f)
+        b.step(dt, sub)
+    b.sync()
+    stats = b.profile_read()
+    b.profile(False)
+    ns_t = sum(s.solidCount for s in ss)
+    infos = [s.info() for s in ss]
+    roof, _ = roofline_of(stats, frames, n_mean, n_mean - ns_t, ns_t, sum(i.num_constraints for i in infos),
+                          sum(i.table_size for i in infos), first_frame=warmup)
     for s in ss:
         b.remove(s)
     b.close()
-    n_mean = 0.5 * (n0 + n1)
-    value = n_mean * sub * frames / sum(lat)
 
     solo = None
     if solo_frames > 0:
@@ -226,6 +284,7 @@ def batched_leg(count, frames, warmup, solo_frames, seed=4):
         "frame_ms_p50": round(_pct(lat, 0.50) * 1e3, 3),
         "frame_ms_p99": round(_pct(lat, 0.99) * 1e3, 3),
         "graph_launches": int(inf.graph_launches),
+        "roofline": roof,
         "device_bytes": int(inf.device_bytes),
         "setup_s": round(setup_s, 2),
         "one_at_a_time": solo,
@@ -252,13 +311,42 @@ def serving_leg(rate_hz, duration_s, budget_bytes, seed=5):
     return out
 
 
+def roofline_of(stats, frames, n_mean, nf, ns, c, t, traffic=None, first_frame=None):
+    """The dominant pass of per-pass event statistics: algorithmic bytes per
+    launch / average launch time, against HBM's 8 TB/s, plus every pass's
+    figures."""
+    passes = {}
+    dom, dom_ms = None, -1.0
+    for name, (launches, total_ms) in stats.items():
+        avg = total_ms / launches
+        passes[name] = {"launches": launches, "avg_us": round(avg * 1e3, 3),
+                        "ms_per_frame": round(total_ms / frames, 4)}
+        b = pass_bytes(name, n_mean, nf, ns, c, t)
+        if b is not None:
+            passes[name]["gbs"] = round(b / (avg * 1e-3) / 1e9, 1)
+            if total_ms > dom_ms:
+                dom, dom_ms = name, total_ms
+    launches, total_ms = stats[dom]
+    avg_s = total_ms / launches * 1e-3
+    bytes_dom = pass_bytes(dom, n_mean, nf, ns, c, t)
+    achieved = bytes_dom / avg_s / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "kg_" + dom,
+            "bytes_per_launch": int(bytes_dom), "avg_launch_us": round(avg_s * 1e6, 3), "launches": launches,
+            "timed_frames": [first_frame, first_frame + frames] if first_frame is not None else None,
+            "timing": "HIP events around the pass on its stream, over a replay of the timed frames from a "
+                      "device checkpoint taken at their start (eager launches)"}
+    if traffic:
+        roof["traffic_gbs"] = round(traffic / avg_s / 1e9, 1)
+    return roof, passes
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="large", choices=["large", "single_gpu", "plain_cpu"])
-    ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=60)
@@ -277,6 +365,17 @@ def main():
 
     from fluidphysics_amd import from_scene, scenes
     scene = scenes.SCENES[args.workload]()
+
+    # the CPU baselines first: the all-cores leg forks workers, which must
+    # happen before this process initialises the GPU
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene)
+        cpu["all_cores"] = cpu_baseline_all_cores(args.workload)
+        cpu["port_over_solver_js"] = PORT_OVER_SOLVER_JS.get(args.workload)
+        print("bench: cpu baseline %.3e p-substeps/s on 1 core, %.3e on %d" %
+              (cpu["value"], cpu["all_cores"]["value"], cpu["all_cores"]["cores"]), file=sys.stderr, flush=True)
+
     solver = from_scene(scene)
     inf = solver.info()
     p = scene.params_with()
@@ -293,8 +392,8 @@ def main():
     run_frames(solver, scene, 0, args.warmup, rest_pos, grab_ids)
     solver.sync()
     n0 = active_count(solver)
-
     f0 = args.warmup
+    solver.checkpoint(0)   # the state the timed frames start from (outside the timed region)
     solver.sync()
     t0 = time.perf_counter()
     run_frames(solver, scene, f0, args.steps, rest_pos, grab_ids)
@@ -303,49 +402,30 @@ def main():
     n1 = active_count(solver)
     n_mean = 0.5 * (n0 + n1)
     value = n_mean * scene.substeps * args.steps / el
+    end_pos = solver.read_positions().copy()
     print("bench: %d frames in %.3f s, %.0f active particles" % (args.steps, el, n_mean), file=sys.stderr, flush=True)
 
-    # per-pass timing: HIP events on the solver's stream, a separate region
+    # the timed frames again from the checkpoint, with events around every pass
+    solver.restore(0)
     solver.profile(True)
-    run_frames(solver, scene, f0 + args.steps, args.profile_steps, rest_pos, grab_ids)
+    run_frames(solver, scene, f0, args.steps, rest_pos, grab_ids)
     solver.sync()
     stats = solver.profile_read()
     solver.profile(False)
+    replay_identical = bool(np.array_equal(end_pos, solver.read_positions()))
 
     nf = n_mean - solver.solidCount
     ns, c, t = solver.solidCount, inf.num_constraints, inf.table_size
     iters = p["constraint_iters"]
-    passes = {}
-    dom, dom_ms = None, -1.0
-    for name, (launches, total_ms) in stats.items():
-        avg = total_ms / launches
-        passes[name] = {"launches": launches, "avg_us": round(avg * 1e3, 3),
-                        "ms_per_frame": round(total_ms / args.profile_steps, 4)}
-        b = pass_bytes(name, n_mean, nf, ns, c, t)
-        if b is not None:
-            if name == "constraints":
-                b = b  # one launch = one iteration
-            passes[name]["gbs"] = round(b / (avg * 1e-3) / 1e9, 1)
-        if total_ms > dom_ms and b is not None:
-            dom, dom_ms = name, total_ms
-    launches, total_ms = stats[dom]
-    avg_s = total_ms / launches * 1e-3
-    bytes_dom = pass_bytes(dom, n_mean, nf, ns, c, t)
-    achieved = bytes_dom / avg_s / 1e9
-    traffic = measured_traffic(scene.name, dom)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                "bytes_per_launch": int(bytes_dom), "avg_launch_us": round(avg_s * 1e6, 3)}
-    if traffic:
-        roofline["traffic_gbs"] = round(traffic / avg_s / 1e9, 1)
+    roofline, passes = roofline_of(stats, args.steps, n_mean, nf, ns, c, t, first_frame=f0)
+    roofline["traffic"] = measured_traffic(scene.name, roofline["kernel"][3:])
+    if roofline["traffic"]:
+        roofline["traffic_gbs"] = round(roofline["traffic"] / (roofline["avg_launch_us"] * 1e-6) / 1e9, 1)
+    roofline["replay_identical"] = replay_identical
     inf_end = solver.info()
     b_sub = model_bytes_per_substep(n_mean, nf, ns, c, t, iters)
     model_frac = b_sub * (value / n_mean) / (HBM_PEAK_GBS * 1e9)
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(scene)
-        print("bench: cpu baseline %.3e p-substeps/s" % cpu["value"], file=sys.stderr, flush=True)
+    solver.close()
 
     batched = None
     if not args.no_batched:

@@ -76,7 +76,7 @@ REFERENCE_DEFAULTS = dict(
     floor_friction_fluid=0.04, wall_x=3.5, wall_z=3.5, wall_height=0.92,
     wall_half_thick=0.07, cull_radius=7.0, max_speed=14.0, grab_stiffness=0.3,
     cell_size=0.22, constraint_iters=3, table_size=0,
-    pair_slots=0, fluid_lanes=0, jacobi_omega=4.0, contact_scale=1.0, gravity_x=0.0,
+    pair_slots=0, fluid_lanes=0, jacobi_omega=8.0, contact_scale=1.0, gravity_x=0.0,
     gravity_z=0.0,
 )
 

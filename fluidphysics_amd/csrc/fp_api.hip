@@ -423,7 +423,7 @@ void fp_params_default(fp_params* p) {
   p->cell_size = 0.22;
   p->constraint_iters = 3;
   p->table_size = 0;
-  p->jacobi_omega = 4.0;
+  p->jacobi_omega = 8.0;  // with 3 iterations: the reference's 300-frame statistics at its own constants
   p->contact_scale = 1.0;
 }
 

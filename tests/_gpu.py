@@ -36,8 +36,13 @@ class GpuAdapter:
 
 
 def make_gpu(d, iters=3, omega=4.0, **overrides):
-    """A GPU solver for a golden scenario dict (tests/golden/gen_inputs.py)."""
-    params = dict(gravity=d["gravity"], constraint_iters=iters, jacobi_omega=omega)
+    """A GPU solver for a golden scenario dict (tests/golden/gen_inputs.py);
+    iters / omega None keep the library defaults."""
+    params = dict(gravity=d["gravity"])
+    if iters is not None:
+        params["constraint_iters"] = iters
+    if omega is not None:
+        params["jacobi_omega"] = omega
     params.update(overrides)
     s = Solver(d["maxFluid"], **params)
     s.addSolidCube(*d["cube"])

@@ -74,10 +74,43 @@ def ring_cap():
                 emit=emit, frames=30, substeps=5, cuts=[], dumpFrames=[30])
 
 
+# the solver's public fields (src/solver.js:14-37, cellSize :50) for the
+# GPU-scale constant set; fluidDamping stays the reference's hard-coded 0.6
+_JS_FIELDS = dict(solid_radius="solidRadius", fluid_radius="fluidRadius", compliance="compliance",
+                  plastic_threshold="plasticThreshold", plastic_rate="plasticRate", fluid_h="fluidH",
+                  rest_density="restDensity", stiffness="stiffness", near_stiffness="nearStiffness",
+                  viscosity="viscosity", solid_damping="solidDamping",
+                  floor_friction_solid="floorFrictionSolid", floor_friction_fluid="floorFrictionFluid",
+                  cull_radius="cullRadius", max_speed="maxSpeed", grab_stiffness="grabStiffness",
+                  cell_size="cellSize")
+
+
+def _scene_stats(sc, name, frames):
+    """A GPU-scale scene as the reference runs it: the scene's constants set
+    on the solver's public fields (cellSize after addSolidCube, which
+    allocates), its emits, its wedges within `frames`; statistics only."""
+    p = sc.params_with()
+    assert p["fluid_damping"] == 0.6
+    fields = {js: float(p[k]) for k, js in _JS_FIELDS.items()}
+    return dict(
+        name=name, maxFluid=sc.max_fluid, gravity=float(p["gravity"]), cube=[float(x) for x in sc.cube],
+        fields=fields,
+        emit=[[*map(float, q), 0.0, 0.0, 0.0, sc.fluid_mass] for q in sc.fluid_pos],
+        frames=frames, substeps=sc.substeps,
+        cuts=[dict(frame=w.frame, o=list(w.o), d1=list(w.d1), d2=list(w.d2)) for w in sc.wedges if w.frame < frames],
+        dumpFrames=[], dumpInit=False, statsOnly=True,
+    )
+
+
+def single_gpu_stats():
+    """The 20^3 + 65,536 scene (its wedge at frame 150) for 300 frames."""
+    return _scene_stats(scenes.single_gpu_scene(), "single_gpu_stats", 300)
+
+
 def main():
     out = os.path.join(HERE, "inputs")
     os.makedirs(out, exist_ok=True)
-    for d in (plain_cpu(), grab_tear(), ring_cap()):
+    for d in (plain_cpu(), grab_tear(), ring_cap(), single_gpu_stats()):
         with open(os.path.join(out, d["name"] + ".json"), "w") as f:
             json.dump(d, f)
         print("wrote", d["name"], len(d["emit"]), "emits")

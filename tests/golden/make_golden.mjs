@@ -61,10 +61,15 @@ function frameDigests(s) {
   };
 }
 
+// d.fields: public tuning fields to set (cellSize only exists once
+// addSolidCube has allocated, so it is set after it)
 function makeSolver(Solver, d) {
   const s = new Solver({ maxFluid: d.maxFluid });
   s.gravity = d.gravity;
+  const f = d.fields || {};
+  for (const k of Object.keys(f)) if (k !== 'cellSize') s[k] = f[k];
   s.addSolidCube(...d.cube);
+  if (f.cellSize !== undefined) s.cellSize = f.cellSize;
   for (const e of d.emit) s.emitFluid(...e);
   return s;
 }
@@ -89,7 +94,7 @@ async function run(Solver, d) {
     out.init = { digests: frameDigests(s), stats: frameStats(s) };
   }
   // one substep
-  {
+  if (!d.statsOnly) {
     const s = makeSolver(Solver, d);
     s.step(1 / 60 / d.substeps, 1);
     if (d.dumpInit !== false) writeState(s, base + '_sub1.bin.gz');
@@ -127,7 +132,9 @@ async function run(Solver, d) {
       s.grabs = grabTargets ? grabTargets.map((t) => ({ i: t.i, x: t.x, y: t.y, z: t.z })) : [];
     }
     s.step(1 / 60, d.substeps);
-    out.frames.push({ f: f + 1, digests: frameDigests(s), stats: frameStats(s) });
+    out.frames.push(d.statsOnly ? { f: f + 1, stats: frameStats(s) }
+      : { f: f + 1, digests: frameDigests(s), stats: frameStats(s) });
+    if (d.statsOnly && (f + 1) % 25 === 0) console.log(d.name, 'frame', f + 1, JSON.stringify(frameStats(s)));
     if (d.dumpFrames.includes(f + 1)) writeState(s, base + '_f' + (f + 1) + '.bin.gz');
   }
   fs.writeFileSync(base + '_trace.json', JSON.stringify(out));

@@ -1,0 +1,68 @@
+"""Multi-frame statistics of the oracle's Jacobi formulation (the form the
+HIP kernels implement; pinned by tests/test_oracle_pins.py) on a GPU-scale
+scene, for the GPU's multi-frame statistics test:
+
+    python3 tests/golden/make_oracle_stats.py single_gpu [frames]
+
+writes tests/golden/<scene>_jacobi_trace.json: per-frame kinetic energy,
+mean fluid density, alive constraints and fluid count of the oracle
+(float32 stores, Jacobi mode, the scene's constraint_iters / jacobi_omega),
+with the scene's wedges.  The reference (src/solver.js) itself is not usable
+as the target at these constants: its single Gauss-Seidel sweep goes
+unstable on the 20^3 cube within 7 frames (single_gpu_stats_trace.json,
+written by make_golden.mjs from the reference)."""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, ROOT)
+
+from fluidphysics_amd import scenes  # noqa: E402
+from oracle.oracle import from_scene  # noqa: E402
+
+
+def frame_stats(o, ns):
+    st = o.read_state()
+    rho, _ = o.read_density()
+    act = st["active"].astype(bool)
+    v = st["vel"].astype(np.float64).reshape(-1, 3)
+    m = 1.0 / st["inv_mass"].astype(np.float64)
+    ke = float(np.sum(0.5 * m[act] * np.sum(v[act] * v[act], axis=1)))
+    fl = act.copy()
+    fl[:ns] = False
+    return dict(ke=ke, meanRho=float(np.mean(rho[fl])) if fl.any() else 0.0,
+                alive=int(o.read_constraints()["alive"].sum()), fluid=int(fl.sum()))
+
+
+def main():
+    name = sys.argv[1] if len(sys.argv) > 1 else "single_gpu"
+    frames = int(sys.argv[2]) if len(sys.argv) > 2 else 300
+    sc = scenes.SCENES[name]()
+    p = sc.params_with()
+    o = from_scene(sc, storage="f32", mode="jacobi")
+    sp = 1.91 * p["fluid_radius"]
+    out = dict(name=name, oracle="f32 storage, jacobi mode", constraint_iters=p["constraint_iters"],
+               jacobi_omega=p["jacobi_omega"], frames=[], cuts=[])
+    t0 = time.time()
+    for f in range(frames):
+        if sc.hose:
+            hp, hv = scenes.hose_burst(sc.hose, f, sp)
+            o.emit(hp, hv, sc.fluid_mass)
+        for w in sc.wedges:
+            if w.frame == f:
+                out["cuts"].append(dict(frame=f, n=o.cut(w.o, w.d1, w.d2)))
+        o.step(sc.dt, sc.substeps)
+        out["frames"].append(dict(f=f + 1, stats=frame_stats(o, sc.solid_count)))
+        if (f + 1) % 25 == 0:
+            print(name, f + 1, out["frames"][-1]["stats"], "%.0fs" % (time.time() - t0), flush=True)
+    with open(os.path.join(HERE, name + "_jacobi_trace.json"), "w") as fh:
+        json.dump(out, fh)
+
+
+if __name__ == "__main__":
+    main()

@@ -10,7 +10,8 @@
 // its partition equals the one `group` labels.
 import { Solver } from '../../fluidphysics_amd/js/solver.mjs';
 
-const s = new Solver({ maxFluid: 2600, params: { constraintIters: 3, jacobiOmega: 8 } });
+// the library defaults: 3 Jacobi iterations, relaxation 8
+const s = new Solver({ maxFluid: 2600 });
 s.addSolidCube(0, 2.0, 0, 7, 0.18);
 const cam = [5.2, 3.6, 6.4];
 let frame = 0;

@@ -266,9 +266,13 @@ def test_pick_matches_oracle():
 # ------------------------------------------------------ trajectories ------
 
 def test_plain_scene_300_frames_statistics_within_1pct():
-    log("case 300 frames plain_cpu")
+    """The solver as the drop-in creates it (library defaults: 3 Jacobi
+    iterations, relaxation 8; only gravity set, as the scene does) against
+    the reference's own 300-frame trace."""
+    log("case 300 frames plain_cpu (library defaults)")
     d, t = G.scenario("plain_cpu"), G.trace("plain_cpu")
-    g = make_gpu(d, iters=3, omega=8.0)
+    g = make_gpu(d, iters=None, omega=None)
+    assert g.params()["constraint_iters"] == 3 and g.params()["jacobi_omega"] == 8.0
     drv = G.Driver(d)
     ad = GpuAdapter(g)
     ke, rho = [], []
