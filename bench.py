@@ -54,7 +54,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "large_hbm_traffic.json")
 # frames, one thread each (oracle/time_reference.py in the development
 # container; profiles/cpu_reference_timing.md): a port rate divided by this
 # is the reference's own rate on the same host.
-PORT_OVER_SOLVER_JS = {}
+PORT_OVER_SOLVER_JS = {"plain_cpu": 1.69, "single_gpu": 1.86, "large": 1.85}
 
 
 def model_bytes_per_substep(n, nf, ns, c, t, iters):
