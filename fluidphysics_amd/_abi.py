@@ -113,7 +113,7 @@ class FpBatchInfo(ctypes.Structure):
         ("frames", ctypes.c_int64),
         ("graph_launches", ctypes.c_int64),
         ("contact_lanes", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("graph_captures", ctypes.c_int32),
     ]
 
 

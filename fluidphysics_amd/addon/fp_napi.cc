@@ -605,6 +605,7 @@ napi_value BatchInfo(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, o, "frames", make_int(env, i.frames));
   napi_set_named_property(env, o, "graphLaunches", make_int(env, i.graph_launches));
   napi_set_named_property(env, o, "contactLanes", make_int(env, i.contact_lanes));
+  napi_set_named_property(env, o, "graphCaptures", make_int(env, i.graph_captures));
   return o;
 }
 

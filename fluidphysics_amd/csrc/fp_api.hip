@@ -1283,6 +1283,7 @@ int fp_batch_get_info(const fp_batch* b, fp_batch_info* out) {
   out->frames = b->frames;
   out->graph_launches = b->g.graph_launches;
   out->contact_lanes = b->g.clanes_best;
+  out->graph_captures = (int32_t)b->g.graph_captures;
   return FP_OK;
 }
 

@@ -308,9 +308,9 @@ __device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb, 
 
 template <int L>
 __global__ void __launch_bounds__(BLOCK) kg_contacts_solid(const SceneArgs* __restrict__ S,
-                                                           const int2* __restrict__ W, int nw) {
+                                                           const int2* __restrict__ W) {
   __shared__ int lds_bonds[(BLOCK / L) * MAXB];
-  FOR_ITEMS(S, W, nw) contacts_solid_body<L>(A, lb, lds_bonds);
+  FOR_ITEMS(S, W) contacts_solid_body<L>(A, lb, lds_bonds);
 }
 
 // Fluid in grid slot e: corrections from the solids of its 27 cells (rows in
@@ -418,8 +418,8 @@ __device__ __forceinline__ void contacts_fluid_body(const SceneArgs& A, int lb) 
 #define CONTACTS_FLUID_WAVES 0
 #endif
 __global__ void __launch_bounds__(BLOCK) WAVES_ATTR(CONTACTS_FLUID_WAVES) kg_contacts_fluid(const SceneArgs* __restrict__ S,
-                                                           const int2* __restrict__ W, int nw) {
-  FOR_ITEMS(S, W, nw) contacts_fluid_body(A, lb);
+                                                           const int2* __restrict__ W) {
+  FOR_ITEMS(S, W) contacts_fluid_body(A, lb);
 }
 
 // Lane index of this thread over the fluid order: the scene's L fluid lanes
@@ -804,9 +804,9 @@ __device__ __forceinline__ int fluid_block(int j, int o, int a) {
   return first + ((j - ((x - c) & 7)) >> 3);
 }
 
-#define FLUID_ITEMS(S, W, nw)                                                                        \
-  for (int w_ = (int)blockIdx.x; w_ < (nw); w_ += (int)gridDim.x)                                    \
-    if (const int2 it_ = (W)[w_]; true)                                                              \
+#define FLUID_ITEMS(S, W)                                                                        \
+  for (int w_ = (int)blockIdx.x, nw_ = items_count(W); w_ < nw_; w_ += (int)gridDim.x)             \
+    if (const int2 it_ = (W)[1 + w_]; true)                                                          \
       if (const SceneArgs& A = (S)[it_.x]; true)                                                     \
         if (const int lb = fluid_block(it_.y, w_ - it_.y,                                            \
                                        (int)((((int64_t)gld(A.src.lstart + A.fp.cg.M)) << A.fp.fl_log2) + \
@@ -829,8 +829,8 @@ __device__ __forceinline__ int fluid_block(int j, int o, int a) {
 #define DENSITY_ATTR WAVES_ATTR(DENSITY_WAVES)
 
 __global__ void __launch_bounds__(FLUID_BLOCK) DENSITY_ATTR kg_density(const SceneArgs* __restrict__ S,
-                                                          const int2* __restrict__ W, int nw) {
-  FLUID_ITEMS(S, W, nw) density_body(A, lb);
+                                                          const int2* __restrict__ W) {
+  FLUID_ITEMS(S, W) density_body(A, lb);
 }
 
 struct DisplaceAcc {
@@ -889,8 +889,8 @@ __device__ __forceinline__ void displace_body(const SceneArgs& A, int lb) {
 }
 
 __global__ void __launch_bounds__(FLUID_BLOCK) WAVES_ATTR(DISPLACE_WAVES) kg_displace(const SceneArgs* __restrict__ S,
-                                                           const int2* __restrict__ W, int nw) {
-  FLUID_ITEMS(S, W, nw) displace_body(A, lb);
+                                                           const int2* __restrict__ W) {
+  FLUID_ITEMS(S, W) displace_body(A, lb);
 }
 
 struct XsphAcc {
@@ -944,9 +944,8 @@ __device__ __forceinline__ void xsph_body(const SceneArgs& A, int lb) {
                                 v.z + fp.visc * (sz * inv_m - v.z), 0.f));
 }
 
-__global__ void __launch_bounds__(FLUID_BLOCK) WAVES_ATTR(XSPH_WAVES) kg_xsph(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
-                                                       int nw) {
-  FLUID_ITEMS(S, W, nw) xsph_body(A, lb);
+__global__ void __launch_bounds__(FLUID_BLOCK) WAVES_ATTR(XSPH_WAVES) kg_xsph(const SceneArgs* __restrict__ S, const int2* __restrict__ W) {
+  FLUID_ITEMS(S, W) xsph_body(A, lb);
 }
 
 // ---- launchers ------------------------------------------------------------------
@@ -999,20 +998,21 @@ int fp_scene_blocks(const fp_solver* s, int kind, int contact_lanes) {
   return 0;
 }
 
-static void launch_contacts_solid(fp_group* g, hipStream_t st, int lanes, const int2* items, int n) {
-  if (n <= 0) return;
+// grid: the workgroups (a multiple of 8), items: a work list with its count.
+static void launch_contacts_solid(fp_group* g, hipStream_t st, int lanes, const int2* items, int grid) {
+  if (grid <= 0) return;
   switch (lanes) {
     case 32:
-      hipLaunchKernelGGL(kg_contacts_solid<32>, dim3(n), dim3(BLOCK), 0, st, g->d_args, items, n);
+      hipLaunchKernelGGL(kg_contacts_solid<32>, dim3(grid), dim3(BLOCK), 0, st, g->d_args, items);
       break;
     case 16:
-      hipLaunchKernelGGL(kg_contacts_solid<16>, dim3(n), dim3(BLOCK), 0, st, g->d_args, items, n);
+      hipLaunchKernelGGL(kg_contacts_solid<16>, dim3(grid), dim3(BLOCK), 0, st, g->d_args, items);
       break;
     case 8:
-      hipLaunchKernelGGL(kg_contacts_solid<8>, dim3(n), dim3(BLOCK), 0, st, g->d_args, items, n);
+      hipLaunchKernelGGL(kg_contacts_solid<8>, dim3(grid), dim3(BLOCK), 0, st, g->d_args, items);
       break;
     default:
-      hipLaunchKernelGGL(kg_contacts_solid<4>, dim3(n), dim3(BLOCK), 0, st, g->d_args, items, n);
+      hipLaunchKernelGGL(kg_contacts_solid<4>, dim3(grid), dim3(BLOCK), 0, st, g->d_args, items);
   }
 }
 
@@ -1031,10 +1031,11 @@ int fp_group_contacts_solid(fp_group* g, hipStream_t st) {
     hipEvent_t* ev = &g->tune_ev[2 * (size_t)g->tune_used++];
     g->tune_of.push_back(c);
     FP_GTRY(g, hipEventRecord(ev[0], st));
-    launch_contacts_solid(g, st, kTuneLanes[c], g->d_tune_items + g->tune_off[c], g->tune_n[c]);
+    launch_contacts_solid(g, st, kTuneLanes[c], g->d_tune_items + g->tune_off[c], fp_grid8(g->tune_n[c]));
     FP_GTRY(g, hipEventRecord(ev[1], st));
   } else {
-    launch_contacts_solid(g, st, g->clanes, g->d_items[W_SOLID], g->n_items[W_SOLID]);
+    launch_contacts_solid(g, st, g->clanes, g->d_items[W_SOLID],
+                          g->capturing ? g->grid_cap[W_SOLID] : fp_grid8(g->n_items[W_SOLID]));
   }
   FP_GTRY(g, hipGetLastError());
   return FP_OK;

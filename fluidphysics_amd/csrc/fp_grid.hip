@@ -157,14 +157,13 @@ __global__ void __launch_bounds__(BLOCK) k_scan_apply(int t, int* __restrict__ c
 }
 
 __global__ void __launch_bounds__(BLOCK) kg_scan_reduce(const SceneArgs* __restrict__ S,
-                                                        const int2* __restrict__ W, int nw) {
-  FOR_ITEMS(S, W, nw) scan_reduce_body((int)A.fp.cg.M, A.lcount, A.lblock_sums, lb);
+                                                        const int2* __restrict__ W) {
+  FOR_ITEMS(S, W) scan_reduce_body((int)A.fp.cg.M, A.lcount, A.lblock_sums, lb);
 }
 
 // fluid: the fluid order's scan (starts in lstart_f), else the grid's (lstart)
-__global__ void __launch_bounds__(BLOCK) kg_scan_apply(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
-                                                       int nw, int fluid) {
-  FOR_ITEMS(S, W, nw)
+__global__ void __launch_bounds__(BLOCK) kg_scan_apply(const SceneArgs* __restrict__ S, const int2* __restrict__ W, int fluid) {
+  FOR_ITEMS(S, W)
   scan_apply_body((int)A.fp.cg.M, A.lcount, fluid ? A.lblock_f : A.lblock_sums, fluid ? A.lstart_f : A.lstart, lb);
 }
 
@@ -192,9 +191,8 @@ __device__ __forceinline__ void cell_count_body(const SceneArgs& A, int lb) {
   box_update(act && i < A.ns, p.x, p.y, p.z, A.sbox);
 }
 
-__global__ void __launch_bounds__(BLOCK) kg_cell_count(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
-                                                       int nw) {
-  FOR_ITEMS(S, W, nw) cell_count_body(A, lb);
+__global__ void __launch_bounds__(BLOCK) kg_cell_count(const SceneArgs* __restrict__ S, const int2* __restrict__ W) {
+  FOR_ITEMS(S, W) cell_count_body(A, lb);
 }
 
 __device__ __forceinline__ void cell_scatter_body(const SceneArgs& A, int lb) {
@@ -206,8 +204,8 @@ __device__ __forceinline__ void cell_scatter_body(const SceneArgs& A, int lb) {
 }
 
 __global__ void __launch_bounds__(BLOCK) kg_cell_scatter(const SceneArgs* __restrict__ S,
-                                                         const int2* __restrict__ W, int nw) {
-  FOR_ITEMS(S, W, nw) cell_scatter_body(A, lb);
+                                                         const int2* __restrict__ W) {
+  FOR_ITEMS(S, W) cell_scatter_body(A, lb);
 }
 
 // Stable order inside a cell (ascending particle index) and the sorted
@@ -241,9 +239,8 @@ __device__ __forceinline__ void cell_rank_body(const SceneArgs& A, int lb) {
   if (r == 0) gst(A.lnsol + k, nsol);
 }
 
-__global__ void __launch_bounds__(BLOCK) kg_cell_rank(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
-                                                      int nw) {
-  FOR_ITEMS(S, W, nw) cell_rank_body(A, lb);
+__global__ void __launch_bounds__(BLOCK) kg_cell_rank(const SceneArgs* __restrict__ S, const int2* __restrict__ W) {
+  FOR_ITEMS(S, W) cell_rank_body(A, lb);
 }
 
 int fp_group_cell_grid(fp_group* g, bool count_done) {
@@ -297,8 +294,8 @@ __device__ __forceinline__ void fluid_scatter_body(const SceneArgs& A, int lb) {
 }
 
 __global__ void __launch_bounds__(BLOCK) kg_fluid_scatter(const SceneArgs* __restrict__ S,
-                                                          const int2* __restrict__ W, int nw) {
-  FOR_ITEMS(S, W, nw) fluid_scatter_body(A, lb);
+                                                          const int2* __restrict__ W) {
+  FOR_ITEMS(S, W) fluid_scatter_body(A, lb);
 }
 
 __device__ __forceinline__ void fluid_rank_body(const SceneArgs& A, int lb) {
@@ -326,9 +323,8 @@ __device__ __forceinline__ void fluid_rank_body(const SceneArgs& A, int lb) {
   gst(A.sphash2 + d, hb);
 }
 
-__global__ void __launch_bounds__(BLOCK) kg_fluid_rank(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
-                                                       int nw) {
-  FOR_ITEMS(S, W, nw) fluid_rank_body(A, lb);
+__global__ void __launch_bounds__(BLOCK) kg_fluid_rank(const SceneArgs* __restrict__ S, const int2* __restrict__ W) {
+  FOR_ITEMS(S, W) fluid_rank_body(A, lb);
 }
 
 int fp_group_resort(fp_group* g) {

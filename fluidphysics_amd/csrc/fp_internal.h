@@ -345,7 +345,7 @@ struct fp_group {
   int2* d_items[W_NUM] = {nullptr};
   int n_items[W_NUM] = {0};
   int items_cap[W_NUM] = {0};
-  int grid_cap[W_NUM] = {0};  // persistent grid size per kind (0 = one block per item)
+  int grid_cap[W_NUM] = {0};  // grid of each kind in the captured frame (its items then, rounded up to 8)
   double built_h = -1.0;
   bool dirty = true;
   int iters = 0;
@@ -357,8 +357,11 @@ struct fp_group {
   double graph_dt = -1.0;
   int graph_sub = 0;
   int graph_iters = -1;
+  int graph_clanes = 0;             // contact lanes the captured frame launches
+  bool graph_unconstrained = false;  // the captured frame has the bounds pass of unconstrained scenes
   bool use_graph = true;
   int64_t graph_launches = 0;
+  int64_t graph_captures = 0;
   // A captured frame of a group with at least FP_SIDE_MIN particle slots runs
   // each substep's solid chain (solid contacts, the constraint iterations) on
   // a second stream, beside the fluid chain (fluid contacts, the fluid order,
@@ -455,42 +458,57 @@ struct PassScope {
 
 static inline int fp_blocks(int n) { return (n + BLOCK - 1) / BLOCK; }
 
-// Launch of a group kernel over work kind k: grid = the items (or the
-// persistent cap), the kernel strides over them.
+// Work lists in device memory: W[0].x = the number of items, the (scene,
+// local block) items at W[1 ...].  The kernels read the count there, so a
+// captured frame stays valid when the group's members change: the host
+// rewrites the scene table and the lists before the next replay, and every
+// kernel of the graph strides over whatever the lists hold then.
+static inline int fp_grid8(int n) { return (n + 7) & ~7; }
+
+// Launch of a group kernel over work kind k.  A direct launch takes one
+// workgroup per item (rounded up to a multiple of 8, the XCD count); a launch
+// being captured takes the grid fixed for the graph (grid_cap: the items at
+// capture, rounded up), whose workgroups stride over the items of later
+// replays -- persistent workgroups over the scene table.
 #define GROUP_LAUNCH(g, kernel, kind, block, ...) GROUP_LAUNCH_ON(g, (g)->stream, kernel, kind, block, ##__VA_ARGS__)
 #define GROUP_LAUNCH_ON(g, st, kernel, kind, block, ...)                                            \
   do {                                                                                               \
-    const int n_ = (g)->n_items[kind];                                                               \
-    if (n_ > 0) {                                                                                    \
-      const int grid_ = (g)->grid_cap[kind] > 0 ? std::min(n_, (g)->grid_cap[kind]) : n_;          \
-      hipLaunchKernelGGL(kernel, dim3(grid_), dim3(block), 0, (st), (g)->d_args, (g)->d_items[kind], n_, \
+    const int grid_ = (g)->capturing ? (g)->grid_cap[kind] : fp_grid8((g)->n_items[kind]);         \
+    if (grid_ > 0)                                                                                   \
+      hipLaunchKernelGGL(kernel, dim3(grid_), dim3(block), 0, (st), (g)->d_args, (g)->d_items[kind], \
                          ##__VA_ARGS__);                                                             \
-    }                                                                                                \
   } while (0)
+
+// The number of items of work list W (the same in every lane).
+__device__ __forceinline__ int items_count(const int2* __restrict__ W) {
+  return __builtin_amdgcn_readfirstlane(*(const FP_GLOBAL int*)&W[0].x);
+}
 
 // The (scene, local block) items of a group kernel: for (w ...) { SceneArgs& A; int lb }
 // Workgroups are dispatched to the 8 XCDs in turn (workgroup b to XCD
-// b % 8), each XCD with its own L2.  With one workgroup per item, the work
-// list is dealt to the XCDs in runs of FP_XCD_CHUNK consecutive items, so
-// items next to each other (neighbouring particles, whose gathers read the
-// same rows) share an L2 while every XCD still gets a share of each part of
-// the list (idle capacity at a fluid order's end, scenes of unequal size).
-// The items past the last whole round of 8 runs keep their own workgroup.
+// b % 8), each XCD with its own L2; grids are multiples of 8, so item
+// position w runs on XCD w % 8 whether a workgroup takes one item or strides
+// over several.  The work list is dealt to the XCDs in runs of FP_XCD_CHUNK
+// consecutive items, so items next to each other (neighbouring particles,
+// whose gathers read the same rows) share an L2 while every XCD still gets a
+// share of each part of the list (idle capacity at a fluid order's end,
+// scenes of unequal size).  The items past the last whole round of 8 runs
+// keep their own position.
 #ifndef FP_XCD_CHUNK
 #define FP_XCD_CHUNK 16
 #endif
 #define FP_XCDS 8
 __device__ __forceinline__ int xcd_item(int b, int n) {
   constexpr int K = FP_XCD_CHUNK, R = FP_XCDS * FP_XCD_CHUNK;
-  if (K <= 1 || (int)gridDim.x != n || b >= n / R * R) return b;
-  const int x = b % FP_XCDS, l = b / FP_XCDS;  // XCD, and the workgroup's place in its turn
+  if (K <= 1 || b >= n / R * R) return b;
+  const int x = b % FP_XCDS, l = b / FP_XCDS;  // XCD, and the position's place in its turn
   return (l / K * FP_XCDS + x) * K + l % K;
 }
 
-#define FOR_ITEMS(S, W, nw)                                                         \
-  for (int w_ = (int)blockIdx.x; w_ < (nw); w_ += (int)gridDim.x)                   \
-    if (const int2 it_ = (W)[xcd_item(w_, (nw))]; true)                             \
-      if (const SceneArgs& A = (S)[it_.x]; true)                                    \
+#define FOR_ITEMS(S, W)                                                                    \
+  for (int w_ = (int)blockIdx.x, nw_ = items_count(W); w_ < nw_; w_ += (int)gridDim.x)    \
+    if (const int2 it_ = (W)[1 + xcd_item(w_, nw_)]; true)                                 \
+      if (const SceneArgs& A = (S)[it_.x]; true)                                           \
         if (const int lb = it_.y; true)
 
 // ---- per-scene set-up (fp_fluid.hip, fp_step.hip, fp_solid.hip) -------------

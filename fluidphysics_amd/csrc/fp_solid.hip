@@ -89,9 +89,8 @@ __device__ __forceinline__ void constraints_body(const SceneArgs& A, int lb, int
   if (tore) *A.tear = 1;
 }
 
-__global__ void __launch_bounds__(BLOCK) kg_constraints(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
-                                                        int nw, int it) {
-  FOR_ITEMS(S, W, nw) constraints_body(A, lb, it);
+__global__ void __launch_bounds__(BLOCK) kg_constraints(const SceneArgs* __restrict__ S, const int2* __restrict__ W, int it) {
+  FOR_ITEMS(S, W) constraints_body(A, lb, it);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_cut(int nc, const float4* __restrict__ pos, const int2* __restrict__ ab,

@@ -72,9 +72,8 @@ __device__ __forceinline__ void integrate_body(const SceneArgs& A, int lb) {
 }
 
 template <bool COUNT>
-__global__ void __launch_bounds__(BLOCK) kg_integrate(const SceneArgs* __restrict__ S, const int2* __restrict__ W,
-                                                      int nw) {
-  FOR_ITEMS(S, W, nw) integrate_body<COUNT>(A, lb);
+__global__ void __launch_bounds__(BLOCK) kg_integrate(const SceneArgs* __restrict__ S, const int2* __restrict__ W) {
+  FOR_ITEMS(S, W) integrate_body<COUNT>(A, lb);
 }
 
 // One round of grab pulls of one scene; indices inside a round are distinct.
@@ -101,8 +100,8 @@ __device__ __forceinline__ void bounds_velocity_body(const SceneArgs& A, int lb)
 }
 
 __global__ void __launch_bounds__(BLOCK) kg_bounds_velocity(const SceneArgs* __restrict__ S,
-                                                            const int2* __restrict__ W, int nw) {
-  FOR_ITEMS(S, W, nw) bounds_velocity_body(A, lb);
+                                                            const int2* __restrict__ W) {
+  FOR_ITEMS(S, W) bounds_velocity_body(A, lb);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_count_active(int begin, int end, const uint8_t* __restrict__ active,
@@ -254,8 +253,28 @@ static void drop_graph(fp_group* g) {
   g->graph_sub = 0;
 }
 
-// Scene table and work lists for substep h: rebuilt when the members, their
-// parameters or h changed.  A changed membership also drops the graph.
+// A device buffer of at least `count` elements of T, grown geometrically;
+// true when it moved (a graph that captured the old address is dropped).
+template <class T>
+static hipError_t reserve(fp_group* g, T*& p, int& cap, int count, bool& moved) {
+  moved = false;
+  if (count <= cap && p) return hipSuccess;
+  hipError_t e = hipStreamSynchronize(g->stream);
+  if (e != hipSuccess) return e;
+  if (p) fp_pool_put(g->device, p);
+  p = nullptr;
+  // headroom as large as the growth a captured frame absorbs (1.5x its grids)
+  const int n = std::max(count + count / 2 + 16, 2 * cap);
+  e = fp_pool_get(g->device, sizeof(T) * (size_t)n, (void**)&p, nullptr);
+  cap = e == hipSuccess ? n : 0;
+  moved = true;
+  return e;
+}
+
+// Scene table and work lists for substep h: rewritten when the members,
+// their parameters or h changed.  A captured frame survives a rewrite (its
+// kernels read the table and the lists' counts on the device); it is dropped
+// only when a buffer has to move.
 int fp_group_prepare(fp_group* g, double h) {
   bool members = g->dirty || g->versions.size() != g->scenes.size();
   if (!members)
@@ -290,74 +309,54 @@ int fp_group_prepare(fp_group* g, double h) {
     args[k].snap_off = snap;
     snap += 3 * (int64_t)g->scenes[k]->n;
   }
-  if (ns > g->args_cap) {
-    FP_GTRY(g, hipStreamSynchronize(g->stream));
-    if (g->d_args) fp_pool_put(g->device, g->d_args);
-    g->d_args = nullptr;
-    g->args_cap = 0;
-    FP_GTRY(g, fp_pool_get(g->device, sizeof(SceneArgs) * std::max(ns, 1), (void**)&g->d_args, nullptr));
-    g->args_cap = ns;
-    drop_graph(g);
-  }
+  bool moved = false;
+  FP_GTRY(g, reserve(g, g->d_args, g->args_cap, std::max(ns, 1), moved));
+  if (moved) drop_graph(g);
   if (ns) {
     FP_GTRY(g, hipMemcpyAsync(g->d_args, args.data(), sizeof(SceneArgs) * ns, hipMemcpyHostToDevice, g->stream));
   }
-  // work lists: rebuilt for a new membership, or when a parameter change
-  // (contact_lanes) changed a scene's block count; either drops the graph
+  // work lists (count first): rewritten for a new membership, or when a
+  // parameter change (contact_lanes) changed a scene's block count
   std::vector<int2> items[W_NUM];
   bool rebuild = g->dirty || g->versions.size() != g->scenes.size() || g->items_clanes != g->clanes;
   for (int kind = 0; kind < W_NUM; kind++) {
+    items[kind].push_back(make_int2(0, 0));
     for (int k = 0; k < ns; k++) {
       const int nb = fp_scene_blocks(g->scenes[k], kind, g->clanes);
       for (int b = 0; b < nb; b++) items[kind].push_back(make_int2(k, b));
     }
-    rebuild |= (int)items[kind].size() != g->n_items[kind];
+    items[kind][0].x = (int)items[kind].size() - 1;
+    rebuild |= items[kind][0].x != g->n_items[kind];
   }
   if (rebuild) {
-    drop_graph(g);
     for (int kind = 0; kind < W_NUM; kind++) {
-      const int n = (int)items[kind].size();
-      if (n > g->items_cap[kind]) {
-        FP_GTRY(g, hipStreamSynchronize(g->stream));
-        if (g->d_items[kind]) fp_pool_put(g->device, g->d_items[kind]);
-        g->d_items[kind] = nullptr;
-        g->items_cap[kind] = 0;
-        FP_GTRY(g, fp_pool_get(g->device, sizeof(int2) * std::max(n, 1), (void**)&g->d_items[kind], nullptr));
-        g->items_cap[kind] = n;
-      }
-      if (n) {
-        FP_GTRY(g, hipMemcpyAsync(g->d_items[kind], items[kind].data(), sizeof(int2) * n, hipMemcpyHostToDevice,
-                                  g->stream));
-      }
+      const int n = (int)items[kind].size() - 1;
+      FP_GTRY(g, reserve(g, g->d_items[kind], g->items_cap[kind], n + 1, moved));
+      if (moved) drop_graph(g);
+      FP_GTRY(g, hipMemcpyAsync(g->d_items[kind], items[kind].data(), sizeof(int2) * (n + 1), hipMemcpyHostToDevice,
+                                g->stream));
       g->n_items[kind] = n;
     }
     g->items_clanes = g->clanes;
     g->tune_items_ok = false;
   }
   // a tuning frame launches the solid contacts from each candidate's list
+  // (count first, then the items; tune_off[c] is the count's place)
   std::vector<int2> titems;
   if (g->tune_record && !g->tune_items_ok) {
     for (int c = 0; c < 4; c++) {
       g->tune_off[c] = (int)titems.size();
+      titems.push_back(make_int2(0, 0));
       for (int k = 0; k < ns; k++) {
         const int nb = fp_scene_blocks(g->scenes[k], W_SOLID, kTuneLanes[c]);
         for (int b = 0; b < nb; b++) titems.push_back(make_int2(k, b));
       }
-      g->tune_n[c] = (int)titems.size() - g->tune_off[c];
+      g->tune_n[c] = (int)titems.size() - g->tune_off[c] - 1;
+      titems[g->tune_off[c]].x = g->tune_n[c];
     }
     const int n = (int)titems.size();
-    if (n > g->tune_items_cap) {
-      FP_GTRY(g, hipStreamSynchronize(g->stream));
-      if (g->d_tune_items) fp_pool_put(g->device, g->d_tune_items);
-      g->d_tune_items = nullptr;
-      g->tune_items_cap = 0;
-      FP_GTRY(g, fp_pool_get(g->device, sizeof(int2) * std::max(n, 1), (void**)&g->d_tune_items, nullptr));
-      g->tune_items_cap = n;
-    }
-    if (n) {
-      FP_GTRY(g, hipMemcpyAsync(g->d_tune_items, titems.data(), sizeof(int2) * n, hipMemcpyHostToDevice,
-                                g->stream));
-    }
+    FP_GTRY(g, reserve(g, g->d_tune_items, g->tune_items_cap, n, moved));
+    FP_GTRY(g, hipMemcpyAsync(g->d_tune_items, titems.data(), sizeof(int2) * n, hipMemcpyHostToDevice, g->stream));
     g->tune_items_ok = true;
   }
   // the host vectors are read by the copies above before they go out of scope
@@ -527,11 +526,21 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
   const bool graphable = g->use_graph && !g->any_grabs && !profiling && !g->tune_record &&
                          !(FP_GRAPH_DEFER && g->members_changed && !g->gexec);
   if (graphable) {
-    if (!g->gexec || g->graph_sub != substeps || g->graph_iters != g->iters) {
+    // the captured frame replays whatever the scene table and the work lists
+    // hold; it is captured again when its structure no longer fits: the
+    // substep or iteration count, the contact pass's lane count (a kernel
+    // template), the bounds pass of unconstrained scenes, or work lists that
+    // outgrew (or shrank far below) the grids captured for them
+    bool fits = g->gexec && g->graph_sub == substeps && g->graph_iters == g->iters &&
+                g->graph_clanes == g->clanes && g->graph_unconstrained == g->any_unconstrained;
+    for (int k = 0; k < W_NUM && fits; k++)
+      fits = 2 * g->n_items[k] <= 3 * g->grid_cap[k] && 4 * fp_grid8(g->n_items[k]) >= g->grid_cap[k];
+    if (!fits) {
       drop_graph(g);
       int64_t slots = 0;
       for (fp_solver* s : g->scenes) slots += s->n;
       g->fork = FP_SIDE_STREAM && slots >= FP_SIDE_MIN;
+      for (int k = 0; k < W_NUM; k++) g->grid_cap[k] = fp_grid8(g->n_items[k]);
       if (g->fork && !g->side) {
         FP_GTRY(g, hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking));
         FP_GTRY(g, hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
@@ -551,8 +560,11 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
       FP_GTRY(g, e);
       g->graph = graph;
       FP_GTRY(g, hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0));
+      g->graph_captures++;
       g->graph_sub = substeps;
       g->graph_iters = g->iters;
+      g->graph_clanes = g->clanes;
+      g->graph_unconstrained = g->any_unconstrained;
     }
     FP_GTRY(g, hipGraphLaunch(g->gexec, g->stream));
     g->graph_launches++;
@@ -572,8 +584,8 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
 
 // Positions of every scene into one packed xyz buffer at each scene's offset.
 __global__ void __launch_bounds__(BLOCK)
-    kg_snapshot(const SceneArgs* __restrict__ S, const int2* __restrict__ W, int nw, float* __restrict__ out) {
-  FOR_ITEMS(S, W, nw) {
+    kg_snapshot(const SceneArgs* __restrict__ S, const int2* __restrict__ W, float* __restrict__ out) {
+  FOR_ITEMS(S, W) {
     const int i = lb * BLOCK + (int)threadIdx.x;
     if (i < A.n) {
       const float4 p = A.pos[i];

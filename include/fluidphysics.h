@@ -314,7 +314,11 @@ typedef struct fp_batch_info {
   int32_t contact_lanes;       /* lanes per solid the solid contact pass uses: tuned over the
                                   first frames (4, 8, 16, 32 timed, the fastest kept; 0 while
                                   tuning).  Results do not depend on it. */
-  int32_t reserved;
+  int32_t graph_captures;      /* frames captured into a graph so far: a captured frame keeps
+                                  replaying while members join and leave (its kernels read the
+                                  scene table and the work lists on the device); it is captured
+                                  again only when the substep or iteration count, the contact
+                                  lanes, or the work lists' size (by more than 1.5x) change */
 } fp_batch_info;
 
 int fp_batch_create(int32_t device, fp_batch** out);

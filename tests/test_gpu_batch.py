@@ -319,3 +319,44 @@ def test_unconstrained_scene_finishes_its_solids():
         if f % 10 == 9:
             _same(_snap(s2), ref_other[f], "%s frame %d" % (other.name, f))
     b.close()
+
+
+def test_captured_frame_survives_membership_changes():
+    """Members join and leave between replayed frames: the frame captured at
+    the start keeps replaying (its kernels read the scene table and the work
+    lists' counts on the device), and every member stays bit-identical to
+    stepping it alone."""
+    from fluidphysics_amd import Batch, from_scene
+    frames = 14
+    scs = _scenes([40, 41, 42, 43, 44], frames)
+    log("case graph across membership changes: %s" % [sc.name for sc in scs])
+    joins = {scs[3].name: 5, scs[4].name: 9}
+    leaves = {scs[1].name: 7}
+    ref = {sc.name: _alone(sc, frames - joins.get(sc.name, 0)) for sc in scs}
+    b = Batch()
+    b.contact_lanes(8)  # no tuning frames
+    ss = {}
+    for sc in scs[:3]:
+        ss[sc.name] = from_scene(sc)
+        b.add(ss[sc.name])
+    for f in range(frames):
+        for sc in scs:
+            if joins.get(sc.name) == f:
+                ss[sc.name] = from_scene(sc)
+                b.add(ss[sc.name])
+            if leaves.get(sc.name) == f:
+                b.remove(ss.pop(sc.name))
+        live = [sc for sc in scs if sc.name in ss]
+        for sc in live:
+            _cuts(ss[sc.name], sc, f - joins.get(sc.name, 0))
+        b.step(DT, SUB)
+        for sc in live:
+            _same(_snap(ss[sc.name]), ref[sc.name][f - joins.get(sc.name, 0)], "%s frame %d" % (sc.name, f))
+    inf = b.info()
+    log("  graph launches %d, captures %d over %d frames" % (inf.graph_launches, inf.graph_captures, frames))
+    # the first frame runs eagerly; every later one replays, and the
+    # membership changes (5 -> 3 -> 4 -> 3 -> 4 scenes of similar sizes) do
+    # not force a new capture each
+    assert inf.graph_launches == frames - 1
+    assert inf.graph_captures <= 2
+    b.close()
