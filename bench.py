@@ -21,7 +21,11 @@ a time (the `batched` object; --no-batched skips it).  A third serves an
 open-loop Poisson stream of scene requests with continuous batching under an
 HBM budget (fluidphysics_amd.serving) and reports frame p50/p99 and
 sustained throughput beside the per-scene path on the same stream (the
-`serving` object; --no-serving skips it).
+`serving` object; --no-serving skips it).  At the default 120 requests/s the
+batched server keeps up with the stream, so its throughput is the stream's
+load; `serving.capacity` repeats both paths on a heavier stream (240/s by
+default) to show each one's throughput where it, not the stream, is the
+limit, with the batched path's frame and queueing latency beside it.
 
 Printed: one JSON line with the metric, a `roofline` object for the dominant
 kernel and a `cpu_baseline` object.  The roofline's launch time is measured
@@ -354,6 +358,8 @@ def main():
     ap.add_argument("--batched-solo-frames", type=int, default=-1,
                     help="frames of the one-at-a-time comparison (-1: a third of --batched-steps, 0: skip)")
     ap.add_argument("--serving-rate", type=float, default=120.0)
+    ap.add_argument("--serving-capacity-rate", type=float, default=240.0,
+                    help="a second, heavier stream (0: skip): each path's throughput where it is the bottleneck")
     ap.add_argument("--serving-seconds", type=float, default=10.0)
     ap.add_argument("--serving-budget-gb", type=float, default=200.0)
     ap.add_argument("--no-serving", action="store_true")
@@ -438,6 +444,13 @@ def main():
     serving = None
     if not args.no_serving:
         serving = serving_leg(args.serving_rate, args.serving_seconds, args.serving_budget_gb * 1e9)
+        if args.serving_capacity_rate > 0:
+            serving["capacity"] = serving_leg(args.serving_capacity_rate, args.serving_seconds,
+                                              args.serving_budget_gb * 1e9)
+            c = serving["capacity"]
+            print("bench: serving at %g Hz: %.3e p-substeps/s, frame p99 %.2f ms, queue p99 %.2f ms; per-scene %.3e" %
+                  (args.serving_capacity_rate, c["batch"]["value"], c["batch"]["frame_ms_p99"],
+                   c["batch"]["queue_ms_p99"], c["per_scene"]["value"]), file=sys.stderr, flush=True)
         print("bench: serving %.3e p-substeps/s, frame p50 %.2f ms p99 %.2f ms; per-scene %.3e p99 %.2f ms" %
               (serving["batch"]["value"], serving["batch"]["frame_ms_p50"], serving["batch"]["frame_ms_p99"],
                serving["per_scene"]["value"], serving["per_scene"]["frame_ms_p99"]), file=sys.stderr, flush=True)

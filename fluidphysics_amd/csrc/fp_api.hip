@@ -146,7 +146,7 @@ void scene_buffers(fp_solver* s, F& take) {
   take(s->nbr, (size_t)(s->pair_slots / s->fluid_lanes + 1) * lanes64), take(s->ncount, lanes);
   take(s->pslot, N), take(s->count, T), take(s->cell_start, T + 1), take(s->entries_tmp, N);
   take(s->cell_entries, N), take(s->block_sums, (size_t)s->scan_blocks);
-  take(s->cons_ab, C), take(s->cent, C), take(s->erest, 2 * C), take(s->ealive, 2 * C);
+  take(s->cons_ab, C), take(s->cent, C), take(s->erest, 2 * C), take(s->ealive, 2 * C), take(s->ealive_snap, 2 * C);
   take(s->dgroup, std::max<size_t>(ns, 1));
   take(s->adj_off, ns + 1), take(s->adj, 2 * C), take(s->spos_c[0], std::max<size_t>(ns, 1));
   take(s->spos_c[1], std::max<size_t>(ns, 1)), take(s->dflags, (size_t)DF_WORDS), take(s->dpick_t, 1);
@@ -326,11 +326,31 @@ void recompute_groups_host(fp_solver* s, const std::vector<uint8_t>& alive) {
 int materialize_groups(fp_solver* s) {
   if (!s->groups_pending) return FP_OK;
   std::vector<uint8_t> al(s->C);
-  int rc = constraints_to_host(s, nullptr, al.data());
-  if (rc) return rc;
+  if (s->groups_from_snap) {
+    // the flags as they were when the labels fell due (kept on the device
+    // before a cut changed them)
+    std::vector<uint8_t> ea(2 * (size_t)s->C);
+    FP_TRY(s, d2h(s, ea.data(), s->ealive_snap, ea.size()));
+    for (int c = 0; c < s->C; c++) al[c] = ea[s->h_cent[c].x];
+  } else {
+    int rc = constraints_to_host(s, nullptr, al.data());
+    if (rc) return rc;
+  }
   recompute_groups_host(s, al);
   s->groups_pending = false;
+  s->groups_from_snap = false;
   s->alive_changed = false;
+  return FP_OK;
+}
+
+// Before a cut changes the alive flags while relabelled groups are due: keep
+// the flags the labels are due from in a device copy (no host round trip;
+// the labels are computed from it if the caller reads them before the next
+// step, whose end makes new labels due from the flags then).
+int defer_groups(fp_solver* s) {
+  if (!s->groups_pending || s->groups_from_snap || s->C == 0) return FP_OK;
+  FP_TRY(s, hipMemcpyAsync(s->ealive_snap, s->ealive, 2 * (size_t)s->C, hipMemcpyDeviceToDevice, s->stream));
+  s->groups_from_snap = true;
   return FP_OK;
 }
 
@@ -351,6 +371,7 @@ int settle(fp_solver* s) {
   s->steps_pending = false;
   if (s->groups_dirty) {
     s->groups_pending = true;
+    s->groups_from_snap = false;
     s->groups_dirty = false;
     s->alive_changed = false;  // the due labels are the current flags' partition
   }
@@ -769,6 +790,7 @@ int fp_reset_solid(fp_solver* s) {
   s->h_group.assign(ns, 0);
   s->groups_dirty = false;
   s->groups_pending = false;
+  s->groups_from_snap = false;
   s->cut_dirty = false;
   s->grab_round_len.clear();
   return FP_OK;
@@ -871,7 +893,7 @@ int fp_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[
   int rc = set_device(s);
   if (rc) return rc;
   if ((rc = settle(s))) return rc;
-  if ((rc = materialize_groups(s))) return rc;  // labels from the flags before the cut
+  if ((rc = defer_groups(s))) return rc;  // labels due from the flags before the cut
   FP_TRY(s, hipMemsetAsync(s->dflags + DF_CUT, 0, sizeof(int), s->stream));
   if ((rc = fp_launch_cut(s, o, d1, d2, nx, ny, nz))) return rc;
   int cnt = 0;
@@ -1431,6 +1453,7 @@ int fp_checkpoint_save(fp_solver* s, int32_t slot) {
   int rc = set_device(s);
   if (rc) return rc;
   if ((rc = settle(s))) return rc;
+  if (s->groups_from_snap && (rc = materialize_groups(s))) return rc;  // saved state keeps no deferred copy
   if ((int)s->checkpoints.size() <= slot) s->checkpoints.resize(slot + 1);
   fp_solver::Checkpoint& cp = s->checkpoints[slot];
   const size_t need = state_array_bytes(s);
@@ -1477,6 +1500,7 @@ int fp_checkpoint_restore(fp_solver* s, int32_t slot) {
   s->substeps_done = cp.substeps_done;
   s->h_group = cp.group;
   s->groups_pending = cp.groups_pending;
+  s->groups_from_snap = false;
   s->groups_dirty = false;
   s->cut_dirty = cp.cut_dirty;
   s->alive_changed = true;  // conservative: the host labels decide selection
@@ -1499,6 +1523,7 @@ int fp_state_export(fp_solver* s, void* buf, int64_t bytes) {
   if (!buf || bytes < need) return fp_fail(s, FP_EINVAL, "fp_state_export: buffer smaller than fp_state_size");
   if ((rc = set_device(s))) return rc;
   if ((rc = settle(s))) return rc;
+  if (s->groups_from_snap && (rc = materialize_groups(s))) return rc;  // the blob keeps no deferred copy
   StateHeader h;
   std::memset(&h, 0, sizeof(h));
   std::memcpy(h.magic, "FPSTATE", 8);
@@ -1563,6 +1588,7 @@ int fp_state_import(fp_solver* s, const void* buf, int64_t bytes) {
   s->fluid_next = h.fluid_next;
   s->substeps_done = h.substeps_done;
   s->groups_pending = (h.flags & 1u) != 0;
+  s->groups_from_snap = false;
   s->groups_dirty = false;
   s->cut_dirty = (h.flags & 2u) != 0;
   s->alive_changed = true;  // conservative: the host labels decide selection

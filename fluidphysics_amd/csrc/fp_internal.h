@@ -169,6 +169,7 @@ struct fp_solver {
   int* dgroup = nullptr;      // component labels, uploaded for a selection
   float* erest = nullptr;     // per CSR entry
   uint8_t* ealive = nullptr;  // per CSR entry
+  uint8_t* ealive_snap = nullptr;  // ealive when due labels were deferred past a cut (groups_from_snap)
   int* adj_off = nullptr;
   int2* adj = nullptr;
   float4* spos_c[2] = {nullptr, nullptr};
@@ -201,6 +202,7 @@ struct fp_solver {
   std::vector<int32_t> h_group;
   bool groups_dirty = false;
   bool groups_pending = false;  // a relabel is due; run when group is read or alive is about to change
+  bool groups_from_snap = false;  // ... from the flags in ealive_snap (a cut changed ealive since)
   bool steps_pending = false;
   bool cut_dirty = false;
   bool cut_dirty_at_step = false;

@@ -372,3 +372,42 @@ def test_config2_300_frames_statistics_vs_jacobi_oracle():
     assert abs(np.mean(ke) / rke - 1) < 0.01
     assert abs(np.mean(rho) / rrho - 1) < 0.01
     assert abs(alive / ral - 1) < 0.01
+
+
+def test_labels_due_from_a_tear_survive_a_cut_before_they_are_read():
+    """A step that tore constraints makes new group labels due (solver.js
+    relabels at the step's end, :238-241); a cut before the caller reads
+    them must not change them: group then is the union-find of the flags at
+    that step's end, and after the next step that of the flags after the
+    cut.  (The library keeps the due flags in a device copy across the cut
+    and computes the labels only when group is read.)"""
+    from fluidphysics_amd import from_scene, scenes
+    sc = scenes.single_gpu_scene(frames=40)
+    log("case due labels across a cut: %s" % sc.name)
+    s = from_scene(sc)
+    cx, cy, cz = sc.cube[:3]
+    s.grabs = [{"i": i, "x": cx - 3.0, "y": cy + 3.0, "z": cz} for i in range(40)]
+    torn = False
+    for _ in range(12):
+        before = int(s.read_constraints()["alive"].sum())
+        s.step(DT, 5)
+        torn = int(s.read_constraints()["alive"].sum()) < before
+        if torn:
+            break
+    assert torn, "the grab tore nothing"
+    s.grabs = []
+    alive_due = s.read_constraints()["alive"].copy()
+    d1 = np.array([0.0, -1.0, -0.9]) / np.linalg.norm([0.0, -1.0, -0.9])
+    d2 = np.array([0.0, -1.0, 0.9]) / np.linalg.norm([0.0, -1.0, 0.9])
+    assert s.cut(cx + 0.013, cy + 5.0, cz, *d1, *d2) > 0
+    from oracle.oracle import OracleSolver
+    o = OracleSolver(sc.params_with(), sc.max_fluid, "f32", "gs")
+    o.add_solid_cube(*sc.cube)
+    o.write_constraints(None, alive_due)
+    o.recompute_groups()
+    assert np.array_equal(s.read_state()["group"], o.read_state()["group"]), "labels changed by the cut"
+    s.step(DT, 5)
+    o.write_constraints(None, s.read_constraints()["alive"])
+    o.recompute_groups()
+    assert np.array_equal(s.read_state()["group"], o.read_state()["group"])
+    s.close()
