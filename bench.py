@@ -232,6 +232,9 @@ def batched_leg(count, frames, warmup, solo_frames, seed=4):
         b.step(dt, sub)
         lat.append(time.perf_counter() - t)
     n1 = active()
+    slow = sorted(range(len(lat)), key=lambda k: -lat[k])[:3]
+    print("bench: batched slowest frames %s" % ", ".join("%d: %.2f ms" % (warmup + k, lat[k] * 1e3) for k in slow),
+          file=sys.stderr, flush=True)
     inf = b.info()
     n_mean = 0.5 * (n0 + n1)
     value = n_mean * sub * frames / sum(lat)
@@ -436,7 +439,9 @@ def main():
     batched = None
     if not args.no_batched:
         solo = args.batched_solo_frames if args.batched_solo_frames >= 0 else max(10, args.batched_steps // 3)
-        batched = batched_leg(args.batched_scenes, args.batched_steps, max(3, args.warmup // 2), solo)
+        # warm-up covers the contact-lane tuning frames and the frame that
+        # captures the graph (one-time costs)
+        batched = batched_leg(args.batched_scenes, args.batched_steps, max(6, args.warmup // 2), solo)
         print("bench: batched %.3e p-substeps/s, frame p50 %.2f ms p99 %.2f ms; one at a time %s" %
               (batched["value"], batched["frame_ms_p50"], batched["frame_ms_p99"],
                batched["one_at_a_time"] and "%.3e" % batched["one_at_a_time"]["value"]), file=sys.stderr, flush=True)

@@ -57,6 +57,9 @@ _FIELDS = {
     "constraintIters": "constraint_iters",
     "gravityX": "gravity_x",
     "gravityZ": "gravity_z",
+    "tetVolume": "tet_volume",
+    "solidContactCache": "solid_contact_cache",
+    "tearGap": "tear_gap",
 }
 
 

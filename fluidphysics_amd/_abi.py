@@ -53,6 +53,9 @@ class FpParams(ctypes.Structure):
         ("contact_scale", ctypes.c_double),
         ("gravity_x", ctypes.c_double),
         ("gravity_z", ctypes.c_double),
+        ("tet_volume", ctypes.c_int32),
+        ("solid_contact_cache", ctypes.c_int32),
+        ("tear_gap", ctypes.c_double),
     ]
 
     def to_dict(self):
@@ -77,7 +80,7 @@ REFERENCE_DEFAULTS = dict(
     wall_half_thick=0.07, cull_radius=7.0, max_speed=14.0, grab_stiffness=0.3,
     cell_size=0.22, constraint_iters=3, table_size=0,
     pair_slots=0, fluid_lanes=0, jacobi_omega=8.0, contact_scale=1.0, gravity_x=0.0,
-    gravity_z=0.0,
+    gravity_z=0.0, tet_volume=0, solid_contact_cache=0, tear_gap=1.3,
 )
 
 

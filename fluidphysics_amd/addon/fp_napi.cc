@@ -148,6 +148,9 @@ const Field kFields[] = {
     FD("contactScale", contact_scale),
     FD("gravityX", gravity_x),
     FD("gravityZ", gravity_z),
+    FI("tetVolume", tet_volume),
+    FI("solidContactCache", solid_contact_cache),
+    FD("tearGap", tear_gap),
 };
 
 void params_from_object(napi_env env, napi_value obj, fp_params* p) {

@@ -446,6 +446,9 @@ void fp_params_default(fp_params* p) {
   p->table_size = 0;
   p->jacobi_omega = 8.0;  // with 3 iterations: the reference's 300-frame statistics at its own constants
   p->contact_scale = 1.0;
+  p->tet_volume = 0;
+  p->solid_contact_cache = 0;
+  p->tear_gap = 1.3;
 }
 
 int fp_create(const fp_params* params, int32_t max_fluid, int32_t device, fp_solver** out) {
@@ -520,8 +523,12 @@ int fp_set_params(fp_solver* s, const fp_params* p) {
   int rc = check_params(s, *p);
   if (rc) return rc;
   if (s->has_cube &&
-      (p->table_size != s->P.table_size || p->pair_slots != s->P.pair_slots || p->fluid_lanes != s->P.fluid_lanes))
-    return fp_fail(s, FP_ESTATE, "table_size / pair_slots / fluid_lanes are fixed once the scene is allocated");
+      (p->table_size != s->P.table_size || p->pair_slots != s->P.pair_slots || p->fluid_lanes != s->P.fluid_lanes ||
+       (p->tet_volume != 0) != (s->P.tet_volume != 0) ||
+       (p->solid_contact_cache != 0) != (s->P.solid_contact_cache != 0)))
+    return fp_fail(s, FP_ESTATE,
+                   "table_size / pair_slots / fluid_lanes / tet_volume / solid_contact_cache are fixed once the scene "
+                   "is allocated");
   s->P = *p;
   s->version++;
   return FP_OK;

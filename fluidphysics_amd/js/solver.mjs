@@ -37,7 +37,7 @@ const TUNING = ['gravity', 'solidRadius', 'fluidRadius', 'compliance', 'plasticT
   'plasticRate', 'breakStrain', 'fluidH', 'restDensity', 'stiffness', 'nearStiffness',
   'viscosity', 'solidDamping', 'fluidDamping', 'floorFrictionSolid', 'floorFrictionFluid',
   'cullRadius', 'maxSpeed', 'grabStiffness', 'cellSize', 'constraintIters', 'jacobiOmega',
-  'contactScale', 'gravityX', 'gravityZ'];
+  'contactScale', 'gravityX', 'gravityZ', 'tetVolume', 'solidContactCache', 'tearGap'];
 
 export class Solver {
   constructor({ maxFluid = 2600, params = {}, device = -1 } = {}) {

@@ -84,6 +84,20 @@ typedef struct fp_params {
   double gravity_x;            /* gravity along x and z (a tilted basin; the reference has y only,
                                   solver.js:15); default 0 */
   double gravity_z;
+  /* Optional solid modes of the reference's GPU twin (src/gpu/wgsl.js; src/solver.js has
+     neither).  Both are fixed once fp_add_solid_cube has allocated the scene, and both act
+     inside the constraint iterations (none without them). */
+  int32_t tet_volume;          /* 1: five volume-preserving tetrahedra per lattice cell, projected
+                                  Jacobi-style in every constraint iteration after the distance
+                                  constraints (wgsl.js:423-468; tets built at sim.js:144-173);
+                                  torn when an edge outgrows its lattice rest by tear_gap solid
+                                  diameters (wgsl.js:471-489), cut with the constraints
+                                  (wgsl.js:492-519).  0 (default): off */
+  int32_t solid_contact_cache; /* 1: solid-solid contacts listed once per substep with a margin
+                                  of one solid diameter and resolved in every constraint
+                                  iteration (wgsl.js:532-596) instead of once in the contact pass.
+                                  0 (default): off */
+  double tear_gap;             /* tet edge tear margin in solid diameters (sim.js:49: 1.3) */
 } fp_params;
 
 /* Opaque solver handle (one scene, one HIP stream). */

@@ -69,7 +69,22 @@ typedef struct Oracle {
   double *jrest; /* Jacobi constraint snapshot */
   uint8_t *jalive;
   int mode;
+  /* the GPU twin's optional solid modes (Jacobi mode only): volume tets and
+     the solid contact cache */
+  int cubeN;
+  double spacing;
+  int numTets;
+  int32_t *tetV;      /* 4 vertices per tet, ordered so the rest volume is positive */
+  double *tetV0;      /* rest volume */
+  uint8_t *tetAlive;
+  int32_t *tetAdjOff, *tetAdj; /* CSR: tets incident to each solid */
+  double *tetScale;   /* 1 / max(incident tets of its 4 vertices) */
+  int32_t *cacheIds, *cacheN;
+  int cacheOverflow;
+  ST *xpos;           /* snapshot of the solids for the extra step */
 } Oracle;
+
+#define OR_CACHE_K 16 /* solid contact partners kept per solid */
 
 /* ---- helpers -------------------------------------------------------- */
 
@@ -138,6 +153,7 @@ void or_params_default(fp_params* p) {
   p->table_size = 0;
   p->jacobi_omega = 1.0;
   p->contact_scale = 1.0;
+  p->tear_gap = 1.3;
 }
 
 /* ---- lifetime ------------------------------------------------------- */
@@ -159,7 +175,8 @@ static void free_arrays(Oracle* o) {
                   o->rho,       o->rhoNear,  o->press,     o->pressNear, o->dispX,   o->dispY,
                   o->dispZ,     o->nbrCnt,   o->consA,     o->consB,     o->consRest, o->consAlive,
                   o->snapPos,   o->snapRest, o->adjOff,    o->adjCons,   o->grabI,   o->grabXYZ,
-                  o->jpos,      o->jrest,    o->jalive};
+                  o->jpos,      o->jrest,    o->jalive,    o->tetV,      o->tetV0,   o->tetAlive,
+                  o->tetAdjOff, o->tetAdj,   o->tetScale,  o->cacheIds,  o->cacheN,  o->xpos};
   for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
 }
 
@@ -206,6 +223,77 @@ static int alloc_state(Oracle* o, int n) {
           o->jpos)
              ? 0
              : -1;
+}
+
+/* ---- the GPU twin's volume tets (src/gpu/sim.js:144-194) ------------- */
+
+static double tet_signed_volume(const ST* p, int a, int b, int c, int d) {
+  const double ab[3] = {(double)p[3 * b] - p[3 * a], (double)p[3 * b + 1] - p[3 * a + 1], (double)p[3 * b + 2] - p[3 * a + 2]};
+  const double ac[3] = {(double)p[3 * c] - p[3 * a], (double)p[3 * c + 1] - p[3 * a + 1], (double)p[3 * c + 2] - p[3 * a + 2]};
+  const double ad[3] = {(double)p[3 * d] - p[3 * a], (double)p[3 * d + 1] - p[3 * a + 1], (double)p[3 * d + 2] - p[3 * a + 2]};
+  return ((ab[1] * ac[2] - ab[2] * ac[1]) * ad[0] + (ab[2] * ac[0] - ab[0] * ac[2]) * ad[1] +
+          (ab[0] * ac[1] - ab[1] * ac[0]) * ad[2]) / 6.0;
+}
+
+/* Five tets per lattice cell, the split mirrored on checkerboard parity so
+ * faces match (sim.js:151-166).  The twin keeps its vertex order and so
+ * half its rest volumes come out negative, which its solver then treats as
+ * dead tets (wgsl.js:432); here the last two vertices are swapped wherever
+ * that makes the rest volume positive, so every tet holds its volume. */
+static void build_tets(Oracle* o) {
+  const int n = o->cubeN, ns = o->solidCount;
+  const int cells = n > 1 ? (n - 1) * (n - 1) * (n - 1) : 0;
+  o->numTets = 5 * cells;
+  o->tetV = (int32_t*)malloc((4 * (size_t)o->numTets + 1) * sizeof(int32_t));
+  o->tetV0 = (double*)malloc(((size_t)o->numTets + 1) * sizeof(double));
+  o->tetAlive = (uint8_t*)malloc((size_t)o->numTets + 1);
+  o->tetScale = (double*)malloc(((size_t)o->numTets + 1) * sizeof(double));
+  int t = 0;
+#define IDX(ax, ay, az) (((ax) * n + (ay)) * n + (az))
+  for (int ix = 0; ix < n - 1; ix++)
+    for (int iy = 0; iy < n - 1; iy++)
+      for (int iz = 0; iz < n - 1; iz++) {
+        const int A = IDX(ix, iy, iz), B = IDX(ix + 1, iy, iz), C = IDX(ix + 1, iy + 1, iz), D = IDX(ix, iy + 1, iz);
+        const int E = IDX(ix, iy, iz + 1), F = IDX(ix + 1, iy, iz + 1), G = IDX(ix + 1, iy + 1, iz + 1),
+                  H = IDX(ix, iy + 1, iz + 1);
+        const int even[5][4] = {{A, C, F, H}, {A, B, C, F}, {A, C, D, H}, {A, E, F, H}, {C, F, G, H}};
+        const int odd[5][4] = {{B, D, E, G}, {A, B, D, E}, {B, C, D, G}, {D, E, H, G}, {B, E, F, G}};
+        const int(*five)[4] = (ix + iy + iz) % 2 == 0 ? even : odd;
+        for (int k = 0; k < 5; k++, t++) {
+          int v[4] = {five[k][0], five[k][1], five[k][2], five[k][3]};
+          double V = tet_signed_volume(o->pos, v[0], v[1], v[2], v[3]);
+          if (V < 0) {
+            const int x = v[2];
+            v[2] = v[3];
+            v[3] = x;
+            V = -V;
+          }
+          for (int q = 0; q < 4; q++) o->tetV[4 * t + q] = v[q];
+          o->tetV0[t] = V;
+          o->tetAlive[t] = 1;
+        }
+      }
+#undef IDX
+  o->tetAdjOff = (int32_t*)calloc(ns + 1, sizeof(int32_t));
+  o->tetAdj = (int32_t*)malloc((4 * (size_t)o->numTets + 1) * sizeof(int32_t));
+  for (int k = 0; k < 4 * o->numTets; k++) o->tetAdjOff[o->tetV[k] + 1]++;
+  for (int i = 0; i < ns; i++) o->tetAdjOff[i + 1] += o->tetAdjOff[i];
+  int32_t* fill = (int32_t*)calloc(ns + 1, sizeof(int32_t));
+  for (int k = 0; k < o->numTets; k++)
+    for (int q = 0; q < 4; q++) {
+      const int i = o->tetV[4 * k + q];
+      o->tetAdj[o->tetAdjOff[i] + fill[i]++] = k;
+    }
+  free(fill);
+  for (int k = 0; k < o->numTets; k++) {
+    int m = 1;
+    for (int q = 0; q < 4; q++) {
+      const int i = o->tetV[4 * k + q];
+      const int deg = o->tetAdjOff[i + 1] - o->tetAdjOff[i];
+      if (deg > m) m = deg;
+    }
+    o->tetScale[k] = 1.0 / m;
+  }
 }
 
 /* addSolidCube (solver.js:72-123). */
@@ -281,6 +369,12 @@ int or_add_solid_cube(Oracle* o, double cx, double cy, double cz, int n, double 
   memcpy(o->snapPos, o->pos, 3 * (size_t)id * sizeof(ST));
   o->snapRest = (double*)malloc((c + 1) * sizeof(double));
   memcpy(o->snapRest, o->consRest, c * sizeof(double));
+  o->cubeN = n;
+  o->spacing = spacing;
+  build_tets(o);
+  o->cacheIds = (int32_t*)calloc((size_t)id * OR_CACHE_K + 1, sizeof(int32_t));
+  o->cacheN = (int32_t*)calloc(id + 1, sizeof(int32_t));
+  o->xpos = (ST*)calloc(3 * (size_t)id + 1, sizeof(ST));
   return 0;
 }
 
@@ -292,6 +386,7 @@ void or_reset_solid(Oracle* o) {
   memcpy(o->consRest, o->snapRest, o->numCons * sizeof(double));
   for (int c = 0; c < o->numCons; c++) o->consAlive[c] = 1;
   for (int i = 0; i < o->solidCount; i++) o->group[i] = 0;
+  for (int t = 0; t < o->numTets; t++) o->tetAlive[t] = 1;
   o->nGrabs = 0;
 }
 
@@ -453,6 +548,8 @@ static void solve_contacts_jacobi(Oracle* o) {
             if (y == x || !o->active[y]) continue;
             const int solidY = o->type[y] == 1;
             if (!solidX && !solidY) continue;
+            /* with the contact cache, solid pairs are resolved in the constraint iterations */
+            if (solidX && solidY && o->P.solid_contact_cache) continue;
             if (solidX && solidY && bonded(o, x < y ? x : y, x < y ? y : x)) continue;
             const int y3 = 3 * y;
             const double ddx = (double)s[x3] - s[y3], ddy = (double)s[x3 + 1] - s[y3 + 1],
@@ -665,6 +762,207 @@ static void solve_constraints_jacobi_iter(Oracle* o, double h) {
   }
 }
 
+/* ---- the GPU twin's optional solid modes (Jacobi mode) --------------- */
+
+static int lattice_adjacent(const Oracle* o, int i, int j) {
+  const int n = o->cubeN;
+  const int dx = i / (n * n) - j / (n * n), dy = (i / n) % n - (j / n) % n, dz = i % n - j % n;
+  return abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1;
+}
+
+/* Rest distance of two lattice sites (wgsl.js:374-381). */
+static double lattice_rest(const Oracle* o, int a, int b) {
+  const int n = o->cubeN;
+  const double dx = a / (n * n) - b / (n * n), dy = (a / n) % n - (b / n) % n, dz = a % n - b % n;
+  return o->spacing * sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+/* Contact distance of two solids: one diameter, or just under the lattice
+ * spacing for lattice neighbours whose bond is gone (a cut face, which then
+ * rests at the spacing instead of being pushed apart; wgsl.js:537-541). */
+static double solid_rsum(const Oracle* o, int i, int j) {
+  const double rsum = 2 * o->P.solid_radius;
+  return lattice_adjacent(o, i, j) ? fmin(rsum, o->spacing * 0.97) : rsum;
+}
+
+/* Collection margin: a solid diameter (wgsl.js:549), but never past the
+ * cell size, so the 27 cells around a solid hold every partner in range. */
+static double cache_margin(const Oracle* o) {
+  return fmax(0.0, fmin(2 * o->P.solid_radius, o->P.cell_size - 2 * o->P.solid_radius));
+}
+
+/* collectContacts (wgsl.js:543-570): for every active solid, the solids
+ * within contact distance + margin that are not bonded to it, from the grid
+ * of the current positions; ascending index, the OR_CACHE_K smallest kept. */
+static void collect_contacts(Oracle* o) {
+  const ST* p = o->pos;
+  const int32_t *cs = o->cellStart, *ce = o->cellEntries;
+  const double inv = inv_cell(o), margin = cache_margin(o);
+  for (int i = 0; i < o->solidCount; i++) {
+    o->cacheN[i] = 0;
+    if (!o->active[i]) continue;
+    int32_t* ids = o->cacheIds + (size_t)i * OR_CACHE_K;
+    int cnt = 0;
+    const int i3 = 3 * i;
+    const double cxf = floor(p[i3] * inv), cyf = floor(p[i3 + 1] * inv), czf = floor(p[i3 + 2] * inv);
+    uint32_t seen[27];
+    int nseen = 0;
+    for (int dx = -1; dx <= 1; dx++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dz = -1; dz <= 1; dz++) {
+          const uint32_t hsh = js_hash(o, cxf + dx, cyf + dy, czf + dz);
+          int dup = 0;
+          for (int q = 0; q < nseen; q++) dup |= seen[q] == hsh;
+          if (dup) continue;
+          seen[nseen++] = hsh;
+          for (int e = cs[hsh]; e < cs[hsh + 1]; e++) {
+            const int j = ce[e];
+            if (j == i || j >= o->solidCount) continue;
+            const int j3 = 3 * j;
+            const double ddx = (double)p[i3] - p[j3], ddy = (double)p[i3 + 1] - p[j3 + 1],
+                         ddz = (double)p[i3 + 2] - p[j3 + 2];
+            const double lim = solid_rsum(o, i, j) + margin;
+            if (ddx * ddx + ddy * ddy + ddz * ddz >= lim * lim) continue;
+            if (bonded(o, i < j ? i : j, i < j ? j : i)) continue;
+            /* insert j, keeping the smallest OR_CACHE_K indices in order */
+            if (cnt == OR_CACHE_K && j > ids[OR_CACHE_K - 1]) {
+              o->cacheOverflow++;
+              continue;
+            }
+            if (cnt == OR_CACHE_K) o->cacheOverflow++;
+            int k = cnt < OR_CACHE_K ? cnt++ : OR_CACHE_K - 1;
+            while (k > 0 && ids[k - 1] > j) {
+              ids[k] = ids[k - 1];
+              k--;
+            }
+            ids[k] = j;
+          }
+        }
+    o->cacheN[i] = cnt;
+  }
+}
+
+static void cap_vec(double* v, double lim) {
+  const double m2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  if (m2 > lim * lim) {
+    const double f = lim / sqrt(m2);
+    v[0] *= f;
+    v[1] *= f;
+    v[2] *= f;
+  }
+}
+
+/* The extra step of one constraint iteration, from the snapshot of the
+ * solids after the distance constraints, every active solid gathering
+ *   tets (solveTets, wgsl.js:423-468): for each alive incident tet, with
+ *     V = (pb - pa) x (pc - pa) . (pd - pa) / 6, C = V - V0 and the volume
+ *     gradients g (summing to zero), the XPBD zero-compliance projection
+ *     dp_i = -C w_i g_i / sum_q(w_q |g_q|^2) scaled by 1 / max(incident tets
+ *     of the 4 vertices), each tet's share capped at half a solid radius;
+ *   contacts (solidContacts, wgsl.js:573-596): for each cached partner
+ *     closer than rsum, dd (rsum - d) / d * w_i / (w_i + w_j) * 0.5;
+ * each sum capped at half a solid radius (applySolidContacts,
+ * wgsl.js:598-606). */
+static void solid_extra_step(Oracle* o) {
+  const int ns = o->solidCount;
+  ST* p = o->pos;
+  ST* x = o->xpos;
+  memcpy(x, p, 3 * (size_t)ns * sizeof(ST));
+  const double cap = 0.5 * o->P.solid_radius;
+  for (int i = 0; i < ns; i++) {
+    if (!o->active[i]) continue;
+    double dt[3] = {0, 0, 0}, dc[3] = {0, 0, 0};
+    if (o->P.tet_volume) {
+      for (int k = o->tetAdjOff[i]; k < o->tetAdjOff[i + 1]; k++) {
+        const int t = o->tetAdj[k];
+        if (!o->tetAlive[t]) continue;
+        const int32_t* v = o->tetV + 4 * t;
+        double q[4][3];
+        for (int m = 0; m < 4; m++)
+          for (int c = 0; c < 3; c++) q[m][c] = x[3 * v[m] + c];
+        double b[3], c[3], d[3];
+        for (int m = 0; m < 3; m++) {
+          b[m] = q[1][m] - q[0][m];
+          c[m] = q[2][m] - q[0][m];
+          d[m] = q[3][m] - q[0][m];
+        }
+        double g[4][3];
+        /* gb = (c x d) / 6, gc = (d x b) / 6, gd = (b x c) / 6, ga = -(gb + gc + gd) */
+        g[1][0] = (c[1] * d[2] - c[2] * d[1]) / 6;
+        g[1][1] = (c[2] * d[0] - c[0] * d[2]) / 6;
+        g[1][2] = (c[0] * d[1] - c[1] * d[0]) / 6;
+        g[2][0] = (d[1] * b[2] - d[2] * b[1]) / 6;
+        g[2][1] = (d[2] * b[0] - d[0] * b[2]) / 6;
+        g[2][2] = (d[0] * b[1] - d[1] * b[0]) / 6;
+        g[3][0] = (b[1] * c[2] - b[2] * c[1]) / 6;
+        g[3][1] = (b[2] * c[0] - b[0] * c[2]) / 6;
+        g[3][2] = (b[0] * c[1] - b[1] * c[0]) / 6;
+        for (int m = 0; m < 3; m++) g[0][m] = -(g[1][m] + g[2][m] + g[3][m]);
+        const double V = g[3][0] * d[0] + g[3][1] * d[1] + g[3][2] * d[2];
+        const double C = V - o->tetV0[t];
+        double denom = 0;
+        for (int m = 0; m < 4; m++)
+          denom += o->invMass[v[m]] * (g[m][0] * g[m][0] + g[m][1] * g[m][1] + g[m][2] * g[m][2]);
+        if (denom < 1e-30) continue;
+        int me = 0;
+        while (v[me] != i) me++;
+        const double f = -C / denom * o->invMass[i] * o->tetScale[t];
+        double c1[3] = {g[me][0] * f, g[me][1] * f, g[me][2] * f};
+        cap_vec(c1, cap);
+        dt[0] += c1[0];
+        dt[1] += c1[1];
+        dt[2] += c1[2];
+      }
+      cap_vec(dt, cap);
+    }
+    if (o->P.solid_contact_cache) {
+      const double wi = o->invMass[i];
+      for (int k = 0; k < o->cacheN[i]; k++) {
+        const int j = o->cacheIds[(size_t)i * OR_CACHE_K + k];
+        const double ddx = (double)x[3 * i] - x[3 * j], ddy = (double)x[3 * i + 1] - x[3 * j + 1],
+                     ddz = (double)x[3 * i + 2] - x[3 * j + 2];
+        const double rsum = solid_rsum(o, i, j);
+        const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        if (d2 >= rsum * rsum || d2 < 1e-12) continue;
+        const double wsum = wi + o->invMass[j];
+        if (wsum == 0) continue;
+        const double d = sqrt(d2);
+        const double f = (rsum - d) / d * wi / wsum * 0.5;
+        dc[0] += ddx * f;
+        dc[1] += ddy * f;
+        dc[2] += ddz * f;
+      }
+      cap_vec(dc, cap);
+    }
+    for (int m = 0; m < 3; m++) p[3 * i + m] = (ST)(x[3 * i + m] + dt[m] + dc[m]);
+  }
+}
+
+/* tetUpdate (wgsl.js:471-489): a tet dies when an edge outgrows its lattice
+ * rest by tear_gap solid diameters. */
+static void tet_update(Oracle* o) {
+  const ST* p = o->pos;
+  const double gap = o->P.tear_gap * 2 * o->P.solid_radius;
+  for (int t = 0; t < o->numTets; t++) {
+    if (!o->tetAlive[t]) continue;
+    const int32_t* v = o->tetV + 4 * t;
+    for (int q = 0; q < 3 && o->tetAlive[t]; q++)
+      for (int r = q + 1; r < 4; r++) {
+        const int a = v[q], b = v[r];
+        const double dx = (double)p[3 * a] - p[3 * b], dy = (double)p[3 * a + 1] - p[3 * b + 1],
+                     dz = (double)p[3 * a + 2] - p[3 * b + 2];
+        if (sqrt(dx * dx + dy * dy + dz * dz) > lattice_rest(o, a, b) + gap) {
+          o->tetAlive[t] = 0;
+          break;
+        }
+      }
+  }
+}
+
+static int extra_modes(const Oracle* o) {
+  return o->mode == OR_MODE_JACOBI && (o->P.tet_volume || o->P.solid_contact_cache) && o->solidCount > 0;
+}
+
 /* ---- bounds (solver.js:389-427) -------------------------------------- */
 
 static double js_sign_or_one(double q) {
@@ -814,6 +1112,7 @@ void or_substep(Oracle* o, double h) {
   integrate(o, h);
   apply_grabs(o);
   or_build_grid(o);
+  if (extra_modes(o) && o->P.solid_contact_cache) collect_contacts(o);
   if (o->mode == OR_MODE_JACOBI)
     solve_contacts_jacobi(o);
   else
@@ -821,11 +1120,14 @@ void or_substep(Oracle* o, double h) {
   solve_fluid(o);
   const int iters = o->P.constraint_iters > 0 ? o->P.constraint_iters : 1;
   for (int it = 0; it < iters; it++) {
-    if (o->mode == OR_MODE_JACOBI)
+    if (o->mode == OR_MODE_JACOBI) {
       solve_constraints_jacobi_iter(o, h);
-    else
+      if (extra_modes(o)) solid_extra_step(o);
+    } else {
       solve_constraints_gs(o, h);
+    }
   }
+  if (extra_modes(o) && o->P.tet_volume) tet_update(o);
   solve_bounds(o);
   const double dampS = fmax(0, 1 - o->P.solid_damping * h);
   const double dampF = fmax(0, 1 - o->P.fluid_damping * h);
@@ -876,11 +1178,35 @@ void or_solve_contacts(Oracle* o) {
 
 void or_solve_constraints(Oracle* o, double h, int iters) {
   for (int it = 0; it < iters; it++) {
-    if (o->mode == OR_MODE_JACOBI)
+    if (o->mode == OR_MODE_JACOBI) {
       solve_constraints_jacobi_iter(o, h);
-    else
+      if (extra_modes(o)) solid_extra_step(o);
+    } else {
       solve_constraints_gs(o, h);
+    }
   }
+}
+
+/* Test hooks of the optional modes: the contact cache of the current
+ * positions (builds the grid), one extra step alone, the tet tear check, and
+ * the tets' state. */
+void or_collect_contacts(Oracle* o) {
+  or_build_grid(o);
+  collect_contacts(o);
+}
+void or_solid_extra_step(Oracle* o) { solid_extra_step(o); }
+void or_tet_update(Oracle* o) { tet_update(o); }
+int or_num_tets(const Oracle* o) { return o->numTets; }
+void or_read_tets(const Oracle* o, int32_t* v, double* v0, uint8_t* alive) {
+  if (v) memcpy(v, o->tetV, 4 * (size_t)o->numTets * sizeof(int32_t));
+  if (v0) memcpy(v0, o->tetV0, (size_t)o->numTets * sizeof(double));
+  if (alive) memcpy(alive, o->tetAlive, (size_t)o->numTets);
+}
+void or_write_tet_alive(Oracle* o, const uint8_t* alive) { memcpy(o->tetAlive, alive, (size_t)o->numTets); }
+void or_read_cache(const Oracle* o, int32_t* ids, int32_t* n, int32_t* overflow) {
+  if (ids) memcpy(ids, o->cacheIds, (size_t)o->solidCount * OR_CACHE_K * sizeof(int32_t));
+  if (n) memcpy(n, o->cacheN, (size_t)o->solidCount * sizeof(int32_t));
+  if (overflow) *overflow = o->cacheOverflow;
 }
 
 void or_step(Oracle* o, double dt, int substeps) {
@@ -894,6 +1220,23 @@ void or_step(Oracle* o, double dt, int substeps) {
 
 /* ---- knife (solver.js:458-486) --------------------------------------- */
 
+/* Does segment a-b cross the wedge (solver.js:466-483, double arithmetic in
+ * the reference's operation order)? */
+static int segment_in_wedge(const ST* p, int a, int b, double ox, double oy, double oz, double d1x, double d1y,
+                            double d1z, double d2x, double d2y, double d2z, double nx, double ny, double nz) {
+  const int a3 = a * 3, b3 = b * 3;
+  const double sa = ((double)p[a3] - ox) * nx + ((double)p[a3 + 1] - oy) * ny + ((double)p[a3 + 2] - oz) * nz;
+  const double sb = ((double)p[b3] - ox) * nx + ((double)p[b3 + 1] - oy) * ny + ((double)p[b3 + 2] - oz) * nz;
+  if (sa * sb >= 0) return 0;
+  const double t = sa / (sa - sb);
+  const double px = p[a3] + ((double)p[b3] - p[a3]) * t - ox;
+  const double py = p[a3 + 1] + ((double)p[b3 + 1] - p[a3 + 1]) * t - oy;
+  const double pz = p[a3 + 2] + ((double)p[b3 + 2] - p[a3 + 2]) * t - oz;
+  const double c1 = (d1y * pz - d1z * py) * nx + (d1z * px - d1x * pz) * ny + (d1x * py - d1y * px) * nz;
+  const double c2 = (py * d2z - pz * d2y) * nx + (pz * d2x - px * d2z) * ny + (px * d2y - py * d2x) * nz;
+  return !(c1 < 0 || c2 < 0);
+}
+
 int or_cut(Oracle* o, double ox, double oy, double oz, double d1x, double d1y, double d1z, double d2x,
            double d2y, double d2z) {
   const double nx = d1y * d2z - d1z * d2y;
@@ -905,20 +1248,23 @@ int or_cut(Oracle* o, double ox, double oy, double oz, double d1x, double d1y, d
   int cuts = 0;
   for (int c = 0; c < o->numCons; c++) {
     if (!o->consAlive[c]) continue;
-    const int a3 = o->consA[c] * 3, b3 = o->consB[c] * 3;
-    const double sa = ((double)p[a3] - ox) * nx + ((double)p[a3 + 1] - oy) * ny + ((double)p[a3 + 2] - oz) * nz;
-    const double sb = ((double)p[b3] - ox) * nx + ((double)p[b3 + 1] - oy) * ny + ((double)p[b3 + 2] - oz) * nz;
-    if (sa * sb >= 0) continue;
-    const double t = sa / (sa - sb);
-    const double px = p[a3] + ((double)p[b3] - p[a3]) * t - ox;
-    const double py = p[a3 + 1] + ((double)p[b3 + 1] - p[a3 + 1]) * t - oy;
-    const double pz = p[a3 + 2] + ((double)p[b3 + 2] - p[a3 + 2]) * t - oz;
-    const double c1 = (d1y * pz - d1z * py) * nx + (d1z * px - d1x * pz) * ny + (d1x * py - d1y * px) * nz;
-    const double c2 = (py * d2z - pz * d2y) * nx + (pz * d2x - px * d2z) * ny + (px * d2y - py * d2x) * nz;
-    if (c1 < 0 || c2 < 0) continue;
+    if (!segment_in_wedge(p, o->consA[c], o->consB[c], ox, oy, oz, d1x, d1y, d1z, d2x, d2y, d2z, nx, ny, nz))
+      continue;
     kill_constraint(o, c);
     cuts++;
   }
+  /* tets whose edges cross the wedge stop holding volume across the cut
+     (the GPU twin's cutTets, wgsl.js:492-519, with this predicate) */
+  if (o->P.tet_volume)
+    for (int t = 0; t < o->numTets; t++) {
+      if (!o->tetAlive[t]) continue;
+      const int32_t* v = o->tetV + 4 * t;
+      int hit = 0;
+      for (int q = 0; q < 3 && !hit; q++)
+        for (int r = q + 1; r < 4 && !hit; r++)
+          hit = segment_in_wedge(p, v[q], v[r], ox, oy, oz, d1x, d1y, d1z, d2x, d2y, d2z, nx, ny, nz);
+      if (hit) o->tetAlive[t] = 0;
+    }
   /* the reference recomputes groups at the end of the next step() */
   return cuts;
 }

@@ -56,6 +56,13 @@ def _lib(storage):
         "or_substep_until_contacts": (None, [P, D]),
         "or_solve_contacts": (None, [P]),
         "or_solve_constraints": (None, [P, D, I]),
+        "or_collect_contacts": (None, [P]),
+        "or_solid_extra_step": (None, [P]),
+        "or_tet_update": (None, [P]),
+        "or_num_tets": (I, [P]),
+        "or_read_tets": (None, [P, IP, DP, U8]),
+        "or_write_tet_alive": (None, [P, U8]),
+        "or_read_cache": (None, [P, IP, IP, IP]),
         "or_step": (None, [P, D, I]),
         "or_recompute_groups": (None, [P]),
         "or_cut": (I, [P, D, D, D, D, D, D, D, D, D]),
@@ -166,6 +173,36 @@ class OracleSolver:
         """`iters` constraint sweeps (gs) or iterations (jacobi) alone, with
         the compliance of substep h."""
         self.lib.or_solve_constraints(self.h, h, int(iters))
+
+    # ---- the GPU twin's optional solid modes (tets, contact cache) ------
+    def collect_contacts(self):
+        self.lib.or_collect_contacts(self.h)
+
+    def solid_extra_step(self):
+        self.lib.or_solid_extra_step(self.h)
+
+    def tet_update(self):
+        self.lib.or_tet_update(self.h)
+
+    def read_tets(self):
+        nt = self.lib.or_num_tets(self.h)
+        v = np.zeros(4 * nt, np.int32)
+        v0 = np.zeros(nt, np.float64)
+        al = np.zeros(nt, np.uint8)
+        self.lib.or_read_tets(self.h, _ptr(v, ctypes.c_int32), _ptr(v0, ctypes.c_double), _ptr(al, ctypes.c_uint8))
+        return dict(v=v.reshape(-1, 4), v0=v0, alive=al)
+
+    def write_tet_alive(self, alive):
+        a = np.ascontiguousarray(alive, dtype=np.uint8)
+        self.lib.or_write_tet_alive(self.h, _ptr(a, ctypes.c_uint8))
+
+    def read_cache(self):
+        ns = self.info()["solid_count"]
+        ids = np.zeros(ns * 16, np.int32)
+        n = np.zeros(ns, np.int32)
+        ov = np.zeros(1, np.int32)
+        self.lib.or_read_cache(self.h, _ptr(ids, ctypes.c_int32), _ptr(n, ctypes.c_int32), _ptr(ov, ctypes.c_int32))
+        return dict(ids=ids.reshape(ns, 16), n=n, overflow=int(ov[0]))
 
     def step(self, dt=1.0 / 60.0, substeps=5):
         self.lib.or_step(self.h, dt, int(substeps))

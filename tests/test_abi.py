@@ -58,8 +58,8 @@ def test_params_default_and_error_paths_without_device():
 
 
 def test_params_struct_layout_matches_header():
-    # 24 doubles, 4 int32, 4 doubles
-    assert ctypes.sizeof(_abi.FpParams) == 24 * 8 + 4 * 4 + 4 * 8
+    # 24 doubles, 4 int32, 4 doubles, 2 int32, 1 double
+    assert ctypes.sizeof(_abi.FpParams) == 24 * 8 + 4 * 4 + 4 * 8 + 2 * 4 + 8
 
 
 def _c_layout(struct, fields, tmp_path):

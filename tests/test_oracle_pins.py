@@ -238,3 +238,209 @@ def test_tilted_gravity_free_fall_closed_form(phi):
     assert np.allclose(x, want, rtol=0, atol=1e-13)
     v = o.read_state64()["vel"].reshape(-1, 3)[ns]
     assert np.allclose(v, g * H_SUB * damp * (1.0 - damp ** K) / (1.0 - damp), rtol=1e-10, atol=1e-14)
+
+
+# ---- the GPU twin's optional solid modes: volume tets, solid contact cache ----
+
+def _tet_cube(mode_kw, n=2, seed=3, jitter=0.02, masses=True):
+    p = dict(gravity=0.0, constraint_iters=1, jacobi_omega=1.0, table_size=1 << 16)
+    p.update(mode_kw)
+    o = OracleSolver(p, 0, "f64", "jacobi")
+    o.add_solid_cube(0.0, 2.0, 0.0, n, 0.18, 0.08)
+    rng = np.random.default_rng(seed)
+    st = o.read_state()
+    ns = n ** 3
+    pos = st["pos"].reshape(-1, 3).copy()
+    pos[:ns] += rng.uniform(-jitter, jitter, (ns, 3))
+    im = st["inv_mass"].copy()
+    if masses:
+        im[:ns] = rng.uniform(5.0, 20.0, ns).astype(np.float32)
+    o.write_state(pos, pos, np.zeros_like(pos), im, st["active"])
+    return o
+
+
+def _signed_volume(q):
+    return np.dot(np.cross(q[1] - q[0], q[2] - q[0]), q[3] - q[0]) / 6.0
+
+
+def test_lattice_tets_tile_the_cube_with_positive_rest_volumes():
+    """Five tets per cell (the twin's split, mirrored on parity): every rest
+    volume positive and equal to the signed volume of its vertices, and each
+    cell's five add up to the cell's volume."""
+    o = _tet_cube(dict(tet_volume=1), n=4, jitter=0.0, masses=False)
+    t = o.read_tets()
+    pos = o.read_state64()["pos"].reshape(-1, 3)
+    assert len(t["v0"]) == 5 * 27 and (t["v0"] > 0).all() and t["alive"].all()
+    vols = np.array([_signed_volume(pos[v]) for v in t["v"]])
+    # (the lattice entered as float32 positions after V0 was taken in double)
+    assert np.allclose(vols, t["v0"], rtol=1e-5, atol=0)
+    assert np.allclose(t["v0"].reshape(27, 5).sum(axis=1), 0.18 ** 3, rtol=1e-6)
+
+
+def test_tet_step_is_the_volume_gradient_projection():
+    """One tet alive in a jittered 2^3 cell: each vertex moves by
+    -C s w_i dV/dp_i / sum_q w_q |dV/dp_q|^2 with the gradient taken by
+    central differences of the signed volume (independent of the cross
+    products), C = V - V0, s = 1 / max(incident tets) = 1/4; momentum
+    conserved; the volume error shrinks to (1 - s) C up to second order."""
+    o = _tet_cube(dict(tet_volume=1))
+    t = o.read_tets()
+    alive = np.zeros_like(t["alive"])
+    alive[0] = 1
+    o.write_tet_alive(alive)
+    st = o.read_state()
+    w = st["inv_mass"].astype(np.float64)
+    x = o.read_state64()["pos"].reshape(-1, 3).copy()
+    v = t["v"][0]
+    q = x[v]
+    C = _signed_volume(q) - t["v0"][0]
+    g = np.zeros((4, 3))
+    eps = 1e-6
+    for m in range(4):
+        for c in range(3):
+            qp, qm = q.copy(), q.copy()
+            qp[m, c] += eps
+            qm[m, c] -= eps
+            g[m, c] = (_signed_volume(qp) - _signed_volume(qm)) / (2 * eps)
+    denom = sum(w[v[m]] * g[m] @ g[m] for m in range(4))
+    s = 0.25
+    o.solid_extra_step()
+    y = o.read_state64()["pos"].reshape(-1, 3)
+    for m in range(4):
+        want = -C * s * w[v[m]] * g[m] / denom
+        assert np.allclose(y[v[m]] - x[v[m]], want, rtol=1e-6, atol=1e-15), m
+    others = np.setdiff1d(np.arange(8), v)
+    assert np.array_equal(y[others], x[others])
+    dp = y[v] - x[v]
+    assert np.abs(((1.0 / w[v])[:, None] * dp).sum(axis=0)).max() < 1e-15
+    c1 = _signed_volume(y[v]) - t["v0"][0]
+    assert abs(c1 - (1 - s) * C) < 50 * C * C / t["v0"][0] + 1e-18
+    assert abs(C) > 1e-6  # the jitter made a real volume error
+
+
+def test_tet_step_conserves_momentum_with_every_tet():
+    o = _tet_cube(dict(tet_volume=1), n=4, seed=5)
+    st = o.read_state()
+    m = 1.0 / st["inv_mass"].astype(np.float64)
+    x = o.read_state64()["pos"].reshape(-1, 3).copy()
+    o.solid_extra_step()
+    dp = o.read_state64()["pos"].reshape(-1, 3) - x
+    mdp = m[:, None] * dp
+    assert np.abs(mdp).sum() > 1e-6
+    assert np.abs(mdp.sum(axis=0)).max() <= 1e-12 * np.abs(mdp).sum()
+
+
+def _two_solids(d, adjacent, rs=0.09, wi=12.5, wj=20.0):
+    """Solids 0 and (1 if adjacent else 2) of a 3^3 lattice (spacing 0.18)
+    placed d apart along x, every other solid far away, every bond dead."""
+    p = dict(gravity=0.0, constraint_iters=1, jacobi_omega=1.0, table_size=1 << 16, solid_radius=rs,
+             solid_contact_cache=1)
+    o = OracleSolver(p, 0, "f64", "jacobi")
+    o.add_solid_cube(0.0, 2.0, 0.0, 3, 0.18, 0.08)
+    o.write_constraints(None, np.zeros(o.info()["num_constraints"], np.uint8))
+    j = 9 if adjacent else 18  # (1,0,0) or (2,0,0)
+    st = o.read_state()
+    pos = st["pos"].reshape(-1, 3).copy()
+    for k in range(27):
+        pos[k] = (3.0 + 0.5 * (k % 5), 2.0 + 0.5 * (k // 5), -3.0)
+    pos[0] = (0.0, 2.0, 0.0)
+    pos[j] = (np.float32(d), 2.0, 0.0)
+    im = st["inv_mass"].copy()
+    im[0], im[j] = wi, wj
+    o.write_state(pos, pos, np.zeros_like(pos), im, st["active"])
+    return o, j
+
+
+@pytest.mark.parametrize("adjacent", [False, True])
+def test_cached_solid_contact_halves_the_overlap(adjacent):
+    """Two unbonded solids inside their contact distance: the cache lists
+    each for the other; one extra step closes half the overlap (the 0.5 of
+    wgsl.js:594) with equal and opposite momenta.  Lattice neighbours (a cut
+    face) use min(2 r, 0.97 spacing) as their contact distance."""
+    d0 = float(np.float32(0.15))
+    o, j = _two_solids(d0, adjacent)
+    o.collect_contacts()
+    cache = o.read_cache()
+    assert list(cache["ids"][0][: cache["n"][0]]) == [j] and list(cache["ids"][j][: cache["n"][j]]) == [0]
+    rsum = min(0.18, 0.97 * 0.18) if adjacent else 0.18
+    x = o.read_state64()["pos"].reshape(-1, 3).copy()
+    o.solid_extra_step()
+    y = o.read_state64()["pos"].reshape(-1, 3)
+    d1 = np.linalg.norm(y[j] - y[0])
+    assert d1 == pytest.approx(d0 + 0.5 * (rsum - d0), rel=1e-12)
+    assert np.allclose((y[0] - x[0]) / 12.5, -(y[j] - x[j]) / 20.0, rtol=1e-12, atol=1e-18)
+
+
+def test_cached_contact_correction_is_capped():
+    """A deep overlap moves each solid by at most half a solid radius per
+    iteration (applySolidContacts, wgsl.js:598-606)."""
+    o, j = _two_solids(float(np.float32(0.01)), False, wi=20.0, wj=12.5)
+    o.collect_contacts()
+    x = o.read_state64()["pos"].reshape(-1, 3).copy()
+    o.solid_extra_step()
+    y = o.read_state64()["pos"].reshape(-1, 3)
+    for k in (0, j):
+        assert np.linalg.norm(y[k] - x[k]) <= 0.5 * 0.09 * (1 + 1e-12)
+    assert np.linalg.norm(y[0] - x[0]) == pytest.approx(0.045, rel=1e-12)
+
+
+def test_cache_keeps_the_smallest_partners_and_counts_the_rest():
+    """More unbonded partners in range than the cache holds (16): the 16
+    smallest indices are kept, in order, and the rest are counted."""
+    p = dict(gravity=0.0, constraint_iters=1, table_size=1 << 16, solid_contact_cache=1, solid_radius=0.09,
+             cell_size=0.4)
+    o = OracleSolver(p, 0, "f64", "jacobi")
+    o.add_solid_cube(0.0, 2.0, 0.0, 3, 0.18, 0.08)
+    o.write_constraints(None, np.zeros(o.info()["num_constraints"], np.uint8))
+    st = o.read_state()
+    rng = np.random.default_rng(8)
+    pos = 0.0 + rng.uniform(-0.04, 0.04, (27, 3)) + np.array([0.0, 2.0, 0.0])
+    o.write_state(pos, pos, None, None, None)
+    o.collect_contacts()
+    c = o.read_cache()
+    assert c["n"][0] == 16 and list(c["ids"][0]) == list(range(1, 17))
+    assert c["overflow"] >= 27 * 10
+
+
+def test_tet_tears_at_its_threshold():
+    """A tet edge longer than its lattice rest + tear_gap * 2 r kills the tet;
+    exactly at it the tet lives (wgsl.js:478-486)."""
+    for extra, alive in ((0.0, True), (1e-6, False)):
+        o = _tet_cube(dict(tet_volume=1, tear_gap=1.0), jitter=0.0, masses=False)
+        t = o.read_tets()
+        v = t["v"][0]
+        pos = o.read_state64()["pos"].reshape(-1, 3).copy()
+        a, b = v[0], v[1]
+        # move b along a-b until |a - b| = rest + gap (+ extra)
+        rest = np.linalg.norm(pos[b] - pos[a])
+        target = rest + 2 * 0.09 + extra
+        u = (pos[b] - pos[a]) / rest
+        pos[b] = pos[a] + u * target
+        o.write_state(pos.astype(np.float32), None, None, None, None)
+        q = o.read_state64()["pos"].reshape(-1, 3)
+        dist = np.linalg.norm(q[b] - q[a])
+        o.tet_update()
+        got = bool(o.read_tets()["alive"][0])
+        assert got == (dist <= rest + 2 * 0.09), (extra, dist, rest + 0.18)
+    assert not got
+
+
+def test_cut_kills_the_tets_whose_edges_it_cuts():
+    """After a cut, a tet is dead exactly when one of its six edges -- every
+    one a lattice constraint -- was cut."""
+    o = _tet_cube(dict(tet_volume=1), n=4, jitter=0.005, masses=False)
+    cons = o.read_constraints()
+    edge = {(int(a), int(b)): k for k, (a, b) in enumerate(zip(cons["a"], cons["b"]))}
+    # the plane x = 0.01, between two lattice layers, across the whole cube
+    cam = np.array([0.01, 3.5, 0.0])
+    d1 = np.array([0.0, -1.0, -0.9])
+    d2 = np.array([0.0, -1.0, 0.9])
+    d1 /= np.linalg.norm(d1)
+    d2 /= np.linalg.norm(d2)
+    assert o.cut(cam, d1, d2) > 0
+    alive_c = o.read_constraints()["alive"]
+    t = o.read_tets()
+    for v, al in zip(t["v"], t["alive"]):
+        cut = any(not alive_c[edge[(min(p, q), max(p, q))]] for i, p in enumerate(v) for q in v[i + 1:])
+        assert bool(al) == (not cut)
+    assert 0 < int(t["alive"].sum()) < len(t["alive"])
