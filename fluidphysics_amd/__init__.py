@@ -405,6 +405,16 @@ class Solver:
         self._check(self._lib.fp_read_grid(self._h, _ip(cs), _ip(ce)))
         return cs, ce
 
+    def read_tets(self):
+        """The optional tet mode's tets: (verts [NT, 4] int32, alive [NT] uint8)."""
+        nt = ctypes.c_int32(0)
+        self._check(self._lib.fp_read_tets(self._h, None, None, ctypes.byref(nt)))
+        v = np.zeros((nt.value, 4), np.int32)
+        a = np.zeros(nt.value, np.uint8)
+        if nt.value:
+            self._check(self._lib.fp_read_tets(self._h, _ip(v), _u8p(a), ctypes.byref(nt)))
+        return v, a
+
     def read_density(self):
         n = self.capacity
         rho = np.zeros(n, np.float32)

@@ -20,7 +20,7 @@ thread_local std::string g_last_error = "";
 const char* kPassNames[] = {"integrate",   "grab",        "hash_count",      "scan_reduce", "scan_apply",
                             "scatter",     "rank",        "contacts",        "resort",      "density",
                             "displace",    "constraints", "bounds_velocity", "xsph",        "cut",
-                            "pick",        "count_active", "emit"};
+                            "pick",        "count_active", "emit",           "solid_extra"};
 static_assert(sizeof(kPassNames) / sizeof(kPassNames[0]) == P_NUM, "one name per PassId");
 
 // Synchronous host <-> device copies ordered on the solver's (non-blocking)
@@ -150,6 +150,12 @@ void scene_buffers(fp_solver* s, F& take) {
   take(s->dgroup, std::max<size_t>(ns, 1));
   take(s->adj_off, ns + 1), take(s->adj, 2 * C), take(s->spos_c[0], std::max<size_t>(ns, 1));
   take(s->spos_c[1], std::max<size_t>(ns, 1)), take(s->dflags, (size_t)DF_WORDS), take(s->dpick_t, 1);
+  // the optional solid modes (empty when off)
+  const size_t NT = s->tets_on ? (size_t)s->NT : 0, nsx = (s->tets_on || s->cache_on) ? ns : 0;
+  take(s->tet_v, NT), take(s->tet_v0, NT), take(s->tet_alive, NT), take(s->tet_scale, NT);
+  take(s->tet_adj_off, s->tets_on ? ns + 1 : 0), take(s->tet_adj, 4 * NT);
+  take(s->cache_ids, s->cache_on ? ns * FP_CACHE_K : 0), take(s->cache_n, s->cache_on ? ns : 0);
+  take(s->spos_x, nsx);
   take(s->dpick_i, 1);
 }
 
@@ -233,6 +239,10 @@ bool cube_fits(int64_t n, int64_t max_fluid) {
 
 // Particle counts and the reference table of a cube of side n.
 void scene_dims(fp_solver* s, int n) {
+  s->cube_n = n;
+  s->tets_on = s->P.tet_volume != 0;
+  s->cache_on = s->P.solid_contact_cache != 0;
+  s->NT = n > 1 ? 5 * (n - 1) * (n - 1) * (n - 1) : 0;
   s->ns = n * n * n;
   s->n = s->ns + s->max_fluid;
   s->nf = s->max_fluid;
@@ -646,6 +656,63 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
     hab[c] = make_int2(a, b);
   }
   s->h_group.assign(ns, 0);
+  s->spacing = spacing;
+
+  // volume tets (the GPU twin's split, sim.js:151-166): five per lattice
+  // cell, mirrored on checkerboard parity; the last two vertices swapped
+  // where that makes the rest volume (double, from the float32 lattice)
+  // positive -- the twin keeps its order and so treats half its tets as dead
+  std::vector<int4> htv;
+  std::vector<float> htv0, hts;
+  std::vector<int> htoff, htadj;
+  if (s->tets_on) {
+    htv.reserve(s->NT);
+    auto vol = [&](int a, int b, int c, int d) {
+      const double abx = (double)hpos[b].x - hpos[a].x, aby = (double)hpos[b].y - hpos[a].y,
+                   abz = (double)hpos[b].z - hpos[a].z;
+      const double acx = (double)hpos[c].x - hpos[a].x, acy = (double)hpos[c].y - hpos[a].y,
+                   acz = (double)hpos[c].z - hpos[a].z;
+      const double adx = (double)hpos[d].x - hpos[a].x, ady = (double)hpos[d].y - hpos[a].y,
+                   adz = (double)hpos[d].z - hpos[a].z;
+      return ((aby * acz - abz * acy) * adx + (abz * acx - abx * acz) * ady + (abx * acy - aby * acx) * adz) / 6.0;
+    };
+    for (int ix = 0; ix + 1 < n; ix++)
+      for (int iy = 0; iy + 1 < n; iy++)
+        for (int iz = 0; iz + 1 < n; iz++) {
+          const int A = idx(ix, iy, iz), B = idx(ix + 1, iy, iz), C = idx(ix + 1, iy + 1, iz), D = idx(ix, iy + 1, iz);
+          const int E = idx(ix, iy, iz + 1), F = idx(ix + 1, iy, iz + 1), G = idx(ix + 1, iy + 1, iz + 1),
+                    H = idx(ix, iy + 1, iz + 1);
+          const int even[5][4] = {{A, C, F, H}, {A, B, C, F}, {A, C, D, H}, {A, E, F, H}, {C, F, G, H}};
+          const int odd[5][4] = {{B, D, E, G}, {A, B, D, E}, {B, C, D, G}, {D, E, H, G}, {B, E, F, G}};
+          const int(*five)[4] = (ix + iy + iz) % 2 == 0 ? even : odd;
+          for (int k = 0; k < 5; k++) {
+            int4 v = make_int4(five[k][0], five[k][1], five[k][2], five[k][3]);
+            double V = vol(v.x, v.y, v.z, v.w);
+            if (V < 0) {
+              std::swap(v.z, v.w);
+              V = -V;
+            }
+            htv.push_back(v);
+            htv0.push_back((float)V);
+          }
+        }
+    htoff.assign(ns + 1, 0);
+    for (const int4& v : htv) {
+      htoff[v.x + 1]++, htoff[v.y + 1]++, htoff[v.z + 1]++, htoff[v.w + 1]++;
+    }
+    for (int i = 0; i < ns; i++) htoff[i + 1] += htoff[i];
+    htadj.assign(4 * (size_t)s->NT, 0);
+    std::vector<int> tfill(ns, 0);
+    for (int t = 0; t < (int)htv.size(); t++) {
+      const int q[4] = {htv[t].x, htv[t].y, htv[t].z, htv[t].w};
+      int m = 1;
+      for (int u = 0; u < 4; u++) {
+        htadj[htoff[q[u]] + tfill[q[u]]++] = t;
+        m = std::max(m, htoff[q[u] + 1] - htoff[q[u]]);
+      }
+      hts.push_back(1.f / (float)m);
+    }
+  }
 
   scene_tables(s);
   const int N = s->n;
@@ -679,6 +746,15 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
     FP_TRY(s, h2d(s, s->adj, hadj.data(), 2 * C * sizeof(int2)));
   }
   FP_TRY(s, h2d(s, s->adj_off, hoff.data(), (ns + 1) * sizeof(int)));
+  if (s->tets_on && s->NT) {
+    std::vector<uint8_t> tones(s->NT, 1);
+    FP_TRY(s, h2d(s, s->tet_v, htv.data(), htv.size() * sizeof(int4)));
+    FP_TRY(s, h2d(s, s->tet_v0, htv0.data(), htv0.size() * sizeof(float)));
+    FP_TRY(s, h2d(s, s->tet_scale, hts.data(), hts.size() * sizeof(float)));
+    FP_TRY(s, h2d(s, s->tet_alive, tones.data(), tones.size()));
+    FP_TRY(s, h2d(s, s->tet_adj_off, htoff.data(), htoff.size() * sizeof(int)));
+    FP_TRY(s, h2d(s, s->tet_adj, htadj.data(), htadj.size() * sizeof(int)));
+  }
   // prev stays zero until the first integrate (solver.js:43)
   s->has_cube = true;
   return FP_OK;
@@ -793,6 +869,10 @@ int fp_reset_solid(fp_solver* s) {
   if (s->C) {
     std::vector<uint8_t> ones(s->C, 1);
     if ((rc = constraints_to_device(s, s->h_snap_rest.data(), ones.data()))) return rc;
+  }
+  if (s->tets_on && s->NT) {
+    std::vector<uint8_t> tones(s->NT, 1);
+    FP_TRY(s, h2d(s, s->tet_alive, tones.data(), tones.size()));
   }
   s->h_group.assign(ns, 0);
   s->groups_dirty = false;
@@ -1086,6 +1166,20 @@ int fp_write_constraints(fp_solver* s, const float* rest, const uint8_t* alive) 
   if ((rc = materialize_groups(s))) return rc;
   if (alive) s->alive_changed = true;
   return constraints_to_device(s, rest, alive);
+}
+
+int fp_read_tets(fp_solver* s, int32_t* verts, uint8_t* alive, int32_t* num_tets) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_read_tets: NULL solver");
+  if (!num_tets) return fp_fail(s, FP_EINVAL, "fp_read_tets: NULL num_tets");
+  const int NT = s->tets_on ? s->NT : 0;
+  *num_tets = NT;
+  if (NT == 0 || (!verts && !alive)) return FP_OK;
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  if (verts) FP_TRY(s, d2h(s, verts, s->tet_v, (size_t)NT * sizeof(int4)));
+  if (alive) FP_TRY(s, d2h(s, alive, s->tet_alive, (size_t)NT));
+  return FP_OK;
 }
 
 int fp_build_grid(fp_solver* s) {
@@ -1432,6 +1526,7 @@ void state_arrays(fp_solver* s, F&& f) {
   f((void*)s->active, N);
   f((void*)s->erest, E * sizeof(float));
   f((void*)s->ealive, E);
+  f((void*)s->tet_alive, s->tets_on ? (size_t)s->NT : 0);
 }
 
 size_t state_array_bytes(fp_solver* s) {
@@ -1578,7 +1673,7 @@ int fp_state_import(fp_solver* s, const void* buf, int64_t bytes) {
   std::vector<uint8_t> flags;
   state_arrays(s, [&](void* p, size_t b) {
     if (e == hipSuccess && b) {
-      if (p == (void*)s->active || p == (void*)s->ealive) {
+      if (p == (void*)s->active || p == (void*)s->ealive || p == (void*)s->tet_alive) {
         // flags enter the device as 0 / 1 only
         flags.assign(in + off, in + off + b);
         for (uint8_t& f : flags) f = f ? 1 : 0;

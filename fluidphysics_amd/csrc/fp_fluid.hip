@@ -270,6 +270,9 @@ __device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb, 
             const float4& py = i ? p1 : p0;
             if ((i && q + 1 >= e1) || y == x) continue;
             if (y < fp.ns) {
+              // with the solid contact cache, solid pairs are resolved in the
+              // constraint iterations
+              if (A.sm.cache) continue;
               // a bonded pair does not collide; test the bond only when close
               const float dx = px.x - py.x, dy = px.y - py.y, dz = px.z - py.z;
               const float d2 = dx * dx + dy * dy + dz * dz;
@@ -994,6 +997,8 @@ int fp_scene_blocks(const fp_solver* s, int kind, int contact_lanes) {
       return (int)(((int64_t)s->n * s->fluid_lanes + FLUID_BLOCK - 1) / FLUID_BLOCK);
     case W_CONS:
       return (int)(((int64_t)s->ns * 32 + BLOCK - 1) / BLOCK);
+    case W_TET:
+      return s->tets_on ? fp_blocks(s->NT) : 0;
   }
   return 0;
 }

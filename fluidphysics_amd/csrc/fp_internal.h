@@ -72,6 +72,7 @@
 #define DF_SCRATCH 2     // fluid count
 #define DF_PAIR_OVF 3    // particle-substeps whose pair list overflowed
 #define DF_NCOMP 4       // components counted by fp_launch_components
+#define DF_CACHE_OVF 5   // solid contact partners beyond FP_CACHE_K (dropped, cumulative)
 #define DF_WORDS 8
 
 enum PassId {
@@ -93,6 +94,7 @@ enum PassId {
   P_PICK,
   P_COUNT,
   P_EMIT,
+  P_SOLID_EXTRA,  // the optional solid modes: cache collection, tet / cached-contact steps, tet tearing
   P_NUM
 };
 
@@ -173,6 +175,17 @@ struct fp_solver {
   int* adj_off = nullptr;
   int2* adj = nullptr;
   float4* spos_c[2] = {nullptr, nullptr};
+  // the GPU twin's optional solid modes (fp_params.tet_volume / solid_contact_cache)
+  bool tets_on = false, cache_on = false;
+  int cube_n = 0, NT = 0;
+  double spacing = 0.0;
+  int4* tet_v = nullptr;        // vertices, ordered so the rest volume is positive
+  float* tet_v0 = nullptr;      // rest volume
+  uint8_t* tet_alive = nullptr;
+  float* tet_scale = nullptr;   // 1 / max(incident tets of its vertices)
+  int *tet_adj_off = nullptr, *tet_adj = nullptr;  // tets incident to each solid (CSR)
+  int *cache_ids = nullptr, *cache_n = nullptr;    // solid contact partners (FP_CACHE_K per solid)
+  float4* spos_x = nullptr;     // solids after an iteration's extra step
   int* dflags = nullptr;
   double* dpick_t = nullptr;
   int* dpick_i = nullptr;
@@ -279,6 +292,27 @@ struct ContactCtx {
   const int* __restrict__ sbox;  // solids' bounding box at grid time (ordered-int bits)
 };
 
+// The GPU twin's optional solid modes of one scene (none when both flags are 0).
+#define FP_CACHE_K 16  // solid contact partners kept per solid
+struct SolidModes {
+  int tets, cache;      // the modes in use (only with constraint iterations)
+  int n, NT;            // lattice side, tets
+  double spacing;       // lattice spacing
+  double gap;           // tet edge tear margin (tear_gap solid diameters)
+  float rsum, rsum_adj; // solid contact distance, and for lattice neighbours (a cut face)
+  float lim, lim_adj;   // the same + the collection margin
+  float cap;            // per-tet, per-sum correction cap (half a solid radius)
+  const int4* __restrict__ tet_v;
+  const float* __restrict__ tet_v0;
+  uint8_t* tet_alive;
+  const float* __restrict__ tet_scale;
+  const int* __restrict__ tet_adj_off;
+  const int* __restrict__ tet_adj;
+  int* cache_ids;
+  int* cache_n;
+  float4* spos_x;
+};
+
 // Everything a substep's kernels need about one scene, in device memory: the
 // kernels of a launch group read their scene's entry through a work list of
 // (scene, block of that scene) items, so one launch serves every scene.
@@ -318,6 +352,7 @@ struct SceneArgs {
   int* adj_off;
   int2* adj;
   float4* spos_c[2];
+  SolidModes sm;
 };
 
 // Work lists: the blocks one scene needs in a pass of each kind.
@@ -327,6 +362,7 @@ enum WorkKind {
   W_SOLID,     // contact lanes (4..32, chosen per group) per solid
   W_FLUID,     // fluid lanes (the scene's) per particle slot (FLUID_BLOCK per block)
   W_CONS,      // 32 threads per solid
+  W_TET,       // one thread per tet (scenes with volume tets)
   W_NUM
 };
 
@@ -353,6 +389,9 @@ struct fp_group {
   int iters = 0;
   bool any_grabs = false;
   bool any_unconstrained = false;  // a member has solids but no constraint iterations
+  bool any_extra = false;          // a member uses the volume tets or the solid contact cache
+  bool any_cache = false;          // ... the solid contact cache
+  bool any_tets = false;           // ... the volume tets
   // captured frame
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -361,6 +400,7 @@ struct fp_group {
   int graph_iters = -1;
   int graph_clanes = 0;             // contact lanes the captured frame launches
   bool graph_unconstrained = false;  // the captured frame has the bounds pass of unconstrained scenes
+  int graph_extra = -1;              // the optional-mode passes the captured frame has (bit 0 cache, 1 tets)
   bool use_graph = true;
   int64_t graph_launches = 0;
   int64_t graph_captures = 0;
@@ -546,6 +586,9 @@ void fp_group_free(fp_group* g);
 int fp_launch_ref_grid(fp_solver* s, bool rehash);
 int fp_launch_cut(fp_solver* s, const double o[3], const double d1[3], const double d2[3], double nx, double ny,
                   double nz);
+int fp_group_solid_collect(fp_group* g, hipStream_t st);
+int fp_group_solid_extra(fp_group* g, int it, hipStream_t st);
+int fp_group_tet_update(fp_group* g, hipStream_t st);
 int fp_launch_pick(fp_solver* s, const double o[3], const double d[3], double max_perp2);
 int fp_launch_select(fp_solver* s, const double c[3], double r2, int same, int* ids, int max_ids, int* n);
 int fp_launch_components(fp_solver* s);

@@ -214,6 +214,30 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
   a.adj = s->adj;
   a.spos_c[0] = s->spos_c[0];
   a.spos_c[1] = s->spos_c[1];
+  // the optional solid modes run in the constraint iterations
+  SolidModes& m = a.sm;
+  m.tets = s->tets_on && a.cp.iters > 0 && s->NT > 0;
+  m.cache = s->cache_on && a.cp.iters > 0;
+  m.n = s->cube_n;
+  m.NT = s->NT;
+  m.spacing = s->spacing;
+  const double rsum = 2.0 * P.solid_radius, rsum_adj = std::min(rsum, 0.97 * s->spacing);
+  const double margin = std::max(0.0, std::min(rsum, P.cell_size - rsum));
+  m.rsum = (float)rsum;
+  m.rsum_adj = (float)rsum_adj;
+  m.lim = (float)(rsum + margin);
+  m.lim_adj = (float)(rsum_adj + margin);
+  m.cap = (float)(0.5 * P.solid_radius);
+  m.gap = P.tear_gap * rsum;
+  m.tet_v = s->tet_v;
+  m.tet_v0 = s->tet_v0;
+  m.tet_alive = s->tet_alive;
+  m.tet_scale = s->tet_scale;
+  m.tet_adj_off = s->tet_adj_off;
+  m.tet_adj = s->tet_adj;
+  m.cache_ids = s->cache_ids;
+  m.cache_n = s->cache_n;
+  m.spos_x = s->spos_x;
   return a;
 }
 
@@ -303,7 +327,13 @@ int fp_group_prepare(fp_group* g, double h) {
   }
   if (g->iters < 0) g->iters = 0;
   g->any_unconstrained = false;
-  for (int k = 0; k < ns; k++) g->any_unconstrained |= args[k].cp.ns > 0 && args[k].cp.iters == 0;
+  g->any_cache = g->any_tets = false;
+  for (int k = 0; k < ns; k++) {
+    g->any_unconstrained |= args[k].cp.ns > 0 && args[k].cp.iters == 0;
+    g->any_cache |= args[k].sm.cache != 0;
+    g->any_tets |= args[k].sm.tets != 0;
+  }
+  g->any_extra = g->any_cache || g->any_tets;
   int64_t snap = 0;
   for (int k = 0; k < ns; k++) {
     args[k].snap_off = snap;
@@ -410,6 +440,7 @@ static int group_substep(fp_group* g) {
     // constraint iteration its solids', the displacement its fluid's)
     FP_GTRY(g, hipEventRecord(g->ev_fork, g->stream));
     FP_GTRY(g, hipStreamWaitEvent(g->side, g->ev_fork, 0));
+    if ((rc = fp_group_solid_collect(g, g->side))) return rc;
     if ((rc = fp_group_contacts_solid(g, g->side))) return rc;
     if ((rc = fp_group_constraints(g, g->iters, g->side))) return rc;
     FP_GTRY(g, hipEventRecord(g->ev_join, g->side));
@@ -419,6 +450,7 @@ static int group_substep(fp_group* g) {
     if ((rc = fp_group_displace(g))) return rc;
     FP_GTRY(g, hipStreamWaitEvent(g->stream, g->ev_join, 0));
   } else {
+    if ((rc = fp_group_solid_collect(g, g->stream))) return rc;
     if ((rc = fp_group_contacts(g))) return rc;
     if ((rc = fp_group_resort(g))) return rc;
     if ((rc = fp_group_density(g))) return rc;
@@ -529,10 +561,13 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
     // the captured frame replays whatever the scene table and the work lists
     // hold; it is captured again when its structure no longer fits: the
     // substep or iteration count, the contact pass's lane count (a kernel
-    // template), the bounds pass of unconstrained scenes, or work lists that
-    // outgrew (or shrank far below) the grids captured for them
+    // template), the bounds pass of unconstrained scenes, the optional solid
+    // modes' passes, or work lists that outgrew (or shrank far below) the
+    // grids captured for them
+    const int extra = (g->any_cache ? 1 : 0) | (g->any_tets ? 2 : 0);
     bool fits = g->gexec && g->graph_sub == substeps && g->graph_iters == g->iters &&
-                g->graph_clanes == g->clanes && g->graph_unconstrained == g->any_unconstrained;
+                g->graph_clanes == g->clanes && g->graph_unconstrained == g->any_unconstrained &&
+                g->graph_extra == extra;
     for (int k = 0; k < W_NUM && fits; k++)
       fits = 2 * g->n_items[k] <= 3 * g->grid_cap[k] && 4 * fp_grid8(g->n_items[k]) >= g->grid_cap[k];
     if (!fits) {
@@ -565,6 +600,7 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
       g->graph_iters = g->iters;
       g->graph_clanes = g->clanes;
       g->graph_unconstrained = g->any_unconstrained;
+      g->graph_extra = extra;
     }
     FP_GTRY(g, hipGraphLaunch(g->gexec, g->stream));
     g->graph_launches++;

@@ -234,6 +234,12 @@ int fp_write_state(fp_solver* s, const float* pos, const float* prev, const floa
 int fp_read_constraints(fp_solver* s, int32_t* a, int32_t* b, float* rest, uint8_t* alive);
 int fp_write_constraints(fp_solver* s, const float* rest, const uint8_t* alive);
 
+/* The volume tets of the GPU twin's optional tet mode (the tets of
+ * src/gpu/sim.js, five per lattice cell; none without fp_params.tet_volume):
+ * *num_tets is set to their count; verts (4 per tet, ordered so the rest
+ * volume is positive) and alive (1 per tet) may be NULL. */
+int fp_read_tets(fp_solver* s, int32_t* verts, uint8_t* alive, int32_t* num_tets);
+
 /* _buildGrid() (solver.js:174-184) on the current positions, on the device. */
 int fp_build_grid(fp_solver* s);
 

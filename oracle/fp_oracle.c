@@ -518,6 +518,13 @@ static void solve_contacts_gs(Oracle* o) {
  * from all non-bonded solids and all fluids, fluids from all solids.  Each
  * distinct bucket among the 27 probes is visited once (a second visit of the
  * same bucket finds the pair already resolved in the Gauss-Seidel order). */
+/* The optional solid modes run in Jacobi mode, in the constraint iterations
+ * of a scene that has distance constraints. */
+static int extra_modes(const Oracle* o) {
+  return o->mode == OR_MODE_JACOBI && (o->P.tet_volume || o->P.solid_contact_cache) && o->solidCount > 0 &&
+         o->numCons > 0;
+}
+
 static void solve_contacts_jacobi(Oracle* o) {
   ST* p = o->pos;
   ST* s = o->jpos;
@@ -549,7 +556,7 @@ static void solve_contacts_jacobi(Oracle* o) {
             const int solidY = o->type[y] == 1;
             if (!solidX && !solidY) continue;
             /* with the contact cache, solid pairs are resolved in the constraint iterations */
-            if (solidX && solidY && o->P.solid_contact_cache) continue;
+            if (solidX && solidY && o->P.solid_contact_cache && extra_modes(o)) continue;
             if (solidX && solidY && bonded(o, x < y ? x : y, x < y ? y : x)) continue;
             const int y3 = 3 * y;
             const double ddx = (double)s[x3] - s[y3], ddy = (double)s[x3 + 1] - s[y3 + 1],
@@ -959,9 +966,6 @@ static void tet_update(Oracle* o) {
   }
 }
 
-static int extra_modes(const Oracle* o) {
-  return o->mode == OR_MODE_JACOBI && (o->P.tet_volume || o->P.solid_contact_cache) && o->solidCount > 0;
-}
 
 /* ---- bounds (solver.js:389-427) -------------------------------------- */
 

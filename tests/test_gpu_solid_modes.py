@@ -1,0 +1,258 @@
+"""GPU parity of the GPU twin's optional solid modes (src/gpu/wgsl.js:423-606:
+volume tets, the solid contact cache) against the FP64 oracle in the Jacobi
+formulation, through the C ABI.
+
+Bars:
+  * tet topology (vertices, order) and the alive flags after cuts:
+    bit-exact (both compute the wedge test in IEEE double from the same
+    float32 positions);
+  * positions after one substep from an identical state: within 1e-4 * h
+    of the FP64 oracle (h = fluidH), velocities as v * dt_sub, as for the
+    default modes (fluid pairs at |d - H| < 2e-6 excluded from the velocity
+    bound, as there);
+  * tet alive flags after that substep: equal, except tets with an edge
+    within 1e-4 * h of its tear length (float32 positions against the FP64
+    replay decide those);
+  * a captured (graph) run and an eager run of the same frames:
+    bit-identical.
+"""
+import numpy as np
+import pytest
+
+from tests import _golden as G
+from tests._gpu import GpuAdapter, log, make_gpu, tol_report
+from tests.test_gpu_parity import SUB, _ambiguous, _state_from_dump
+
+pytestmark = pytest.mark.gpu
+
+MODES = {"tets": dict(tet_volume=1), "cache": dict(solid_contact_cache=1),
+         "both": dict(tet_volume=1, solid_contact_cache=1)}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _lib():
+    from fluidphysics_amd import _abi
+    _abi.load_library()
+
+
+def test_tet_topology_matches_oracle():
+    log("case tets topology plain_cpu")
+    d = G.scenario("plain_cpu")
+    g = make_gpu(d, **MODES["tets"])
+    o = G.make_oracle(d, storage="f64", mode="jacobi", iters=3, jacobi_omega=4.0, **MODES["tets"])
+    v, a = g.read_tets()
+    ot = o.read_tets()
+    n = d["cube"][3]
+    assert len(v) == 5 * (n - 1) ** 3
+    assert np.array_equal(v, ot["v"])
+    assert a.all() and ot["alive"].all()
+    # without the mode there are none
+    v0, a0 = make_gpu(d).read_tets()
+    assert len(v0) == 0 and len(a0) == 0
+
+
+def _near_tear(pos, verts, spacing, n, gap, band):
+    """Tets with an edge whose length is within `band` of its tear length."""
+    p = pos.reshape(-1, 3).astype(np.float64)
+    idx = np.stack([verts // (n * n), (verts // n) % n, verts % n], axis=-1)
+    out = np.zeros(len(verts), bool)
+    for q in range(3):
+        for r in range(q + 1, 4):
+            d = np.linalg.norm(p[verts[:, q]] - p[verts[:, r]], axis=1)
+            rest = spacing * np.linalg.norm((idx[:, q] - idx[:, r]).astype(np.float64), axis=1)
+            out |= np.abs(d - (rest + gap)) < band
+    return out
+
+
+@pytest.mark.parametrize("mode", sorted(MODES))
+def test_one_substep_mid_run_vs_fp64_jacobi(mode):
+    """Frame-150 state of the reference run (the cube sliced and resting in
+    the fluid): one substep with the optional mode on, GPU vs the FP64
+    oracle; the tets start alive, so the stretched ones across the slices
+    stay alive through it."""
+    log("case one-substep solid modes %s plain_cpu f150" % mode)
+    d = G.scenario("plain_cpu")
+    ref = _state_from_dump("plain_cpu", "f150")
+    g = make_gpu(d, iters=3, omega=8.0, **MODES[mode])
+    o = G.make_oracle(d, storage="f64", mode="jacobi", iters=3, jacobi_omega=8.0, **MODES[mode])
+    inv_mass = g.read_state()["inv_mass"]
+    for s in (g, o):
+        s.write_state(ref["pos"], ref["prev"], ref["vel"], inv_mass, ref["active"])
+    g.write_constraints(ref["rest"].astype(np.float32), ref["alive"])
+    o.write_constraints(ref["rest"], ref["alive"])
+    params = dict(g.params())
+    H = params["fluid_h"]
+    state = dict(state=(ref["pos"], ref["prev"], ref["vel"], inv_mass, ref["active"]),
+                 cons=(ref["rest"].astype(np.float64), ref["alive"]))
+    oparams = dict(gravity=d["gravity"], constraint_iters=3, jacobi_omega=8.0, **MODES[mode])
+    amb = _ambiguous(state, H, oparams, d["maxFluid"], d["cube"], SUB)
+    g.step(SUB, 1)
+    o.step(SUB, 1)
+    st = g.read_state()
+    ref64 = o.read_state64()
+    ost = o.read_state()
+    bound = 1e-4 * H
+    assert np.array_equal(st["active"], ost["active"])
+    act = st["active"].astype(bool)
+    a3 = np.repeat(act, 3)
+    ns = d["cube"][3] ** 3
+    s3 = np.zeros_like(a3)
+    s3[: 3 * ns] = a3[: 3 * ns]
+    ep = tol_report(mode + " pos", st["pos"][a3], ref64["pos"][a3], bound)
+    tol_report(mode + " solid pos", st["pos"][s3], ref64["pos"][s3], bound)
+    keep = np.repeat(act & ~amb, 3)
+    ev = tol_report(mode + " vel*dt_sub", st["vel"][keep] * SUB, ref64["vel"][keep] * SUB, bound)
+    assert ep <= bound, "positions exceed 1e-4*h"
+    assert ev <= bound, "velocities (v*dt_sub) exceed 1e-4*h"
+    if "tet_volume" in MODES[mode]:
+        v, ga = g.read_tets()
+        ot = o.read_tets()
+        n, sp = d["cube"][3], d["cube"][4]
+        gap = params["tear_gap"] * 2 * params["solid_radius"]
+        near = _near_tear(ref64["pos"], v, sp, n, gap, bound)
+        diff = ga != ot["alive"]
+        log("  tets alive %d (oracle %d) of %d; %d within %.1e of tearing; %d differ"
+            % (int(ga.sum()), int(ot["alive"].sum()), len(ga), int(near.sum()), bound, int(diff.sum())))
+        assert not (diff & ~near).any(), "tet alive flags differ away from the tear length"
+
+
+def test_stretched_tets_tear_like_the_oracle():
+    """Frame-150 state with the lattice's upper x half displaced (positions
+    and previous positions) by 2 tear gaps: the tets spanning the gap
+    tear in the first substep, on the GPU as in the FP64 oracle, and the
+    positions after it stay within 1e-4 * h."""
+    log("case stretched tets plain_cpu f150")
+    d = G.scenario("plain_cpu")
+    ref = _state_from_dump("plain_cpu", "f150")
+    g = make_gpu(d, iters=3, omega=8.0, **MODES["both"])
+    o = G.make_oracle(d, storage="f64", mode="jacobi", iters=3, jacobi_omega=8.0, **MODES["both"])
+    params = dict(g.params())
+    n = d["cube"][3]
+    gap = params["tear_gap"] * 2 * params["solid_radius"]
+    shift = np.zeros((len(ref["active"]), 3), np.float32)
+    shift[: n ** 3][np.arange(n ** 3) // (n * n) >= n // 2, 0] = np.float32(2.0 * gap)
+    pos = (ref["pos"].reshape(-1, 3) + shift).ravel()
+    prev = (ref["prev"].reshape(-1, 3) + shift).ravel()
+    inv_mass = g.read_state()["inv_mass"]
+    for s in (g, o):
+        s.write_state(pos, prev, ref["vel"], inv_mass, ref["active"])
+    g.write_constraints(ref["rest"].astype(np.float32), ref["alive"])
+    o.write_constraints(ref["rest"], ref["alive"])
+    g.step(SUB, 1)
+    o.step(SUB, 1)
+    st, ref64 = g.read_state(), o.read_state64()
+    bound = 1e-4 * params["fluid_h"]
+    act = np.repeat(st["active"].astype(bool), 3)
+    ep = tol_report("stretched pos", st["pos"][act], ref64["pos"][act], bound)
+    v, ga = g.read_tets()
+    oa = o.read_tets()["alive"]
+    near = _near_tear(ref64["pos"], v, d["cube"][4], n, gap, bound)
+    diff = ga != oa
+    log("  tets alive %d (oracle %d) of %d; %d within %.1e of tearing; %d differ"
+        % (int(ga.sum()), int(oa.sum()), len(ga), int(near.sum()), bound, int(diff.sum())))
+    assert ep <= bound
+    assert not (diff & ~near).any()
+    assert 0 < ga.sum() < len(ga)
+
+
+def test_cut_kills_tets_bit_exact():
+    log("case cut tets plain_cpu f150")
+    d = G.scenario("plain_cpu")
+    ref = _state_from_dump("plain_cpu", "f150")
+    g = make_gpu(d, **MODES["both"])
+    o = G.make_oracle(d, storage="f32", mode="jacobi", iters=3, jacobi_omega=4.0, **MODES["both"])
+    inv_mass = g.read_state()["inv_mass"]
+    for s in (g, o):
+        s.write_state(ref["pos"], ref["prev"], ref["vel"], inv_mass, ref["active"])
+    from fluidphysics_amd import scenes
+    rng = scenes.uniform01(13, 32 * 6).reshape(32, 6)
+    cam = np.array([5.2, 3.6, 6.4])
+    for k in range(32):
+        a = np.array([rng[k, 0] - 0.5, 0.2 + 1.2 * rng[k, 1], rng[k, 2] - 0.5])
+        b = np.array([rng[k, 3] - 0.5, 0.2 + 1.2 * rng[k, 4], rng[k, 5] - 0.5])
+        d1 = (a - cam) / np.linalg.norm(a - cam)
+        d2 = (b - cam) / np.linalg.norm(b - cam)
+        ng = g.cut(*cam, *d1, *d2)
+        no = o.cut(cam, d1, d2)
+        assert ng == no, "cut %d: %d vs %d" % (k, ng, no)
+    _, ga = g.read_tets()
+    oa = o.read_tets()["alive"]
+    log("  tets alive after 32 cuts: %d (oracle %d) of %d" % (int(ga.sum()), int(oa.sum()), len(ga)))
+    assert np.array_equal(ga, oa)
+    assert 0 < ga.sum() < len(ga)
+    assert np.array_equal(g.read_constraints()["alive"], o.read_constraints()["alive"])
+    # reset_solid brings every tet back
+    g.resetSolid()
+    assert g.read_tets()[1].all()
+
+
+@pytest.mark.parametrize("scene_name,frames", [("plain_cpu", 40), ("single_gpu", 12), ("large", 6)])
+def test_captured_frames_match_eager_frames(scene_name, frames):
+    """The captured frame (side stream at large) and the eager frame (the
+    pass profiler keeps frames eager) give bit-identical states with both
+    optional modes on, across a cut."""
+    from fluidphysics_amd import from_scene, scenes
+    log("case solid modes graph vs eager %s frames=%d" % (scene_name, frames))
+    if scene_name == "plain_cpu":
+        sc = scenes.plain_cpu_scene()
+    else:
+        sc = scenes.SCENES[scene_name]()
+    runs = []
+    for eager in (False, True):
+        s = from_scene(sc, **MODES["both"])
+        if eager:
+            s.profile(True)
+        for f in range(frames):
+            if f == frames // 2:
+                cam, c = np.array([5.2, 3.6, 6.4]), np.array(sc.cube[:3])
+                d1 = c + [-0.05, 1.0, -1.0] - cam
+                d2 = c + [-0.05, -1.0, 1.0] - cam
+                s.cut(*cam, *(d1 / np.linalg.norm(d1)), *(d2 / np.linalg.norm(d2)))
+            s.step(sc.dt, sc.substeps)
+        st = s.read_state()
+        runs.append((st, s.read_tets()[1], s.read_constraints()["alive"], s.info()))
+        s.close()
+    (a, ta, ca, ia), (b, tb, cb, ib) = runs
+    log("  tets alive %d of %d, constraints alive %d" % (int(ta.sum()), len(ta), int(ca.sum())))
+    assert np.isfinite(a["pos"]).all()
+    for k in ("pos", "prev", "vel", "active"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(ta, tb)
+    assert np.array_equal(ca, cb)
+
+
+def test_modes_150_frames_statistics_vs_fp64_jacobi():
+    """150 frames of the plain scene with both modes and its slice at frame
+    100, GPU against the FP64 oracle: time-mean kinetic energy and fluid
+    density within 2%, alive tets and constraints within 2% (the runs
+    decorrelate at the level of single contacts, so the bar is statistical,
+    as for the default modes' 300-frame trace)."""
+    log("case solid modes 150 frames plain_cpu")
+    d, t = G.scenario("plain_cpu"), G.trace("plain_cpu")
+    g = make_gpu(d, iters=3, omega=8.0, **MODES["both"])
+    o = G.make_oracle(d, storage="f64", mode="jacobi", iters=3, jacobi_omega=8.0, **MODES["both"])
+    dg, do = G.Driver(d), G.Driver(d)
+    ad = GpuAdapter(g)
+    ke, ok, rho, orho = [], [], [], []
+    for f in range(150):
+        dg.before_frame(ad, f, lambda: g.read_state()["pos"], lambda: g.read_state()["group"])
+        do.before_frame(o, f, lambda: o.read_state()["pos"], lambda: o.read_state()["group"])
+        g.step(1.0 / 60.0, 5)
+        o.step(1.0 / 60.0, 5)
+        st, ost = g.read_state(), o.read_state()
+        assert np.isfinite(st["pos"]).all()
+        r, _ = g.read_density()
+        orr = o.read_density()[0]
+        fg, fo = G.frame_stats(st, t["solidCount"], r), G.frame_stats(ost, t["solidCount"], orr)
+        ke.append(fg["ke"])
+        ok.append(fo["ke"])
+        rho.append(fg["meanRho"])
+        orho.append(fo["meanRho"])
+    ta, oa = int(g.read_tets()[1].sum()), int(o.read_tets()["alive"].sum())
+    ca, oca = int(g.read_constraints()["alive"].sum()), int(o.read_constraints()["alive"].sum())
+    log("  mean KE %.4f vs %.4f; mean rho %.5f vs %.5f; tets %d vs %d; constraints %d vs %d; cuts %s vs %s"
+        % (np.mean(ke), np.mean(ok), np.mean(rho), np.mean(orho), ta, oa, ca, oca, dg.cuts, do.cuts))
+    assert abs(np.mean(ke) / np.mean(ok) - 1) < 0.02
+    assert abs(np.mean(rho) / np.mean(orho) - 1) < 0.02
+    assert abs(ta / max(oa, 1) - 1) < 0.02 and oa < 1080
+    assert abs(ca / max(oca, 1) - 1) < 0.02
