@@ -415,6 +415,20 @@ class Solver:
             self._check(self._lib.fp_read_tets(self._h, _ip(v), _u8p(a), ctypes.byref(nt)))
         return v, a
 
+    def read_cells(self, which=0):
+        """The last substep's cell sort (0: grid order, 1: fluid order):
+        dict(inv_cell, M, mx, my, slots, start [M+1], ids [slots], pos [slots, 4])."""
+        desc = np.zeros(5, np.float64)
+        self._check(self._lib.fp_read_cells(self._h, int(which), _dp(desc), None, None, None))
+        M = int(desc[1])
+        start = np.zeros(M + 1, np.int32)
+        ids = np.zeros(self.capacity, np.int32)
+        pos = np.zeros((self.capacity, 4), np.float32)
+        self._check(self._lib.fp_read_cells(self._h, int(which), _dp(desc), _ip(start), _ip(ids), _fp(pos)))
+        k = int(desc[4])
+        return dict(inv_cell=float(desc[0]), M=M, mx=int(desc[2]), my=int(desc[3]), slots=k, start=start,
+                    ids=ids[:k], pos=pos[:k])
+
     def read_density(self):
         n = self.capacity
         rho = np.zeros(n, np.float32)

@@ -166,6 +166,7 @@ SIGNATURES = {
     "fp_read_constraints": (_I, [_P, _IP, _IP, _FP, _U8P]),
     "fp_write_constraints": (_I, [_P, _FP, _U8P]),
     "fp_read_tets": (_I, [_P, _IP, _U8P, _IP]),
+    "fp_read_cells": (_I, [_P, _I, _DP, _IP, _IP, _FP]),
     "fp_build_grid": (_I, [_P]),
     "fp_read_grid": (_I, [_P, _IP, _IP]),
     "fp_build_grid_noalloc": (_I, [_P, _P, _I, _I, _D, _P, _P, _P]),

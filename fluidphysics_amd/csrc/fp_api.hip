@@ -1203,6 +1203,31 @@ int fp_read_grid(fp_solver* s, int32_t* cell_start, int32_t* cell_entries) {
   return FP_OK;
 }
 
+int fp_read_cells(fp_solver* s, int32_t which, double* desc, int32_t* start, int32_t* ids, float* pos) {
+  if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_read_cells: NULL solver");
+  if (which != 0 && which != 1) return fp_fail(s, FP_EINVAL, "fp_read_cells: which must be 0 or 1");
+  if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_read_cells before fp_add_solid_cube");
+  if (!s->cells_built) return fp_fail(s, FP_ESTATE, "fp_read_cells before the first substep");
+  int rc = set_device(s);
+  if (rc) return rc;
+  if ((rc = settle(s))) return rc;
+  const size_t M = s->cg.M;
+  const int* dstart = which ? s->lstart_f : s->lstart;
+  int32_t slots = 0;
+  FP_TRY(s, d2h(s, &slots, dstart + M, sizeof(int32_t)));
+  if (desc) {
+    desc[0] = s->cg.inv_cell;
+    desc[1] = (double)s->cg.M;
+    desc[2] = (double)s->cg.mx;
+    desc[3] = (double)s->cg.my;
+    desc[4] = (double)slots;
+  }
+  if (start) FP_TRY(s, d2h(s, start, dstart, (M + 1) * sizeof(int32_t)));
+  if (ids) FP_TRY(s, d2h(s, ids, which ? s->fsid : s->sid, (size_t)s->n * sizeof(int32_t)));
+  if (pos) FP_TRY(s, d2h(s, pos, which ? s->spos2 : s->spos, (size_t)s->n * sizeof(float4)));
+  return FP_OK;
+}
+
 int fp_read_density(fp_solver* s, float* rho, float* rho_near) {
   if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_read_density: NULL solver");
   if (!s->has_cube) return fp_fail(s, FP_ESTATE, "fp_read_density before fp_add_solid_cube");

@@ -265,7 +265,10 @@ int fp_group_cell_grid(fp_group* g, bool count_done) {
     GROUP_LAUNCH(g, kg_cell_rank, W_PARTICLE, BLOCK);
   }
   FP_GTRY(g, hipGetLastError());
-  for (fp_solver* s : g->scenes) s->ref_grid_valid = false;
+  for (fp_solver* s : g->scenes) {
+    s->ref_grid_valid = false;
+    s->cells_built = true;
+  }
   return FP_OK;
 }
 

@@ -161,6 +161,7 @@ struct fp_solver {
   int* sbox = nullptr;  // solids' bounding box at grid time (6 ordered ints)
   // the reference's hash grid, built on demand
   bool ref_grid_valid = false;
+  bool cells_built = false;  // a substep has built the two cell orders (fp_read_cells)
   int scan_blocks = 0;
   uint32_t* pslot = nullptr;
   int *count = nullptr, *cell_start = nullptr, *entries_tmp = nullptr, *cell_entries = nullptr;

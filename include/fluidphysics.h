@@ -240,6 +240,20 @@ int fp_write_constraints(fp_solver* s, const float* rest, const uint8_t* alive);
  * volume is positive) and alive (1 per tet) may be NULL. */
 int fp_read_tets(fp_solver* s, int32_t* verts, uint8_t* alive, int32_t* num_tets);
 
+/* No counterpart in the reference: read access to the substep's own cell
+ * sort, for tests.  The last substep sorted the particles twice by cell
+ * (sort cells of cell_size / 2, keyed (cx * mx + cy * my + cz) mod M with the
+ * cell coordinates floor(p * inv_cell) as 32-bit integers): which = 0 is the
+ * grid order (every active particle by its position after integrate and the
+ * grab pull), which = 1 the fluid order (every active fluid particle by its
+ * position after the contact pass).  Inside a key, ascending particle index.
+ * desc (5 doubles): inv_cell, M, mx, my, filled slots; start (M + 1 int32):
+ * first slot of each key; ids (capacity int32): particle of each slot; pos
+ * (4 floats per slot, capacity slots): the position the slot was sorted by
+ * (w: the inverse mass in the grid order, index and flag bits in the fluid
+ * order).  Any output may be NULL; FP_ESTATE before the first substep. */
+int fp_read_cells(fp_solver* s, int32_t which, double* desc, int32_t* start, int32_t* ids, float* pos);
+
 /* _buildGrid() (solver.js:174-184) on the current positions, on the device. */
 int fp_build_grid(fp_solver* s);
 
