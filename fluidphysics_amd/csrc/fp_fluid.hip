@@ -996,7 +996,7 @@ int fp_scene_blocks(const fp_solver* s, int kind, int contact_lanes) {
     case W_FLUID:
       return (int)(((int64_t)s->n * s->fluid_lanes + FLUID_BLOCK - 1) / FLUID_BLOCK);
     case W_CONS:
-      return (int)(((int64_t)s->ns * 32 + BLOCK - 1) / BLOCK);
+      return (int)(((int64_t)s->ns * FP_CLANES + BLOCK - 1) / BLOCK);
     case W_TET:
       return s->tets_on ? fp_blocks(s->NT) : 0;
   }
