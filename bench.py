@@ -452,10 +452,10 @@ def main():
         if args.serving_capacity_rate > 0:
             serving["capacity"] = serving_leg(args.serving_capacity_rate, args.serving_seconds,
                                               args.serving_budget_gb * 1e9)
-            c = serving["capacity"]
+            cap = serving["capacity"]
             print("bench: serving at %g Hz: %.3e p-substeps/s, frame p99 %.2f ms, queue p99 %.2f ms; per-scene %.3e" %
-                  (args.serving_capacity_rate, c["batch"]["value"], c["batch"]["frame_ms_p99"],
-                   c["batch"]["queue_ms_p99"], c["per_scene"]["value"]), file=sys.stderr, flush=True)
+                  (args.serving_capacity_rate, cap["batch"]["value"], cap["batch"]["frame_ms_p99"],
+                   cap["batch"]["queue_ms_p99"], cap["per_scene"]["value"]), file=sys.stderr, flush=True)
         print("bench: serving %.3e p-substeps/s, frame p50 %.2f ms p99 %.2f ms; per-scene %.3e p99 %.2f ms" %
               (serving["batch"]["value"], serving["batch"]["frame_ms_p50"], serving["batch"]["frame_ms_p99"],
                serving["per_scene"]["value"], serving["per_scene"]["frame_ms_p99"]), file=sys.stderr, flush=True)
