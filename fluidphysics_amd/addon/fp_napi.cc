@@ -479,6 +479,20 @@ napi_value ReadGrid(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// readTets(h, verts | null, alive | null) -> the tet count (the optional tet
+// mode's tets; 0 without it); verts holds 4 ints per tet, alive 1 byte.
+napi_value ReadTets(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_solver* s = handle(env, a.v[0]);
+  int32_t nt = 0;
+  FP_CALL(env, s, fp_read_tets(s, nullptr, nullptr, &nt), "fp_read_tets");
+  TYPED(env, a.v[1], int32_t, napi_int32_array, 4 * (size_t)nt, verts);
+  TYPED(env, a.v[2], uint8_t, napi_uint8_array, (size_t)nt, alive);
+  if (verts || alive) FP_CALL(env, s, fp_read_tets(s, verts, alive, &nt), "fp_read_tets");
+  return make_int(env, nt);
+}
+
 napi_value BuildGrid(napi_env env, napi_callback_info info) {
   Args a;
   NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
@@ -686,7 +700,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"pickSolid", PickSolid},     {"fluidCount", FluidCount}, {"resetSolid", ResetSolid},
       {"selectSolids", SelectSolids}, {"components", Components},
       {"clearFluid", ClearFluid},   {"readState", ReadState},   {"writeState", WriteState},
-      {"readConstraints", ReadConstraints}, {"readGrid", ReadGrid}, {"buildGrid", BuildGrid},
+      {"readConstraints", ReadConstraints}, {"readGrid", ReadGrid}, {"readTets", ReadTets}, {"buildGrid", BuildGrid},
       {"readDensity", ReadDensity},
       {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy}, {"batchAdd", BatchAdd},
       {"batchRemove", BatchRemove}, {"batchStep", BatchStep},       {"batchSync", BatchSync},

@@ -216,6 +216,15 @@ export class Solver {
     return { cellStart: cs, cellEntries: ce };
   }
 
+  // the optional tet mode's tets (tetVolume): 4 vertex indices and an alive
+  // flag per tet
+  readTets() {
+    const n = addon.readTets(this._h, null, null);
+    const verts = new Int32Array(4 * n), alive = new Uint8Array(n);
+    if (n) addon.readTets(this._h, verts, alive);
+    return { verts, alive };
+  }
+
   _buildGrid() { this._flush(); addon.buildGrid(this._h); }
 
   info() { return addon.info(this._h); }

@@ -18,5 +18,10 @@ for (let f = 0; f < frames; f++) {
 }
 const p = s.pos, v = s.vel, a = s.active;
 fs.writeFileSync(out, Buffer.concat([Buffer.from(p.buffer), Buffer.from(v.buffer), Buffer.from(a.buffer)]));
+if (d.dumpTets) {
+  // the optional tet mode's topology and alive flags: verts i32[4T] then alive u8[T]
+  const t = s.readTets();
+  fs.writeFileSync(out + '.tets', Buffer.concat([Buffer.from(t.verts.buffer), Buffer.from(t.alive.buffer)]));
+}
 console.log(JSON.stringify({ capacity: s.capacity, fluid: s.fluidCount(), tableSize: s.tableSize }));
 s.destroy();

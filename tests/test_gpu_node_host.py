@@ -62,6 +62,35 @@ def test_node_host_bit_identical_to_ctypes_host(tmp_path):
     assert np.array_equal(st["active"], act)
 
 
+def test_node_host_solid_modes_bit_identical_to_ctypes_host(tmp_path):
+    """The optional solid modes set through the Node host's tuning names
+    (tetVolume, solidContactCache): 110 frames with the slice at frame 100
+    equal the ctypes host's bit for bit, tets included."""
+    _need_node()
+    log("case node host vs ctypes host, solid modes, plain_cpu 110 frames")
+    d = G.scenario("plain_cpu")
+    d["params"] = {"tetVolume": 1, "solidContactCache": 1}
+    d["dumpTets"] = True
+    pos, vel, act = _run_node(d, tmp_path, 110, 5, 1.0 / 60.0)
+    raw = (tmp_path / "out.bin.tets").read_bytes()
+    g = make_gpu(d, iters=None, omega=None, tet_volume=1, solid_contact_cache=1)
+    for f in range(110):
+        for c in d["cuts"]:
+            if c["frame"] == f:
+                g.cut(*c["o"], *c["d1"], *c["d2"])
+        g.step(1.0 / 60.0, 5)
+    st = g.read_state()
+    v, a = g.read_tets()
+    nt = len(a)
+    assert nt == 5 * 6 ** 3 and len(raw) == 17 * nt
+    assert np.array_equal(np.frombuffer(raw, np.int32, 4 * nt, 0).reshape(-1, 4), v)
+    assert np.array_equal(np.frombuffer(raw, np.uint8, nt, 16 * nt), a)
+    assert a.sum() < nt  # the slice killed tets
+    assert np.array_equal(st["pos"], pos)
+    assert np.array_equal(st["vel"], vel)
+    assert np.array_equal(st["active"], act)
+
+
 def test_node_host_first_substep_vs_fp64_reference(tmp_path):
     _need_node()
     log("case node host one substep plain_cpu")
