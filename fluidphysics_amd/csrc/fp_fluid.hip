@@ -63,6 +63,9 @@
 #ifndef REPLAY_BATCH
 #define REPLAY_BATCH 4  // pair-list entries per step of the displacement and XSPH replay
 #endif
+#ifndef DENSITY_STAGE
+#define DENSITY_STAGE 16  // pair-list entries per lane the staging density kernel holds in LDS (> 0)
+#endif
 #ifndef DENSITY_BATCH
 #define DENSITY_BATCH 1  // candidates per step of the density loop (1: fastest measured, also in batches)
 #endif
@@ -562,8 +565,11 @@ struct DensityGather {
   uint32_t* base;  // the wave's block of the pair list
   uint32_t wofs;   // byte offset of entry cnt (the sink entry once cnt >= cap)
   uint32_t wend;   // byte offset of this lane's sink entry
+  uint32_t* lds;   // (staging kernel) this thread's first staged entry (entry k at lds[k * FLUID_BLOCK])
+  int pb;          // (staging kernel) entries staged in LDS (the rest go straight to the pair list)
 };
 
+template <bool kStage>
 __device__ __forceinline__ void density_take(const FluidParams& fp, const PairList& pl, int q, int w, float d2,
                                              DensityGather& g) {
   const float u = fmaf(-fast_sqrt(d2), fp.inv_H, 1.f);
@@ -571,9 +577,16 @@ __device__ __forceinline__ void density_take(const FluidParams& fp, const PairLi
   g.rho += w ? u2 : 0.f;
   g.rhon += w ? u2 * u : 0.f;
   if (w) {
-    // entries past pl.cap all land in the sink row pl.cap, which nothing reads
-    gst_at(g.base, g.wofs, (uint32_t)q | ((uint32_t)(w - 1) << NB_QBITS));
-    g.wofs = min(g.wofs + NB_ROWB, g.wend);
+    const uint32_t e = (uint32_t)q | ((uint32_t)(w - 1) << NB_QBITS);
+    if (kStage && g.cnt < g.pb) {
+      // the first pb entries wait in LDS and are written out row by row at
+      // the end (whole rows of the wave's block per store)
+      g.lds[g.cnt * FLUID_BLOCK] = e;
+    } else {
+      // entries past pl.cap all land in the sink row pl.cap, which nothing reads
+      gst_at(g.base, g.wofs, e);
+      g.wofs = min(g.wofs + NB_ROWB, g.wend);
+    }
     g.cnt++;
   }
 }
@@ -583,6 +596,7 @@ __device__ __forceinline__ void density_take(const FluidParams& fp, const PairLi
 // either particle; the band and flagged cases go to the one rare branch.
 // flag_min: the least w bits of a candidate that make the pair flagged (0
 // when me itself is flagged, so every candidate is).
+template <bool kStage>
 __device__ __forceinline__ void density_batch(const FluidParams& fp, const PairSource& src, const PairList& pl,
                                               const PairSelf& me, uint32_t flag_min, int q, int last, const float4* c,
                                               DensityGather& g) {
@@ -607,7 +621,7 @@ __device__ __forceinline__ void density_batch(const FluidParams& fp, const PairS
       if ((q + i <= last) & (q + i != me.e) & (d2[i] < fp.H2_hi)) w[i] = slow_weight(fp, src, me, q + i, c[i], d2[i]);
   }
 #pragma unroll
-  for (int i = 0; i < DENSITY_BATCH; i++) density_take(fp, pl, q + i, w[i], d2[i], g);
+  for (int i = 0; i < DENSITY_BATCH; i++) density_take<kStage>(fp, pl, q + i, w[i], d2[i], g);
 }
 
 __device__ __forceinline__ void load_batch(const float4* __restrict__ row, int q, int last, float4* b) {
@@ -615,6 +629,7 @@ __device__ __forceinline__ void load_batch(const float4* __restrict__ row, int q
   for (int i = 0; i < DENSITY_BATCH; i++) b[i] = gld_at(row, (uint32_t)min(q + i, last) << 4);
 }
 
+template <bool kStage>
 __device__ __forceinline__ void density_gather(const FluidParams& fp, const PairSource& src, const PairList& pl,
                                                const PairSelf& me, DensityGather& g) {
   const Cell c = cell_of(me.p.x, me.p.y, me.p.z, fp.cg.inv_cell);
@@ -642,18 +657,18 @@ __device__ __forceinline__ void density_gather(const FluidParams& fp, const Pair
       while (q <= last) {
         const int q1 = q + step;
         load_batch(row, q1, last, b1);
-        density_batch(fp, src, pl, me, flag_min, q, last, b0, g);
+        density_batch<kStage>(fp, src, pl, me, flag_min, q, last, b0, g);
         if (q1 > last) break;
         const int q2 = q1 + step;
         load_batch(row, q2, last, b0);
-        density_batch(fp, src, pl, me, flag_min, q1, last, b1, g);
+        density_batch<kStage>(fp, src, pl, me, flag_min, q1, last, b1, g);
         q = q2;
       }
     }
     for (int q = 0; q < (me.h == 0 ? rr.w1 : 0); q++) {
       float dx, dy, dz, d2;
       const int w = pair_weight(fp, src, me, q, gld(row + q), dx, dy, dz, d2);
-      density_take(fp, pl, q, w, d2, g);
+      density_take<kStage>(fp, pl, q, w, d2, g);
     }
   }
 }
@@ -775,7 +790,12 @@ __device__ __forceinline__ void density_store(const FluidParams& fp, const PairL
   gst(sP + me.e, make_float2(fp.stiff * (rho - fp.rest_density), fp.near_stiff * rhon));
 }
 
-__device__ __forceinline__ void density_body(const SceneArgs& A, int lb) {
+// kStage: the first fp.stage entries of each lane wait in LDS (lds,
+// DENSITY_STAGE per thread) and are written out row by row at the end, so
+// a store covers whole rows of the wave's block instead of one entry per
+// lane scattered over several rows.
+template <bool kStage>
+__device__ __forceinline__ void density_body(const SceneArgs& A, int lb, uint32_t* __restrict__ lds) {
   const FluidParams& fp = A.fp;
   const PairSource& src = A.src;
   const PairList& pl = A.pl;
@@ -785,7 +805,17 @@ __device__ __forceinline__ void density_body(const SceneArgs& A, int lb) {
   g.base = pair_block(pl, me.v);
   g.wofs = pair_lane_off(me.v);
   g.wend = g.wofs + (uint32_t)pl.cap * NB_ROWB;
-  density_gather(fp, src, pl, me, g);
+  g.lds = lds + threadIdx.x;
+  g.pb = kStage ? min(fp.stage, pl.cap) : 0;
+  const uint32_t lane_off = g.wofs;
+  g.wofs += (uint32_t)g.pb * NB_ROWB;
+  density_gather<kStage>(fp, src, pl, me, g);
+  if (kStage) {
+    // the staged entries, row by row: the lanes of a wave that have entry
+    // k write their part of row k together
+    const int n = min(g.cnt, g.pb);
+    for (int k = 0; k < n; k++) gst_at(g.base, lane_off + (uint32_t)k * NB_ROWB, g.lds[k * FLUID_BLOCK]);
+  }
   density_store(fp, pl, me, g, A.sP, A.rho, A.dflags);
 }
 
@@ -831,9 +861,11 @@ __device__ __forceinline__ int fluid_block(int j, int o, int a) {
 #endif
 #define DENSITY_ATTR WAVES_ATTR(DENSITY_WAVES)
 
+template <bool kStage>
 __global__ void __launch_bounds__(FLUID_BLOCK) DENSITY_ATTR kg_density(const SceneArgs* __restrict__ S,
                                                           const int2* __restrict__ W) {
-  FLUID_ITEMS(S, W) density_body(A, lb);
+  __shared__ uint32_t lds[kStage ? DENSITY_STAGE * FLUID_BLOCK : 1];
+  FLUID_ITEMS(S, W) density_body<kStage>(A, lb, lds);
 }
 
 struct DisplaceAcc {
@@ -979,6 +1011,10 @@ FluidParams fp_fluid_params(const fp_solver* s) {
   fp.rest_density = (float)P.rest_density;
   fp.contact_scale = (float)P.contact_scale;
   fp.fl_log2 = s->fluid_lanes == 8 ? 3 : s->fluid_lanes == 4 ? 2 : s->fluid_lanes == 2 ? 1 : 0;
+  // staging pays where a lane records many pairs (2 lanes per particle: the
+  // 256-scene batch's density 5.65 -> 4.90 ms per frame); at 4 lanes it cost
+  // the large scene 2%
+  fp.stage = s->fluid_lanes <= 2 ? DENSITY_STAGE : 0;
 
   fp.visc = (float)P.viscosity;
   return fp;
@@ -1062,7 +1098,10 @@ int fp_group_contacts(fp_group* g) {
 
 int fp_group_density(fp_group* g) {
   PassScope ps(g->prof, g->stream, P_DENSITY);
-  GROUP_LAUNCH(g, kg_density, W_FLUID, FLUID_BLOCK);
+  if (g->any_stage)
+    GROUP_LAUNCH(g, kg_density<true>, W_FLUID, FLUID_BLOCK);
+  else
+    GROUP_LAUNCH(g, kg_density<false>, W_FLUID, FLUID_BLOCK);
   FP_GTRY(g, hipGetLastError());
   return FP_OK;
 }

@@ -243,6 +243,7 @@ struct FluidParams {
   CellGrid cg;
   int ns, n, nf;
   int fl_log2;  // log2 of the lanes per fluid particle
+  int stage;    // pair-list entries per lane the density pass stages in LDS
   float rs, rf;
   float inv_H, H2_lo, H2_hi, max_pair, max_total;
   double H2d;
@@ -403,6 +404,7 @@ struct fp_group {
   bool any_extra = false;          // a member uses the volume tets or the solid contact cache
   bool any_cache = false;          // ... the solid contact cache
   bool any_tets = false;           // ... the volume tets
+  bool any_stage = false;          // a member's density pass stages its pair-list entries in LDS
   // captured frame
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -411,7 +413,7 @@ struct fp_group {
   int graph_iters = -1;
   int graph_clanes = 0;             // contact lanes the captured frame launches
   bool graph_unconstrained = false;  // the captured frame has the bounds pass of unconstrained scenes
-  int graph_extra = -1;              // the optional-mode passes the captured frame has (bit 0 cache, 1 tets)
+  int graph_extra = -1;              // the captured frame's optional passes (bit 0 cache, 1 tets) and staging density kernel (bit 2)
   bool use_graph = true;
   int64_t graph_launches = 0;
   int64_t graph_captures = 0;

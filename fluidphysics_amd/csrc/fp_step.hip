@@ -327,9 +327,10 @@ int fp_group_prepare(fp_group* g, double h) {
   }
   if (g->iters < 0) g->iters = 0;
   g->any_unconstrained = false;
-  g->any_cache = g->any_tets = false;
+  g->any_cache = g->any_tets = g->any_stage = false;
   for (int k = 0; k < ns; k++) {
     g->any_unconstrained |= args[k].cp.ns > 0 && args[k].cp.iters == 0;
+    g->any_stage |= args[k].fp.stage > 0;
     g->any_cache |= args[k].sm.cache != 0;
     g->any_tets |= args[k].sm.tets != 0;
   }
@@ -562,9 +563,9 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
     // hold; it is captured again when its structure no longer fits: the
     // substep or iteration count, the contact pass's lane count (a kernel
     // template), the bounds pass of unconstrained scenes, the optional solid
-    // modes' passes, or work lists that outgrew (or shrank far below) the
-    // grids captured for them
-    const int extra = (g->any_cache ? 1 : 0) | (g->any_tets ? 2 : 0);
+    // modes' passes, the density kernel (staging or not), or work lists that
+    // outgrew (or shrank far below) the grids captured for them
+    const int extra = (g->any_cache ? 1 : 0) | (g->any_tets ? 2 : 0) | (g->any_stage ? 4 : 0);
     bool fits = g->gexec && g->graph_sub == substeps && g->graph_iters == g->iters &&
                 g->graph_clanes == g->clanes && g->graph_unconstrained == g->any_unconstrained &&
                 g->graph_extra == extra;
