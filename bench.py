@@ -98,7 +98,12 @@ def measured_traffic(workload, pass_name):
         return None
     if t.get("workload") != workload:
         return None
-    k = t.get("kernels", {}).get("kg_" + pass_name)
+    ks = t.get("kernels", {})
+    # the pass's kernel, or its one template instance in the run (kg_density<false>)
+    k = ks.get("kg_" + pass_name)
+    if k is None:
+        inst = [v for n, v in ks.items() if n.startswith("kg_" + pass_name + "<")]
+        k = inst[0] if len(inst) == 1 else None
     return int(k["hbm_bytes"]) if k else None
 
 
