@@ -221,6 +221,58 @@ def test_captured_frames_match_eager_frames(scene_name, frames):
     assert np.array_equal(ca, cb)
 
 
+def test_mixed_batch_members_equal_scenes_alone():
+    """A batch whose members differ in their optional modes (tets + cache,
+    cache only, none; batched-workload scenes with tilted gravity and
+    slices) is bit-identical, frame by frame, to each scene stepped alone,
+    through eager, captured and recaptured frames (a member leaving changes
+    which optional passes the frame holds)."""
+    from fluidphysics_amd import Batch, from_scene, scenes
+    log("case mixed-mode batch vs alone")
+    frames = 12
+    scs = [scenes.batched_scene(k, frames=frames) for k in (1, 2, 3)]
+    for k, sc in enumerate(scs):
+        for j, w in enumerate(sc.wedges):
+            w.frame = 2 + 3 * j + k
+    modes = [MODES["both"], MODES["cache"], {}]
+
+    def cuts(s, sc, f):
+        for w in sc.wedges:
+            if w.frame == f:
+                s.cut(*w.o, *w.d1, *w.d2)
+
+    alone = []
+    for sc, m in zip(scs, modes):
+        s = from_scene(sc, **m)
+        snaps = []
+        for f in range(frames):
+            cuts(s, sc, f)
+            s.step(sc.dt, sc.substeps)
+            snaps.append((s.read_state()["pos"].copy(), s.read_tets()[1].copy()))
+        alone.append(snaps)
+        s.close()
+    b = Batch()
+    ss = [from_scene(sc, **m) for sc, m in zip(scs, modes)]
+    for s in ss:
+        b.add(s)
+    for f in range(frames):
+        if f == frames - 4:
+            b.remove(ss[0])  # the only member with tets leaves; it steps on alone
+        for k, (s, sc) in enumerate(zip(ss, scs)):
+            cuts(s, sc, f)
+        b.step(scs[0].dt, scs[0].substeps)
+        if f >= frames - 4:
+            ss[0].step(scs[0].dt, scs[0].substeps)
+        for k, s in enumerate(ss):
+            pos, tets = alone[k][f]
+            assert np.array_equal(s.read_state()["pos"], pos), "scene %d frame %d" % (k, f)
+            assert np.array_equal(s.read_tets()[1], tets), "scene %d frame %d tets" % (k, f)
+    inf = b.info()
+    log("  graph captures %d, launches %d" % (inf.graph_captures, inf.graph_launches))
+    assert inf.graph_launches > 0
+    b.close()
+
+
 def test_modes_150_frames_statistics_vs_fp64_jacobi():
     """150 frames of the plain scene with both modes and its slice at frame
     100, GPU against the FP64 oracle: time-mean kinetic energy and fluid
