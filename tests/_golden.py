@@ -123,9 +123,9 @@ class Driver:
 def frame_stats(st, solid_count, rho=None):
     """Kinetic energy, mean density, fluid count (make_golden.mjs frameStats)."""
     act = st["active"].astype(bool)
-    v = st["vel"].astype(np.float64).reshape(-1, 3)
-    m = 1.0 / st["inv_mass"].astype(np.float64)
-    ke = float(np.sum(0.5 * m[act] * np.sum(v[act] * v[act], axis=1)))
+    v = st["vel"].astype(np.float64).reshape(-1, 3)[act]
+    m = 1.0 / st["inv_mass"].astype(np.float64)[act]
+    ke = float(np.sum(0.5 * m * np.sum(v * v, axis=1)))
     fl = act.copy()
     fl[:solid_count] = False
     mean_rho = float(np.mean(rho[fl])) if (rho is not None and fl.any()) else 0.0

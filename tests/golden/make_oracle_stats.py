@@ -3,11 +3,13 @@ HIP kernels implement; pinned by tests/test_oracle_pins.py) on a GPU-scale
 scene, for the GPU's multi-frame statistics test:
 
     python3 tests/golden/make_oracle_stats.py single_gpu [frames]
+    python3 tests/golden/make_oracle_stats.py large [frames]
 
 writes tests/golden/<scene>_jacobi_trace.json: per-frame kinetic energy,
 mean fluid density, alive constraints and fluid count of the oracle
 (float32 stores, Jacobi mode, the scene's constraint_iters / jacobi_omega),
-with the scene's wedges.  The reference (src/solver.js) itself is not usable
+with the scene's wedges and hose bursts (the large scene: about 27 minutes
+for 300 frames on one core).  The reference (src/solver.js) itself is not usable
 as the target at these constants: its single Gauss-Seidel sweep goes
 unstable on the 20^3 cube within 7 frames (single_gpu_stats_trace.json,
 written by make_golden.mjs from the reference)."""
@@ -30,9 +32,9 @@ def frame_stats(o, ns):
     st = o.read_state()
     rho, _ = o.read_density()
     act = st["active"].astype(bool)
-    v = st["vel"].astype(np.float64).reshape(-1, 3)
-    m = 1.0 / st["inv_mass"].astype(np.float64)
-    ke = float(np.sum(0.5 * m[act] * np.sum(v[act] * v[act], axis=1)))
+    v = st["vel"].astype(np.float64).reshape(-1, 3)[act]
+    m = 1.0 / st["inv_mass"].astype(np.float64)[act]
+    ke = float(np.sum(0.5 * m * np.sum(v * v, axis=1)))
     fl = act.copy()
     fl[:ns] = False
     return dict(ke=ke, meanRho=float(np.mean(rho[fl])) if fl.any() else 0.0,

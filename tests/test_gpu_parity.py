@@ -49,11 +49,20 @@ def test_grid_matches_reference_golden(name):
 
 
 def _scene_pair(scene_name, frames):
+    """The scene's GPU solver after `frames` frames of its workload: the
+    hose bursts (config 3) and the wedges (config 2) of those frames
+    included."""
     from fluidphysics_amd import from_scene, scenes
-    from oracle.oracle import OracleSolver
     sc = scenes.SCENES[scene_name]()
     g = from_scene(sc)
-    for _ in range(frames):
+    sp = 1.91 * sc.params_with()["fluid_radius"]
+    for f in range(frames):
+        if sc.hose:
+            p, v = scenes.hose_burst(sc.hose, f, sp)
+            g.emit_batch(p, v, sc.fluid_mass)
+        for w in sc.wedges:
+            if w.frame == f:
+                g.cut(*w.o, *w.d1, *w.d2)
         g.step(sc.dt, sc.substeps)
     return sc, g
 

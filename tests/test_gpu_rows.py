@@ -326,6 +326,49 @@ def test_serving_stream_sample_alone_and_in_the_mix():
 
 # ------------------------------------------- multi-frame statistics, config 2 ----
 
+def test_config3_300_frames_statistics_vs_jacobi_oracle():
+    """The large scene (24^3 cube + 262,144 fluid slab + the 120-per-frame
+    hose) for 300 frames: the time-mean kinetic energy and fluid density, the
+    final alive count and the final fluid count within 1% of the oracle's
+    Jacobi formulation (tests/golden/large_jacobi_trace.json, float32
+    stores, same iterations and relaxation, same hose bursts).  The scene
+    uses the constants of config 2, at which the reference's single
+    Gauss-Seidel sweep is unstable, so the Jacobi oracle is the target."""
+    from fluidphysics_amd import from_scene, scenes
+    sc = scenes.large_scene()
+    t = G.trace("large_jacobi")
+    log("case 300 frames %s vs the Jacobi oracle" % sc.name)
+    assert t["constraint_iters"] == sc.params_with()["constraint_iters"]
+    assert t["jacobi_omega"] == sc.params_with()["jacobi_omega"]
+    g = from_scene(sc)
+    sp = 1.91 * sc.params_with()["fluid_radius"]
+    ke, rho = [], []
+    for f in range(len(t["frames"])):
+        p, v = scenes.hose_burst(sc.hose, f, sp)
+        g.emit_batch(p, v, sc.fluid_mass)
+        g.step(sc.dt, sc.substeps)
+        st = g.read_state()
+        r, _ = g.read_density()
+        fs = G.frame_stats(st, sc.solid_count, r)
+        ke.append(fs["ke"])
+        rho.append(fs["meanRho"])
+        assert np.isfinite(st["pos"]).all()
+    alive = int(g.read_constraints()["alive"].sum())
+    fluid = g.fluidCount()
+    g.close()
+    rke = np.mean([f["stats"]["ke"] for f in t["frames"]])
+    rrho = np.mean([f["stats"]["meanRho"] for f in t["frames"]])
+    ral = t["frames"][-1]["stats"]["alive"]
+    rfl = t["frames"][-1]["stats"]["fluid"]
+    log("  mean KE %.4f vs %.4f (%+.2f%%), mean rho %.5f vs %.5f (%+.2f%%), alive %d vs %d, fluid %d vs %d"
+        % (np.mean(ke), rke, 100 * (np.mean(ke) / rke - 1), np.mean(rho), rrho, 100 * (np.mean(rho) / rrho - 1),
+           alive, ral, fluid, rfl))
+    assert abs(np.mean(ke) / rke - 1) < 0.01
+    assert abs(np.mean(rho) / rrho - 1) < 0.01
+    assert abs(alive / ral - 1) < 0.01
+    assert abs(fluid / rfl - 1) < 0.01
+
+
 def test_config2_300_frames_statistics_vs_jacobi_oracle():
     """The 20^3 + 65,536 scene for 300 frames (its wedge at frame 150): the
     time-mean kinetic energy and fluid density and the final alive count
