@@ -444,9 +444,9 @@ def main():
     batched = None
     if not args.no_batched:
         solo = args.batched_solo_frames if args.batched_solo_frames >= 0 else max(10, args.batched_steps // 3)
-        # warm-up covers the contact-lane tuning frames and the frame that
-        # captures the graph (one-time costs)
-        batched = batched_leg(args.batched_scenes, args.batched_steps, max(6, args.warmup // 2), solo)
+        # warm-up covers the one-time frames: the contact-lane tuning frames,
+        # the launch-shape tuning frames and the frames that capture graphs
+        batched = batched_leg(args.batched_scenes, args.batched_steps, max(20, args.warmup // 2), solo)
         print("bench: batched %.3e p-substeps/s, frame p50 %.2f ms p99 %.2f ms; one at a time %s" %
               (batched["value"], batched["frame_ms_p50"], batched["frame_ms_p99"],
                batched["one_at_a_time"] and "%.3e" % batched["one_at_a_time"]["value"]), file=sys.stderr, flush=True)

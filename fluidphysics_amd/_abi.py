@@ -117,6 +117,9 @@ class FpBatchInfo(ctypes.Structure):
         ("graph_launches", ctypes.c_int64),
         ("contact_lanes", ctypes.c_int32),
         ("graph_captures", ctypes.c_int32),
+        ("density_stage", ctypes.c_int32),
+        ("xcd_run", ctypes.c_int32),
+        ("shape_tuned", ctypes.c_int32),
     ]
 
 

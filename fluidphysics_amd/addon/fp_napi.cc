@@ -623,6 +623,9 @@ napi_value BatchInfo(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, o, "graphLaunches", make_int(env, i.graph_launches));
   napi_set_named_property(env, o, "contactLanes", make_int(env, i.contact_lanes));
   napi_set_named_property(env, o, "graphCaptures", make_int(env, i.graph_captures));
+  napi_set_named_property(env, o, "densityStage", make_int(env, i.density_stage));
+  napi_set_named_property(env, o, "xcdRun", make_int(env, i.xcd_run));
+  napi_set_named_property(env, o, "shapeTuned", make_int(env, i.shape_tuned));
   return o;
 }
 

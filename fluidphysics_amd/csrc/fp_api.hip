@@ -1307,6 +1307,7 @@ int fp_batch_create(int32_t device, fp_batch** out) {
   if (!b) return batch_fail(nullptr, FP_ENOMEM, "fp_batch_create: out of host memory");
   b->g.device = device;
   b->g.prof = &b->prof;
+  b->g.shape_tune = FP_SHAPE_TUNE != 0;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->g.stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -1432,6 +1433,9 @@ int fp_batch_get_info(const fp_batch* b, fp_batch_info* out) {
   out->graph_launches = b->g.graph_launches;
   out->contact_lanes = b->g.clanes_best;
   out->graph_captures = (int32_t)b->g.graph_captures;
+  out->density_stage = b->g.stage_on;
+  out->xcd_run = b->g.xcd_run;
+  out->shape_tuned = b->g.shape_state == 2 ? 1 : 0;
   return FP_OK;
 }
 

@@ -1011,10 +1011,9 @@ FluidParams fp_fluid_params(const fp_solver* s) {
   fp.rest_density = (float)P.rest_density;
   fp.contact_scale = (float)P.contact_scale;
   fp.fl_log2 = s->fluid_lanes == 8 ? 3 : s->fluid_lanes == 4 ? 2 : s->fluid_lanes == 2 ? 1 : 0;
-  // staging pays where a lane records many pairs (2 lanes per particle: the
-  // 256-scene batch's density 5.65 -> 4.90 ms per frame); at 4 lanes it cost
-  // the large scene 2%
-  fp.stage = s->fluid_lanes <= 2 ? DENSITY_STAGE : 0;
+  // entries the staging density kernel holds in LDS per lane (whether the
+  // group runs that kernel is the group's choice, fp_group::stage_on)
+  fp.stage = DENSITY_STAGE;
 
   fp.visc = (float)P.viscosity;
   return fp;
@@ -1098,7 +1097,7 @@ int fp_group_contacts(fp_group* g) {
 
 int fp_group_density(fp_group* g) {
   PassScope ps(g->prof, g->stream, P_DENSITY);
-  if (g->any_stage)
+  if (g->stage_on)
     GROUP_LAUNCH(g, kg_density<true>, W_FLUID, FLUID_BLOCK);
   else
     GROUP_LAUNCH(g, kg_density<false>, W_FLUID, FLUID_BLOCK);

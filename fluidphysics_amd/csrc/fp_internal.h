@@ -66,6 +66,15 @@
 #endif
 #define FLUID_LANES_AUTO 131072
 
+// XCD run length of the group kernels' work lists (the default; a batch may
+// tune it) and whether batches tune their launch shape
+#ifndef FP_XCD_CHUNK
+#define FP_XCD_CHUNK 16
+#endif
+#ifndef FP_SHAPE_TUNE
+#define FP_SHAPE_TUNE 1  // batches tune their launch shape over captured frames
+#endif
+
 // device flag words (dflags)
 #define DF_UNUSED0 0     // (the tear flag lives in host-mapped memory: fp_solver::tear_host)
 #define DF_CUT 1         // knife count
@@ -404,7 +413,21 @@ struct fp_group {
   bool any_extra = false;          // a member uses the volume tets or the solid contact cache
   bool any_cache = false;          // ... the solid contact cache
   bool any_tets = false;           // ... the volume tets
-  bool any_stage = false;          // a member's density pass stages its pair-list entries in LDS
+  bool stage_rule = false;         // a member runs 2 lanes per fluid particle (LDS staging pays there)
+  int stage_on = 0;                // the density kernel stages its pair-list entries in LDS
+  int xcd_run = FP_XCD_CHUNK;      // items per XCD run of the group kernels' lists (their W[0].y)
+  // launch-shape tuner (batches): the density kernel's staging and the XCD
+  // run length, timed over whole captured frames of a stable membership and
+  // cached per group signature (see fp_step.hip)
+  bool shape_tune = false;         // set for batches (FP_SHAPE_TUNE)
+  int shape_state = 0;             // 0 untuned, 1 tuning, 2 settled
+  int shape_cand = 0, shape_done = 0;
+  uint64_t shape_sig = 0;
+  double shape_ms[8] = {0};
+  int shape_n[8] = {0};
+  bool shape_pending = false;
+  int shape_pending_cand = -1;
+  hipEvent_t shape_ev[2] = {nullptr, nullptr};
   // captured frame
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -544,25 +567,29 @@ __device__ __forceinline__ int items_count(const int2* __restrict__ W) {
 // b % 8), each XCD with its own L2; grids are multiples of 8, so item
 // position w runs on XCD w % 8 whether a workgroup takes one item or strides
 // over several.  The work list is dealt to the XCDs in runs of FP_XCD_CHUNK
-// consecutive items, so items next to each other (neighbouring particles,
-// whose gathers read the same rows) share an L2 while every XCD still gets a
-// share of each part of the list (idle capacity at a fluid order's end,
-// scenes of unequal size).  The items past the last whole round of 8 runs
-// keep their own position.
-#ifndef FP_XCD_CHUNK
-#define FP_XCD_CHUNK 16
-#endif
+// consecutive items (K, the list's W[0].y: FP_XCD_CHUNK unless the group's
+// launch-shape tuner chose another), so items next to each other
+// (neighbouring particles, whose gathers read the same rows) share an L2
+// while every XCD still gets a share of each part of the list (idle
+// capacity at a fluid order's end, scenes of unequal size).  The items past
+// the last whole round of 8 runs keep their own position.  A bijection on
+// the list for every K: results do not depend on it.
 #define FP_XCDS 8
-__device__ __forceinline__ int xcd_item(int b, int n) {
-  constexpr int K = FP_XCD_CHUNK, R = FP_XCDS * FP_XCD_CHUNK;
+__device__ __forceinline__ int xcd_item(int b, int n, int K) {
+  const int R = FP_XCDS * K;
   if (K <= 1 || b >= n / R * R) return b;
   const int x = b % FP_XCDS, l = b / FP_XCDS;  // XCD, and the position's place in its turn
   return (l / K * FP_XCDS + x) * K + l % K;
 }
 
+// The XCD run length of work list W (the same in every lane).
+__device__ __forceinline__ int items_run(const int2* __restrict__ W) {
+  return __builtin_amdgcn_readfirstlane(*(const FP_GLOBAL int*)&W[0].y);
+}
+
 #define FOR_ITEMS(S, W)                                                                    \
-  for (int w_ = (int)blockIdx.x, nw_ = items_count(W); w_ < nw_; w_ += (int)gridDim.x)    \
-    if (const int2 it_ = (W)[1 + xcd_item(w_, nw_)]; true)                                 \
+  for (int w_ = (int)blockIdx.x, nw_ = items_count(W), k_ = items_run(W); w_ < nw_; w_ += (int)gridDim.x) \
+    if (const int2 it_ = (W)[1 + xcd_item(w_, nw_, k_)]; true)                             \
       if (const SceneArgs& A = (S)[it_.x]; true)                                           \
         if (const int lb = it_.y; true)
 

@@ -20,6 +20,9 @@
 // A frame without grabs is captured once into a HIP graph and replayed.
 #include <cstring>
 
+#include <map>
+#include <mutex>
+
 #include "fp_internal.h"
 #include "fp_kernels.h"
 #include "fp_bounds.h"
@@ -242,6 +245,11 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
 }
 
 void fp_group_free(fp_group* g) {
+  for (hipEvent_t& e : g->shape_ev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
+  g->shape_pending = false;
   for (hipEvent_t e : g->tune_ev) (void)hipEventDestroy(e);
   g->tune_ev.clear();
   if (g->side) (void)hipStreamSynchronize(g->side);
@@ -327,10 +335,13 @@ int fp_group_prepare(fp_group* g, double h) {
   }
   if (g->iters < 0) g->iters = 0;
   g->any_unconstrained = false;
-  g->any_cache = g->any_tets = g->any_stage = false;
+  g->any_cache = g->any_tets = g->stage_rule = false;
   for (int k = 0; k < ns; k++) {
     g->any_unconstrained |= args[k].cp.ns > 0 && args[k].cp.iters == 0;
-    g->any_stage |= args[k].fp.stage > 0;
+    // staging pays where a lane records many pairs (2 lanes per particle: the
+    // 256-scene batch's density 5.65 -> 4.84 ms per frame); at 4 lanes it
+    // cost the large scene 2%
+    g->stage_rule |= args[k].fp.fl_log2 <= 1;
     g->any_cache |= args[k].sm.cache != 0;
     g->any_tets |= args[k].sm.tets != 0;
   }
@@ -351,7 +362,7 @@ int fp_group_prepare(fp_group* g, double h) {
   std::vector<int2> items[W_NUM];
   bool rebuild = g->dirty || g->versions.size() != g->scenes.size() || g->items_clanes != g->clanes;
   for (int kind = 0; kind < W_NUM; kind++) {
-    items[kind].push_back(make_int2(0, 0));
+    items[kind].push_back(make_int2(0, g->xcd_run));
     for (int k = 0; k < ns; k++) {
       const int nb = fp_scene_blocks(g->scenes[k], kind, g->clanes);
       for (int b = 0; b < nb; b++) items[kind].push_back(make_int2(k, b));
@@ -377,7 +388,7 @@ int fp_group_prepare(fp_group* g, double h) {
   if (g->tune_record && !g->tune_items_ok) {
     for (int c = 0; c < 4; c++) {
       g->tune_off[c] = (int)titems.size();
-      titems.push_back(make_int2(0, 0));
+      titems.push_back(make_int2(0, g->xcd_run));
       for (int k = 0; k < ns; k++) {
         const int nb = fp_scene_blocks(g->scenes[k], W_SOLID, kTuneLanes[c]);
         for (int b = 0; b < nb; b++) titems.push_back(make_int2(k, b));
@@ -546,10 +557,128 @@ int fp_group_tune_end(fp_group* g) {
   return FP_OK;
 }
 
+// ---- launch-shape tuning (batches) -----------------------------------------------
+//
+// Two launch shapes leave every result unchanged: whether the density kernel
+// stages its pair-list entries in LDS, and the XCD run length of the group
+// kernels' work lists (their W[0].y).  A batch times them over whole
+// captured frames -- the solid chain beside the fluid chain, as in the frames
+// it will run -- once its contact lanes are tuned and while its members stay
+// the same: each candidate runs FP_SHAPE_REPS timed frames (the first
+// candidate of each staging after one untimed frame, which captures its
+// graph).  The fastest mean is kept if it beats the default (staging by the
+// 2-lane rule, runs of FP_XCD_CHUNK) by the factor FP_SHAPE_GAIN, else the
+// default, and the choice is cached for later groups of the same signature
+// (device, scene count and slot count to the next power of two, the 2-lane
+// rule, contact lanes), so a serving loop's changing batches do not tune again.
+#define FP_SHAPE_CANDS 6
+#ifndef FP_SHAPE_REPS
+#define FP_SHAPE_REPS 2
+#endif
+#ifndef FP_SHAPE_GAIN
+#define FP_SHAPE_GAIN 0.98
+#endif
+static const int kShapeRun[3] = {FP_XCD_CHUNK, 4, 1};  // candidate c: run kShapeRun[c % 3], staging flipped for c >= 3
+static std::mutex g_shape_mu;
+static std::map<uint64_t, int> g_shape_cache;
+
+static uint64_t shape_signature(const fp_group* g) {
+  uint64_t slots = 0;
+  for (const fp_solver* s : g->scenes) slots += (uint64_t)s->n;
+  const uint64_t a = 64 - __builtin_clzll((uint64_t)g->scenes.size() | 1u);
+  const uint64_t b = 64 - __builtin_clzll(slots | 1u);
+  return a | (b << 8) | ((uint64_t)g->stage_rule << 16) | ((uint64_t)(g->clanes & 63) << 17) |
+         ((uint64_t)(g->device & 255) << 24);
+}
+
+// The shape of candidate c: the group's staging flag and, if it changed, the
+// run length in every work list's header.
+static int shape_apply(fp_group* g, int c) {
+  g->stage_on = (c < 3) ? (g->stage_rule ? 1 : 0) : (g->stage_rule ? 0 : 1);
+  const int run = kShapeRun[c % 3];
+  if (run == g->xcd_run) return FP_OK;
+  g->xcd_run = run;
+  // (a pageable source is copied out before the call returns)
+  for (int k = 0; k < W_NUM; k++)
+    if (g->d_items[k])
+      FP_GTRY(g, hipMemcpyAsync(&g->d_items[k][0].y, &g->xcd_run, sizeof(int), hipMemcpyHostToDevice, g->stream));
+  if (g->d_tune_items && g->tune_items_ok)
+    for (int k = 0; k < 4; k++)
+      FP_GTRY(g, hipMemcpyAsync(&g->d_tune_items[g->tune_off[k]].y, &g->xcd_run, sizeof(int), hipMemcpyHostToDevice,
+                                g->stream));
+  return FP_OK;
+}
+
+// Before a frame: the shape it runs with; timed: it is a tuning frame.
+static int shape_begin(fp_group* g, bool graphable, bool& timed) {
+  timed = false;
+  int c = 0;
+  if (g->shape_tune) {
+    const uint64_t sig = shape_signature(g);
+    if (sig != g->shape_sig || (g->members_changed && g->shape_state == 1)) {
+      g->shape_sig = sig;
+      int cached = -1;
+      {
+        std::lock_guard<std::mutex> lk(g_shape_mu);
+        auto it = g_shape_cache.find(sig);
+        if (it != g_shape_cache.end()) cached = it->second;
+      }
+      g->shape_state = cached >= 0 ? 2 : 0;
+      g->shape_cand = cached >= 0 ? cached : 0;
+    }
+    if (g->shape_state == 0 && graphable && g->clanes_best && !g->members_changed) {
+      g->shape_state = 1;
+      g->shape_cand = 0;
+      g->shape_done = 0;
+      for (int k = 0; k < FP_SHAPE_CANDS; k++) {
+        g->shape_ms[k] = 0.0;
+        g->shape_n[k] = 0;
+      }
+    }
+    if (g->shape_state == 1 && graphable) {
+      c = g->shape_cand;
+      if (g->shape_done == 0 && c % 3 == 0) {
+        g->shape_done = 1;  // untimed: captures the candidate's graph
+      } else {
+        timed = true;
+      }
+    } else if (g->shape_state == 2) {
+      c = g->shape_cand;
+    }
+  }
+  return shape_apply(g, c);
+}
+
+// After a tuning frame has completed (read at the next frame): its time.
+static int shape_end(fp_group* g) {
+  if (!g->shape_pending) return FP_OK;
+  g->shape_pending = false;
+  float ms = 0.f;
+  FP_GTRY(g, hipEventSynchronize(g->shape_ev[1]));
+  FP_GTRY(g, hipEventElapsedTime(&ms, g->shape_ev[0], g->shape_ev[1]));
+  const int c = g->shape_pending_cand;
+  if (g->shape_state != 1 || c != g->shape_cand) return FP_OK;
+  g->shape_ms[c] += ms;
+  if (++g->shape_n[c] < FP_SHAPE_REPS) return FP_OK;
+  g->shape_cand++;
+  g->shape_done = 0;
+  if (g->shape_cand < FP_SHAPE_CANDS) return FP_OK;
+  int best = 0;
+  for (int k = 1; k < FP_SHAPE_CANDS; k++)
+    if (g->shape_ms[k] / g->shape_n[k] < g->shape_ms[best] / g->shape_n[best]) best = k;
+  if (g->shape_ms[best] / g->shape_n[best] > FP_SHAPE_GAIN * g->shape_ms[0] / g->shape_n[0]) best = 0;
+  g->shape_state = 2;
+  g->shape_cand = best;
+  std::lock_guard<std::mutex> lk(g_shape_mu);
+  g_shape_cache[g->shape_sig] = best;
+  return FP_OK;
+}
+
 int fp_group_frame(fp_group* g, double dt, int substeps) {
   const double h = dt / substeps;
   int rc = fp_group_tune_end(g);
   if (rc) return rc;
+  if ((rc = shape_end(g))) return rc;
   fp_group_tune_begin(g);
   rc = fp_group_prepare(g, h);
   if (rc) return rc;
@@ -558,6 +687,8 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
   // graph runs eagerly instead
   const bool graphable = g->use_graph && !g->any_grabs && !profiling && !g->tune_record &&
                          !(FP_GRAPH_DEFER && g->members_changed && !g->gexec);
+  bool shape_timed = false;
+  if ((rc = shape_begin(g, graphable, shape_timed))) return rc;
   if (graphable) {
     // the captured frame replays whatever the scene table and the work lists
     // hold; it is captured again when its structure no longer fits: the
@@ -565,7 +696,7 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
     // template), the bounds pass of unconstrained scenes, the optional solid
     // modes' passes, the density kernel (staging or not), or work lists that
     // outgrew (or shrank far below) the grids captured for them
-    const int extra = (g->any_cache ? 1 : 0) | (g->any_tets ? 2 : 0) | (g->any_stage ? 4 : 0);
+    const int extra = (g->any_cache ? 1 : 0) | (g->any_tets ? 2 : 0) | (g->stage_on ? 4 : 0);
     bool fits = g->gexec && g->graph_sub == substeps && g->graph_iters == g->iters &&
                 g->graph_clanes == g->clanes && g->graph_unconstrained == g->any_unconstrained &&
                 g->graph_extra == extra;
@@ -603,7 +734,19 @@ int fp_group_frame(fp_group* g, double dt, int substeps) {
       g->graph_unconstrained = g->any_unconstrained;
       g->graph_extra = extra;
     }
+    if (shape_timed) {
+      if (!g->shape_ev[0]) {
+        FP_GTRY(g, hipEventCreate(&g->shape_ev[0]));
+        FP_GTRY(g, hipEventCreate(&g->shape_ev[1]));
+      }
+      FP_GTRY(g, hipEventRecord(g->shape_ev[0], g->stream));
+    }
     FP_GTRY(g, hipGraphLaunch(g->gexec, g->stream));
+    if (shape_timed) {
+      FP_GTRY(g, hipEventRecord(g->shape_ev[1], g->stream));
+      g->shape_pending = true;
+      g->shape_pending_cand = g->shape_cand;
+    }
     g->graph_launches++;
   } else {
     for (int k = 0; k < substeps; k++) {

@@ -353,6 +353,12 @@ typedef struct fp_batch_info {
                                   scene table and the work lists on the device); it is captured
                                   again only when the substep or iteration count, the contact
                                   lanes, or the work lists' size (by more than 1.5x) change */
+  int32_t density_stage;       /* 1: the density pass stages its pair-list entries in LDS */
+  int32_t xcd_run;             /* work-list items dealt to each XCD in turn */
+  int32_t shape_tuned;         /* 1 once the batch has chosen these two launch shapes: timed
+                                  over captured frames after the contact lanes (or taken from
+                                  an earlier batch of the same signature).  Results do not
+                                  depend on them. */
 } fp_batch_info;
 
 int fp_batch_create(int32_t device, fp_batch** out);

@@ -360,3 +360,44 @@ def test_captured_frame_survives_membership_changes():
     assert inf.graph_launches == frames - 1
     assert inf.graph_captures <= 2
     b.close()
+
+
+def test_launch_shape_tuning_keeps_results():
+    """The batch's launch-shape tuner runs every candidate (the density
+    kernel's LDS staging on and off, XCD runs of 16, 4 and 1 items) over
+    captured frames: every frame stays bit-identical to the scenes stepped
+    alone, the batch settles on one shape, and a second batch of the same
+    signature takes it from the cache without timing frames."""
+    from fluidphysics_amd import Batch, from_scene
+    frames = 24
+    scs = _scenes([4, 9, 10, 11], frames)
+    log("case launch-shape tuning: %s" % [sc.name for sc in scs])
+    ref = [_alone(sc, frames) for sc in scs]
+    b = Batch()
+    ss = [from_scene(sc) for sc in scs]
+    for s in ss:
+        b.add(s)
+    seen = set()
+    for f in range(frames):
+        for s, sc in zip(ss, scs):
+            _cuts(s, sc, f)
+        b.step(DT, SUB)
+        inf = b.info()
+        seen.add((inf.density_stage, inf.xcd_run))
+        for k, s in enumerate(ss):
+            _same(_snap(s), ref[k][f], "%s frame %d" % (scs[k].name, f))
+    inf = b.info()
+    log("  shapes run %s; settled on staging %d, XCD run %d" % (sorted(seen), inf.density_stage, inf.xcd_run))
+    assert inf.shape_tuned == 1
+    assert len(seen) == 6
+    choice = (inf.density_stage, inf.xcd_run)
+    b.close()
+    b2 = Batch()
+    ss2 = [from_scene(sc) for sc in scs]
+    for s in ss2:
+        b2.add(s)
+    for f in range(TUNE_FRAMES + 2):
+        b2.step(DT, SUB)
+    inf2 = b2.info()
+    assert inf2.shape_tuned == 1 and (inf2.density_stage, inf2.xcd_run) == choice
+    b2.close()
