@@ -591,21 +591,22 @@ static uint64_t shape_signature(const fp_group* g) {
          ((uint64_t)(g->device & 255) << 24);
 }
 
+__global__ void k_set_run(int2* __restrict__ w, int run) { w[0].y = run; }
+
 // The shape of candidate c: the group's staging flag and, if it changed, the
-// run length in every work list's header.
+// run length in every work list's header (written on the group's stream, in
+// order before the next frame's kernels read it).
 static int shape_apply(fp_group* g, int c) {
   g->stage_on = (c < 3) ? (g->stage_rule ? 1 : 0) : (g->stage_rule ? 0 : 1);
   const int run = kShapeRun[c % 3];
   if (run == g->xcd_run) return FP_OK;
   g->xcd_run = run;
-  // (a pageable source is copied out before the call returns)
   for (int k = 0; k < W_NUM; k++)
-    if (g->d_items[k])
-      FP_GTRY(g, hipMemcpyAsync(&g->d_items[k][0].y, &g->xcd_run, sizeof(int), hipMemcpyHostToDevice, g->stream));
+    if (g->d_items[k]) hipLaunchKernelGGL(k_set_run, dim3(1), dim3(1), 0, g->stream, g->d_items[k], run);
   if (g->d_tune_items && g->tune_items_ok)
     for (int k = 0; k < 4; k++)
-      FP_GTRY(g, hipMemcpyAsync(&g->d_tune_items[g->tune_off[k]].y, &g->xcd_run, sizeof(int), hipMemcpyHostToDevice,
-                                g->stream));
+      hipLaunchKernelGGL(k_set_run, dim3(1), dim3(1), 0, g->stream, g->d_tune_items + g->tune_off[k], run);
+  FP_GTRY(g, hipGetLastError());
   return FP_OK;
 }
 
@@ -625,6 +626,7 @@ static int shape_begin(fp_group* g, bool graphable, bool& timed) {
       }
       g->shape_state = cached >= 0 ? 2 : 0;
       g->shape_cand = cached >= 0 ? cached : 0;
+      g->shape_pending = false;  // a frame timed for the old members does not count
     }
     if (g->shape_state == 0 && graphable && g->clanes_best && !g->members_changed) {
       g->shape_state = 1;
