@@ -223,6 +223,14 @@ class Solver:
         self._check(self._lib.fp_fluid_count(self._h, ctypes.byref(c)))
         return c.value
 
+    def nonfinite(self):
+        """True once a finished substep ended a particle with a non-finite
+        position or velocity (fp_nonfinite; up to date after a sync, sticky
+        until restore() or import_state())."""
+        c = ctypes.c_int32()
+        self._check(self._lib.fp_nonfinite(self._h, ctypes.byref(c)))
+        return bool(c.value)
+
     def step(self, dt, substeps=5, wait=True):
         """src/solver.js:187-242.  wait=False leaves the final wait to sync()."""
         self._flush()

@@ -104,6 +104,8 @@ class FpInfo(ctypes.Structure):
         ("substeps_done", ctypes.c_int64),
         ("device_bytes", ctypes.c_int64),
         ("pair_overflow", ctypes.c_int64),
+        ("nonfinite", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -164,6 +166,7 @@ SIGNATURES = {
     "fp_select_solids": (_I, [_P, _DP, _D, _I, _IP, _I, _IP]),
     "fp_components": (_I, [_P, _IP, _IP]),
     "fp_fluid_count": (_I, [_P, _IP]),
+    "fp_nonfinite": (_I, [_P, _IP]),
     "fp_read_state": (_I, [_P, _FP, _FP, _FP, _FP, _U8P, _IP]),
     "fp_write_state": (_I, [_P, _FP, _FP, _FP, _FP, _U8P]),
     "fp_read_constraints": (_I, [_P, _IP, _IP, _FP, _U8P]),

@@ -403,6 +403,17 @@ napi_value FluidCount(napi_env env, napi_callback_info info) {
   return make_int(env, c);
 }
 
+napi_value NonFinite(napi_env env, napi_callback_info info) {
+  Args a;
+  NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
+  fp_solver* s = handle(env, a.v[0]);
+  int32_t c = 0;
+  FP_CALL(env, s, fp_nonfinite(s, &c), "fp_nonfinite");
+  napi_value out;
+  NAPI_CALL(env, napi_get_boolean(env, c != 0, &out));
+  return out;
+}
+
 napi_value ResetSolid(napi_env env, napi_callback_info info) {
   Args a;
   NAPI_CALL(env, napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr));
@@ -701,6 +712,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"addSolidCube", AddSolidCube}, {"emit", Emit},           {"step", Step},
       {"sync", Sync},               {"setGrabs", SetGrabs},     {"cut", Cut},
       {"pickSolid", PickSolid},     {"fluidCount", FluidCount}, {"resetSolid", ResetSolid},
+      {"nonFinite", NonFinite},
       {"selectSolids", SelectSolids}, {"components", Components},
       {"clearFluid", ClearFluid},   {"readState", ReadState},   {"writeState", WriteState},
       {"readConstraints", ReadConstraints}, {"readGrid", ReadGrid}, {"readTets", ReadTets}, {"buildGrid", BuildGrid},

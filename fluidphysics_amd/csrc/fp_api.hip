@@ -369,6 +369,7 @@ int defer_groups(fp_solver* s) {
 int settle(fp_solver* s) {
   FP_TRY(s, hipStreamSynchronize(s->stream));
   if (s->prof.on) profile_collect(&s->prof);
+  if (s->steps_pending && s->tear_host[1]) s->nonfinite = true;  // sticky: word 1 is never cleared by a step
   if (s->steps_pending && s->C > 0) {
     volatile int* tear = s->tear_host;
     if (*tear) {
@@ -486,7 +487,8 @@ int fp_create(const fp_params* params, int32_t max_fluid, int32_t device, fp_sol
   if (e == hipSuccess)
     e = hipHostMalloc((void**)&s->tear_host, 64, hipHostMallocMapped | hipHostMallocCoherent);
   if (e == hipSuccess) {
-    *s->tear_host = 0;
+    // every word (pinned blocks are recycled: a freed solver's words would remain)
+    std::memset(s->tear_host, 0, 64);
     e = hipHostGetDevicePointer((void**)&s->tear_dev, s->tear_host, 0);
   }
   if (e != hipSuccess) {
@@ -564,6 +566,8 @@ int fp_get_info(const fp_solver* s, fp_info* out) {
   out->device_bytes = s->device_bytes;
   out->pair_overflow = 0;
   out->pair_slots = s->pair_slots;
+  out->nonfinite = s->nonfinite ? 1 : 0;
+  out->reserved = 0;
   if (s->dflags) {
     (void)hipSetDevice(s->device);
     int ov = 0;
@@ -1611,6 +1615,16 @@ int fp_checkpoint_save(fp_solver* s, int32_t slot) {
   return FP_OK;
 }
 
+// A restored or imported state starts with the non-finite flag down (the
+// stream is idle after settle: no kernel writes either word now).
+static int clear_nonfinite(fp_solver* s) {
+  FP_TRY(s, hipMemsetAsync(s->dflags + DF_NONFINITE, 0, sizeof(int), s->stream));
+  FP_TRY(s, hipStreamSynchronize(s->stream));
+  s->tear_host[1] = 0;
+  s->nonfinite = false;
+  return FP_OK;
+}
+
 int fp_checkpoint_restore(fp_solver* s, int32_t slot) {
   if (!s) return fp_fail(nullptr, FP_EINVAL, "fp_checkpoint_restore: NULL solver");
   if (slot < 0 || slot >= (int)s->checkpoints.size() || !s->checkpoints[slot].block)
@@ -1635,7 +1649,7 @@ int fp_checkpoint_restore(fp_solver* s, int32_t slot) {
   s->groups_dirty = false;
   s->cut_dirty = cp.cut_dirty;
   s->alive_changed = true;  // conservative: the host labels decide selection
-  return FP_OK;
+  return clear_nonfinite(s);
 }
 
 int fp_state_size(fp_solver* s, int64_t* bytes) {
@@ -1723,5 +1737,11 @@ int fp_state_import(fp_solver* s, const void* buf, int64_t bytes) {
   s->groups_dirty = false;
   s->cut_dirty = (h.flags & 2u) != 0;
   s->alive_changed = true;  // conservative: the host labels decide selection
+  return clear_nonfinite(s);
+}
+
+int fp_nonfinite(const fp_solver* s, int32_t* out) {
+  if (!s || !out) return fp_fail(const_cast<fp_solver*>(s), FP_EINVAL, "fp_nonfinite: NULL argument");
+  *out = s->nonfinite ? 1 : 0;
   return FP_OK;
 }

@@ -54,6 +54,17 @@ __device__ __forceinline__ float4 substep_velocity(const BoundsParams& bp, bool 
   return make_float4(vx, vy, vz, 0.f);
 }
 
+// The scene's non-finite flag (the reference counts NaN positions in its
+// metrics, src/gpu/main.js:546-556; NaNs otherwise propagate as there):
+// raised, and kept, when a particle ends a substep with a non-finite
+// position or velocity.  Every lane that raises it stores the same word.
+__device__ __forceinline__ bool finite3(float x, float y, float z) {
+  return __builtin_isfinite(x) & __builtin_isfinite(y) & __builtin_isfinite(z);
+}
+__device__ __forceinline__ void mark_nonfinite(const SceneArgs& A, bool ok) {
+  if (!ok) gst(A.dflags + DF_NONFINITE, 1);
+}
+
 // Solid i whose position after the constraints is p: pos and vel (an
 // inactive solid only moves: the reference's constraint sweep moves it too).
 __device__ __forceinline__ void solid_finish(const SceneArgs& A, int i, float4 p) {
@@ -65,7 +76,9 @@ __device__ __forceinline__ void solid_finish(const SceneArgs& A, int i, float4 p
   }
   bounds_apply(A.bp, true, p, pv);
   gst(A.pos + i, p);
-  gst(A.vel + i, substep_velocity(A.bp, true, p, pv));
+  const float4 v = substep_velocity(A.bp, true, p, pv);
+  gst(A.vel + i, v);
+  mark_nonfinite(A, finite3(p.x, p.y, p.z) & finite3(v.x, v.y, v.z));
 }
 
 // Active fluid particle i, in fluid slot fs, whose position after the
@@ -82,5 +95,7 @@ __device__ __forceinline__ void fluid_finish(const SceneArgs& A, int i, int fs, 
     gst(A.svtmp + fs, gld(A.vel + i));
     return;
   }
-  gst(A.svtmp + fs, substep_velocity(bp, false, p, pv));
+  const float4 v = substep_velocity(bp, false, p, pv);
+  gst(A.svtmp + fs, v);
+  mark_nonfinite(A, finite3(p.x, p.y, p.z) & finite3(v.x, v.y, v.z));
 }

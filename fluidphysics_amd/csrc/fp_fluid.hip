@@ -959,6 +959,10 @@ __device__ __forceinline__ void xsph_body(const SceneArgs& A, int lb) {
   // the solids' bounding box is rebuilt by the next integrate; both contact
   // passes, its readers, are done
   if (lb == 0 && threadIdx.x < 6) gst(A.sbox + threadIdx.x, threadIdx.x < 3 ? INT_MAX : INT_MIN);
+  // every particle has ended the substep (the displacement and the last
+  // constraint iteration ran before this pass): a raised non-finite flag
+  // goes to the scene's host-mapped word, which the host reads after a sync
+  if (lb == 0 && threadIdx.x == 0 && gld(A.dflags + DF_NONFINITE)) A.tear[1] = 1;
   PairSelf me;
   if (!pair_self(fp, src, fluid_lane(lb), me)) return;
   if (!gld(active + me.x())) return;

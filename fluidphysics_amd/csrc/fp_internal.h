@@ -82,6 +82,7 @@
 #define DF_PAIR_OVF 3    // particle-substeps whose pair list overflowed
 #define DF_NCOMP 4       // components counted by fp_launch_components
 #define DF_CACHE_OVF 5   // solid contact partners beyond FP_CACHE_K (dropped, cumulative)
+#define DF_NONFINITE 6   // a substep ended a particle with a non-finite position or velocity (sticky)
 #define DF_WORDS 8
 
 enum PassId {
@@ -235,12 +236,14 @@ struct fp_solver {
   bool alive_changed = false;
   Prof prof;
   hipStream_t own_stream = nullptr;  // the stream fp_create made (stream may be a batch's)
-  // "a constraint tore" flag in pinned host memory, set by the constraint
-  // kernel through tear_dev and read/cleared by the host after a stream sync
+  // "a constraint tore" flag (word 0) and the forwarded non-finite flag
+  // (word 1) in pinned host memory, set by kernels through tear_dev and read
+  // by the host after a stream sync
   int* tear_host = nullptr;
   int* tear_dev = nullptr;
   void* arena = nullptr;  // one pool block holding every fixed buffer of the scene
   int fluid_lanes = FLUID_LANES_SMALL;  // lanes per fluid particle (resolved from fp_params)
+  bool nonfinite = false;            // a finished substep ended a particle non-finite (sticky)
   fp_group* group = nullptr;         // this scene alone, for fp_step
   fp_batch* batch = nullptr;         // the batch it belongs to, if any
   uint64_t version = 1;              // bumped when its parameters change

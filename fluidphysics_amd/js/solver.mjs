@@ -117,6 +117,10 @@ export class Solver {
   clearFluid() { this._flush(); addon.clearFluid(this._h); this._stale(); }
 
   fluidCount() { this._flush(); return addon.fluidCount(this._h); }
+  // true once a finished step ended a particle with a non-finite position or
+  // velocity (fp_nonfinite; sticky); the reference counts NaNs in its GPU
+  // page's metrics (src/gpu/main.js:546-556)
+  nonFinite() { return addon.nonFinite(this._h); }
 
   _pushGrabs() {
     const g = this.grabs;

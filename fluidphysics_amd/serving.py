@@ -20,6 +20,13 @@ Host work of a frame (admission, scene set-up, slices) overlaps the device
 work of the previous one: the frame is queued, the host prepares the next,
 then waits for the snapshot.
 
+Failures and cancellation, at frame boundaries: a scene whose frame ended a
+particle with a non-finite position or velocity (the solver's non-finite
+flag, raised on the device and read from host memory) is retired as failed
+after that frame is delivered; cancel(request_id) retires a running request
+(or drops a waiting one) at the next boundary.  The other scenes step on
+unchanged: a scene's arithmetic does not depend on the batch around it.
+
 The reference steps one scene in the browser (src/main.js:396 calls
 solver.step once per animation frame); this is its many-scene form.
 """
@@ -33,12 +40,14 @@ from . import Batch, from_scene, scene_bytes
 
 @dataclass
 class Served:
-    """Per-request record: admission time, frames delivered, completion."""
+    """Per-request record: admission time, frames delivered, completion
+    (status "done", "failed" or "cancelled")."""
     request: object
     admitted: float = None
     frames: int = 0
     done: float = None
     bytes: int = 0
+    status: str = None
 
 
 @dataclass
@@ -50,6 +59,8 @@ class ServeReport:
     queue_ms: list = field(default_factory=list)   # per admitted request: arrival -> admission
     particle_substeps: int = 0
     completed: int = 0
+    failed: int = 0                                # retired after a non-finite frame
+    cancelled: int = 0                             # retired or dropped by cancel()
     admitted: int = 0
     arrived: int = 0
     refused: int = 0
@@ -78,6 +89,8 @@ class ServeReport:
             "arrived": self.arrived,
             "admitted": self.admitted,
             "completed": self.completed,
+            "failed": self.failed,
+            "cancelled": self.cancelled,
             "refused": self.refused,
             "peak_scenes": self.peak_scenes,
             "peak_bytes": self.peak_bytes,
@@ -94,10 +107,13 @@ class Server:
 
     on_frame(request, frame_index, positions) is called for every delivered
     frame; `positions` is a (capacity, 3) float32 view that is reused two
-    frames later (copy what must be kept)."""
+    frames later (copy what must be kept).  on_admit(request, solver) is
+    called once a request's solver is set up, before its first frame;
+    on_retire(request, status, frames) when it leaves ("done", "failed",
+    "cancelled")."""
 
     def __init__(self, budget_bytes=200e9, mode="batch", device=-1, dt=1.0 / 60.0, substeps=5,
-                 on_frame=None, use_graph=True):
+                 on_frame=None, use_graph=True, on_admit=None, on_retire=None):
         if mode not in ("batch", "per_scene"):
             raise ValueError("mode must be 'batch' or 'per_scene'")
         self.budget = int(budget_bytes)
@@ -107,6 +123,14 @@ class Server:
         self.substeps = substeps
         self.on_frame = on_frame
         self.use_graph = use_graph
+        self.on_admit = on_admit
+        self.on_retire = on_retire
+        self._cancel = set()
+
+    def cancel(self, request_id):
+        """Retire request `request_id` at the next frame boundary (a waiting
+        one is dropped before admission)."""
+        self._cancel.add(request_id)
 
     def _admit(self, waiting, used, now, by_frame, report):
         """Head-of-line admission in arrival order within the byte budget.
@@ -114,6 +138,13 @@ class Server:
         out = []
         while waiting:
             req, sc, need = waiting[0]
+            if req.id in self._cancel:
+                waiting.popleft()
+                self._cancel.discard(req.id)
+                report.cancelled += 1
+                if self.on_retire is not None:
+                    self.on_retire(req, "cancelled", 0)
+                continue
             if used + need > self.budget:
                 if need > self.budget:
                     waiting.popleft()  # can never fit: refuse it rather than stall the queue
@@ -122,6 +153,8 @@ class Server:
                 break
             waiting.popleft()
             s = from_scene(sc, device=self.device)
+            if self.on_admit is not None:
+                self.on_admit(req, s)
             rec = Served(request=req, admitted=now, bytes=need)
             report.queue_ms.append(0.0 if by_frame else max(0.0, now - req.arrival) * 1e3)
             out.append((rec, sc, s))
@@ -171,14 +204,25 @@ class Server:
                     for m in active:
                         rec, sc, s = m
                         if rec.frames >= rec.request.horizon:
-                            rec.done = clock() - t0
+                            rec.status = "done"
                             report.completed += 1
-                            if batch is not None:
-                                batch.remove(s)
-                            used -= rec.bytes
-                            s.close()
+                        elif s.nonfinite():
+                            rec.status = "failed"
+                            report.failed += 1
+                        elif rec.request.id in self._cancel:
+                            self._cancel.discard(rec.request.id)
+                            rec.status = "cancelled"
+                            report.cancelled += 1
                         else:
                             keep.append(m)
+                            continue
+                        rec.done = clock() - t0
+                        if self.on_retire is not None:
+                            self.on_retire(rec.request, rec.status, rec.frames)
+                        if batch is not None:
+                            batch.remove(s)
+                        used -= rec.bytes
+                        s.close()
                     active = keep
                 if stop:
                     break

@@ -116,6 +116,8 @@ typedef struct fp_info {
   int64_t device_bytes;        /* HBM bytes held by this solver */
   int64_t pair_overflow;       /* particle-substeps whose pairs did not fit pair_slots
                                   (cumulative; those were gathered again, results unchanged) */
+  int32_t nonfinite;           /* fp_nonfinite's flag */
+  int32_t reserved;
 } fp_info;
 
 /* ---- lifetime ------------------------------------------------------- */
@@ -215,6 +217,16 @@ int fp_pick_solid(fp_solver* s, const double o[3], const double d[3], double max
 
 /* fluidCount() (solver.js:155-159). */
 int fp_fluid_count(fp_solver* s, int32_t* count);
+
+/* The scene's non-finite flag: 1 once a finished substep has ended a
+ * particle with a non-finite position or velocity (the velocity pass
+ * raises it on the device; the host picks it up at the next sync, with no
+ * copy).  The reference lets NaNs propagate and counts NaN positions in
+ * the GPU page's metrics (src/gpu/main.js:546-556); the solver does the
+ * same arithmetic and reports the event, so a serving loop can retire the
+ * scene.  Sticky; fp_checkpoint_restore and fp_state_import clear it.
+ * Reads host memory only (cheap to call every frame). */
+int fp_nonfinite(const fp_solver* s, int32_t* out);
 
 /* ---- state access (the JS public typed arrays, solver.js:42-48) ------ */
 

@@ -49,7 +49,7 @@ function report(phase, frames) {
     if (!byLabel.has(l)) byLabel.set(l, q); else if (byLabel.get(l) !== q) same = false;
     if (!byGroup.has(q)) byGroup.set(q, l); else if (byGroup.get(q) !== l) same = false;
   }
-  console.log(JSON.stringify({ phase, frames, fluid: s.fluidCount(), alive, ke, minY, maxY, nan, comps: c.count, same }));
+  console.log(JSON.stringify({ phase, frames, fluid: s.fluidCount(), alive, ke, minY, maxY, nan, nonFinite: s.nonFinite(), comps: c.count, same }));
 }
 
 function run(n, each) { for (let k = 0; k < n; k++) { if (each) each(k); s.step(1 / 60, 5); frame++; } }

@@ -117,7 +117,7 @@ def test_headless_drive_timeline():
                           for p in phases))
     by = {p["phase"]: p for p in phases}
     assert [p["phase"] for p in phases] == ["settle", "slice", "grab", "hose"]
-    assert all(p["nan"] == 0 for p in phases)
+    assert all(p["nan"] == 0 and p["nonFinite"] is False for p in phases)
     assert by["slice"]["alive"] < by["settle"]["alive"]      # the strokes cut constraints
     assert by["hose"]["fluid"] > 0                            # the hose emitted water
     assert by["settle"]["minY"] >= 0.09 - 1e-3                # the cube rests on the floor
