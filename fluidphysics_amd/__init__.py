@@ -18,6 +18,7 @@ batch before the next operation that needs them.  All solving happens in the
 HIP kernels of libfluidphysics.so; there is no CPU fallback.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -570,6 +571,30 @@ def pool_trim(device=-1):
     _abi.load_library().fp_pool_trim(int(device))
 
 
+def _last_error():
+    lib = _abi.load_library()
+    return lib.fp_last_error(None).decode(errors="replace")
+
+
+def tune_cache_save(path):
+    """Write the batches' tuning choices of this process (contact lanes and
+    launch shape per group signature) to a JSON file (fp_tune_cache_save)."""
+    if _abi.load_library().fp_tune_cache_save(os.fsencode(path)) != 0:
+        raise FpError(_last_error())
+
+
+def tune_cache_load(path):
+    """Merge a tuning cache file (fp_tune_cache_load): later batches of a
+    cached signature skip their tuning frames."""
+    if _abi.load_library().fp_tune_cache_load(os.fsencode(path)) != 0:
+        raise FpError(_last_error())
+
+
+def tune_cache_reset():
+    """Forget every tuning choice and stop using loaded contact lanes."""
+    _abi.load_library().fp_tune_cache_reset()
+
+
 def from_scene(scene, device=-1, **param_overrides):
     """A solver set up with a scene's cube and initial fluid (fluidphysics_amd.scenes)."""
     s = Solver(scene.max_fluid, params=scene.params_with(**param_overrides), device=device)
@@ -580,4 +605,4 @@ def from_scene(scene, device=-1, **param_overrides):
 
 
 __all__ = ["Solver", "Batch", "FLUID", "SOLID", "FpError", "FpParams", "make_params", "from_scene",
-           "scene_bytes", "pool_info", "pool_trim"]
+           "scene_bytes", "pool_info", "pool_trim", "tune_cache_save", "tune_cache_load", "tune_cache_reset"]

@@ -15,6 +15,7 @@ FP_EINVAL = -1
 FP_EDEVICE = -2
 FP_ENOMEM = -3
 FP_ESTATE = -4
+FP_EIO = -5
 
 
 class FpParams(ctypes.Structure):
@@ -201,6 +202,9 @@ SIGNATURES = {
     "fp_scene_bytes": (_I, [ctypes.POINTER(FpParams), _I, _I, ctypes.POINTER(ctypes.c_int64)]),
     "fp_pool_info": (_I, [_I, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "fp_pool_trim": (_I, [_I]),
+    "fp_tune_cache_save": (_I, [ctypes.c_char_p]),
+    "fp_tune_cache_load": (_I, [ctypes.c_char_p]),
+    "fp_tune_cache_reset": (_I, []),
 }
 
 _lib = None

@@ -5,6 +5,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -1535,6 +1536,140 @@ int fp_batch_snapshot(fp_batch* b, const float** pos, const int64_t** offsets, i
   *offsets = b->snap_off_out[slot].data();
   *scenes = ns;
   b->snap_slot ^= 1;
+  return FP_OK;
+}
+
+
+// ---- tuning cache file ----------------------------------------------------------
+//
+// {"version": 1, "contact_lanes": {"<signature>": lanes, ...},
+//  "launch_shape": {"<signature>": candidate, ...}}: the two tables of
+// fp_step.hip's tuning caches, signatures as decimal strings.
+
+namespace {
+
+void put_table(std::string& out, const char* name, const std::vector<std::pair<uint64_t, int>>& t, bool last) {
+  out += "  \"";
+  out += name;
+  out += "\": {";
+  for (size_t k = 0; k < t.size(); k++) {
+    out += k ? ", \"" : "\"";
+    out += std::to_string(t[k].first) + "\": " + std::to_string(t[k].second);
+  }
+  out += last ? "}\n" : "},\n";
+}
+
+// A reader of the format above: objects, decimal strings as keys, integers.
+struct JsonIn {
+  const char* p;
+  const char* end;
+  void ws() {
+    while (p < end && (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t')) p++;
+  }
+  bool lit(char c) {
+    ws();
+    if (p < end && *p == c) {
+      p++;
+      return true;
+    }
+    return false;
+  }
+  bool str(std::string& s) {
+    ws();
+    if (p >= end || *p != '"') return false;
+    const char* q = ++p;
+    while (p < end && *p != '"') {
+      if (*p == '\\') return false;  // no escapes in this format
+      p++;
+    }
+    if (p >= end) return false;
+    s.assign(q, p++);
+    return true;
+  }
+  bool integer(int64_t& v) {
+    ws();
+    const char* q = p;
+    if (p < end && *p == '-') p++;
+    while (p < end && *p >= '0' && *p <= '9') p++;
+    if (p == q || (p == q + 1 && *q == '-') || p - q > 19) return false;
+    v = std::strtoll(std::string(q, p).c_str(), nullptr, 10);
+    return true;
+  }
+  // {"<u64>": int, ...}
+  bool table(std::vector<std::pair<uint64_t, int>>& t) {
+    if (!lit('{')) return false;
+    if (lit('}')) return true;
+    do {
+      std::string k;
+      int64_t v;
+      if (!str(k) || k.empty() || k.size() > 20 || !lit(':') || !integer(v)) return false;
+      for (char c : k)
+        if (c < '0' || c > '9') return false;
+      t.emplace_back(std::strtoull(k.c_str(), nullptr, 10), (int)v);
+      if (v < INT_MIN || v > INT_MAX) return false;
+    } while (lit(','));
+    return lit('}');
+  }
+};
+
+}  // namespace
+
+int fp_tune_cache_save(const char* path) {
+  if (!path) return fp_fail(nullptr, FP_EINVAL, "fp_tune_cache_save: NULL path");
+  std::string out = "{\n  \"version\": 1,\n";
+  put_table(out, "contact_lanes", fp_tune_cache_entries(true), false);
+  put_table(out, "launch_shape", fp_tune_cache_entries(false), true);
+  out += "}\n";
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fp_fail(nullptr, FP_EIO, std::string("fp_tune_cache_save: cannot open ") + path);
+  const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+  if (std::fclose(f) != 0 || !ok) return fp_fail(nullptr, FP_EIO, std::string("fp_tune_cache_save: cannot write ") + path);
+  return FP_OK;
+}
+
+int fp_tune_cache_load(const char* path) {
+  if (!path) return fp_fail(nullptr, FP_EINVAL, "fp_tune_cache_load: NULL path");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fp_fail(nullptr, FP_EIO, std::string("fp_tune_cache_load: cannot open ") + path);
+  std::string text;
+  char buf[4096];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0 && text.size() < (1u << 24)) text.append(buf, n);
+  const bool rerr = std::ferror(f) != 0;
+  std::fclose(f);
+  if (rerr) return fp_fail(nullptr, FP_EIO, std::string("fp_tune_cache_load: cannot read ") + path);
+  JsonIn in{text.data(), text.data() + text.size()};
+  std::vector<std::pair<uint64_t, int>> lanes, shape;
+  bool ok = in.lit('{'), seen_v = false;
+  while (ok) {
+    std::string key;
+    ok = in.str(key) && in.lit(':');
+    if (!ok) break;
+    if (key == "version") {
+      int64_t v = 0;
+      ok = in.integer(v) && v == 1;
+      seen_v = true;
+    } else if (key == "contact_lanes") {
+      ok = in.table(lanes);
+    } else if (key == "launch_shape") {
+      ok = in.table(shape);
+    } else {
+      ok = false;
+    }
+    if (!ok || !in.lit(',')) break;
+  }
+  ok = ok && seen_v && in.lit('}');
+  in.ws();
+  ok = ok && in.p == in.end;
+  for (const auto& e : lanes) ok = ok && (e.second == 4 || e.second == 8 || e.second == 16 || e.second == 32);
+  for (const auto& e : shape) ok = ok && e.second >= 0 && e.second < 6;
+  if (!ok) return fp_fail(nullptr, FP_EINVAL, std::string("fp_tune_cache_load: not a tuning cache of version 1: ") + path);
+  fp_tune_cache_merge(lanes, shape);
+  return FP_OK;
+}
+
+int fp_tune_cache_reset(void) {
+  fp_tune_caches_clear();
   return FP_OK;
 }
 

@@ -492,6 +492,11 @@ struct fp_group {
 // ---- host helpers (fp_api.hip) ---------------------------------------------
 
 int fp_fail(fp_solver* s, int code, const std::string& msg);
+// the tuning caches of fp_step.hip (contact lanes, launch shape), by signature
+std::vector<std::pair<uint64_t, int>> fp_tune_cache_entries(bool lanes);
+void fp_tune_cache_merge(const std::vector<std::pair<uint64_t, int>>& lanes,
+                         const std::vector<std::pair<uint64_t, int>>& shape);
+void fp_tune_caches_clear();
 
 // ---- per-device memory pool (fp_pool.hip) -----------------------------------
 size_t fp_pool_class(size_t bytes);

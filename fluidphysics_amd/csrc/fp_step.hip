@@ -483,6 +483,44 @@ static int group_substep(fp_group* g) {
 // into a HIP graph once and replayed.
 // ---- contact-lane autotuning ---------------------------------------------------
 
+// Tuning results of this process, by group signature: the contact lanes
+// (consulted only once a cache file has been loaded: a fresh process tunes)
+// and the launch shape (always consulted).  fp_tune_cache_save / _load
+// (fp_api.hip) keep them in a JSON file between processes.
+static std::mutex g_tune_mu;
+static std::map<uint64_t, int> g_lanes_cache;
+static std::map<uint64_t, int> g_shape_cache;
+static bool g_lanes_cache_on = false;
+
+// The shape a contact-lane choice is kept for: device, scene count and
+// solid count to the next power of two.
+static uint64_t lanes_signature(const fp_group* g, int64_t solids) {
+  const uint64_t a = 64 - __builtin_clzll((uint64_t)g->scenes.size() | 1u);
+  const uint64_t b = 64 - __builtin_clzll((uint64_t)solids | 1u);
+  return a | (b << 8) | ((uint64_t)(g->device & 255) << 24) | (1ull << 40);
+}
+
+std::vector<std::pair<uint64_t, int>> fp_tune_cache_entries(bool lanes) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const std::map<uint64_t, int>& m = lanes ? g_lanes_cache : g_shape_cache;
+  return std::vector<std::pair<uint64_t, int>>(m.begin(), m.end());
+}
+
+void fp_tune_cache_merge(const std::vector<std::pair<uint64_t, int>>& lanes,
+                         const std::vector<std::pair<uint64_t, int>>& shape) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (const auto& e : lanes) g_lanes_cache[e.first] = e.second;
+  for (const auto& e : shape) g_shape_cache[e.first] = e.second;
+  g_lanes_cache_on = true;
+}
+
+void fp_tune_caches_clear() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_lanes_cache.clear();
+  g_shape_cache.clear();
+  g_lanes_cache_on = false;
+}
+
 // Before a frame: decide whether it tunes (the candidates' lane counts are
 // then picked per substep), else use the tuned lane count.
 void fp_group_tune_begin(fp_group* g) {
@@ -497,6 +535,15 @@ void fp_group_tune_begin(fp_group* g) {
     g->tuned_solids = 0;
   }
   g->tune_record = false;
+  if (!g->clanes_best && !g->clanes_pinned && g->tune_k == 0) {
+    // a choice loaded from a tuning cache file for a group of this shape
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_lanes_cache.find(lanes_signature(g, solids));
+    if (g_lanes_cache_on && it != g_lanes_cache.end()) {
+      g->clanes_best = it->second;
+      g->tuned_solids = solids;
+    }
+  }
   if (g->clanes_best) {
     g->clanes = g->clanes_best;
     return;
@@ -547,6 +594,10 @@ int fp_group_tune_end(fp_group* g) {
     }
     g->clanes_best = kTuneLanes[best < 0 ? 0 : best];
     g->tune_k = 0;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_lanes_cache[lanes_signature(g, g->tuned_solids)] = g->clanes_best;
+    }
     // the candidates' lists are idle: every launch that read them has been
     // waited for above
     if (g->d_tune_items) fp_pool_put(g->device, g->d_tune_items);
@@ -579,8 +630,6 @@ int fp_group_tune_end(fp_group* g) {
 #define FP_SHAPE_GAIN 0.98
 #endif
 static const int kShapeRun[3] = {FP_XCD_CHUNK, 4, 1};  // candidate c: run kShapeRun[c % 3], staging flipped for c >= 3
-static std::mutex g_shape_mu;
-static std::map<uint64_t, int> g_shape_cache;
 
 static uint64_t shape_signature(const fp_group* g) {
   uint64_t slots = 0;
@@ -620,7 +669,7 @@ static int shape_begin(fp_group* g, bool graphable, bool& timed) {
       g->shape_sig = sig;
       int cached = -1;
       {
-        std::lock_guard<std::mutex> lk(g_shape_mu);
+        std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_shape_cache.find(sig);
         if (it != g_shape_cache.end()) cached = it->second;
       }
@@ -671,7 +720,7 @@ static int shape_end(fp_group* g) {
   if (g->shape_ms[best] / g->shape_n[best] > FP_SHAPE_GAIN * g->shape_ms[0] / g->shape_n[0]) best = 0;
   g->shape_state = 2;
   g->shape_cand = best;
-  std::lock_guard<std::mutex> lk(g_shape_mu);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
   g_shape_cache[g->shape_sig] = best;
   return FP_OK;
 }

@@ -32,10 +32,11 @@ solver.step once per animation frame); this is its many-scene form.
 """
 import collections
 import math
+import os
 import time
 from dataclasses import dataclass, field
 
-from . import Batch, from_scene, scene_bytes
+from . import Batch, from_scene, scene_bytes, tune_cache_load, tune_cache_save
 
 
 @dataclass
@@ -113,7 +114,7 @@ class Server:
     "cancelled")."""
 
     def __init__(self, budget_bytes=200e9, mode="batch", device=-1, dt=1.0 / 60.0, substeps=5,
-                 on_frame=None, use_graph=True, on_admit=None, on_retire=None):
+                 on_frame=None, use_graph=True, on_admit=None, on_retire=None, tune_cache=None):
         if mode not in ("batch", "per_scene"):
             raise ValueError("mode must be 'batch' or 'per_scene'")
         self.budget = int(budget_bytes)
@@ -125,6 +126,7 @@ class Server:
         self.use_graph = use_graph
         self.on_admit = on_admit
         self.on_retire = on_retire
+        self.tune_cache = tune_cache  # a tuning cache file: loaded before run() if present, saved after
         self._cancel = set()
 
     def cancel(self, request_id):
@@ -167,6 +169,8 @@ class Server:
         seconds; by_frame=True: arrivals are loop-frame indices (a
         deterministic mix for tests) and the loop runs until every request
         completed or max_frames."""
+        if self.tune_cache and os.path.exists(self.tune_cache):
+            tune_cache_load(self.tune_cache)
         report = ServeReport(mode=self.mode, wall_s=0.0, frames=0, budget_bytes=self.budget)
         pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.id)))
         waiting = collections.deque()
@@ -261,6 +265,8 @@ class Server:
                 s.close()
             if batch is not None:
                 batch.close()
+            if self.tune_cache:
+                tune_cache_save(self.tune_cache)
         return report
 
     def _deliver(self, batch, members, report):

@@ -34,6 +34,7 @@ extern "C" {
 #define FP_EDEVICE (-2)    /* HIP runtime error */
 #define FP_ENOMEM (-3)     /* device or host allocation failed */
 #define FP_ESTATE (-4)     /* call not valid in the solver's current state */
+#define FP_EIO (-5)        /* a file could not be opened, read or written */
 
 /* Solver constants.  Defaults (fp_params_default) are the reference's
  * tuning fields, src/solver.js:14-37, plus cellSize (:50) and the
@@ -420,6 +421,27 @@ int fp_scene_bytes(const fp_params* p, int32_t max_fluid, int32_t cube_n, int64_
 int fp_pool_info(int32_t device, int64_t* reserved, int64_t* cached);
 /* Return the cached blocks to the runtime (waits for the device). */
 int fp_pool_trim(int32_t device);
+
+/* ---- tuning cache ---------------------------------------------------- */
+
+/* No counterpart in the reference (its WGSL has compile-time workgroup
+ * sizes).  Batches tune launch shapes that leave every result unchanged:
+ * the solid contact pass's lanes per solid over their first frames, then
+ * the density kernel's LDS staging and the work lists' XCD run length over
+ * whole captured frames.  The choices are kept per group signature (device,
+ * scene count, solid or slot count to the next power of two) for the
+ * process; these calls keep them in a JSON file between processes:
+ *   {"version": 1, "contact_lanes": {"<signature>": lanes, ...},
+ *    "launch_shape": {"<signature>": candidate, ...}}
+ * fp_tune_cache_load merges a file's entries, after which a new group of a
+ * cached signature uses the cached lanes without tuning frames (a process
+ * that loads no file always tunes its contact lanes).  Errors: FP_EIO for a
+ * file that cannot be opened or written, FP_EINVAL for a malformed file or
+ * an out-of-range value (nothing is merged then); fp_last_error(NULL) says
+ * which.  fp_tune_cache_reset empties both and turns the lane cache off. */
+int fp_tune_cache_save(const char* path);
+int fp_tune_cache_load(const char* path);
+int fp_tune_cache_reset(void);
 
 #ifdef __cplusplus
 }

@@ -2,6 +2,7 @@
 point include/fluidphysics.h declares; the ctypes host binds the same set.
 No compute calls (no device here)."""
 import ctypes
+import json
 import os
 import re
 import subprocess
@@ -122,3 +123,38 @@ def test_noalloc_grid_refuses_sizes_beyond_its_array_without_a_device():
     assert call(100, 1024, cs=0.0) == _abi.FP_EINVAL
     assert call(100, 1024, cs=float("nan")) == _abi.FP_EINVAL
     assert call(100, 1024, p=None) == _abi.FP_EINVAL
+
+
+def test_tuning_cache_file_round_trip_and_rejects(tmp_path):
+    """fp_tune_cache_save / _load / _reset need no device: an empty cache
+    round-trips, a written file loads back, and malformed files or
+    out-of-range values are refused without merging anything."""
+    import fluidphysics_amd as F
+    F.tune_cache_reset()
+    p = tmp_path / "tune.json"
+    F.tune_cache_save(str(p))
+    d = json.loads(p.read_text())
+    assert d == {"version": 1, "contact_lanes": {}, "launch_shape": {}}
+    good = {"version": 1, "contact_lanes": {"1099528577282": 8}, "launch_shape": {"16843266": 3}}
+    p.write_text(json.dumps(good))
+    F.tune_cache_load(str(p))
+    q = tmp_path / "back.json"
+    F.tune_cache_save(str(q))
+    assert json.loads(q.read_text()) == good
+    for bad in ['{"version": 2, "contact_lanes": {}, "launch_shape": {}}',
+                '{"version": 1, "contact_lanes": {"12": 7}}',
+                '{"version": 1, "launch_shape": {"12": 6}}',
+                '{"version": 1, "contact_lanes": {"x1": 8}}',
+                '{"version": 1, "other": {}}',
+                '{"version": 1} trailing',
+                'not json']:
+        p.write_text(bad)
+        with pytest.raises(F.FpError):
+            F.tune_cache_load(str(p))
+    with pytest.raises(F.FpError):
+        F.tune_cache_load(str(tmp_path / "missing.json"))
+    F.tune_cache_save(str(q))
+    assert json.loads(q.read_text()) == good   # nothing merged by the refused files
+    F.tune_cache_reset()
+    F.tune_cache_save(str(q))
+    assert json.loads(q.read_text())["contact_lanes"] == {}
