@@ -46,6 +46,7 @@
 #include "fp_internal.h"
 #include "fp_kernels.h"
 #include "fp_bounds.h"
+#include "fp_cons.h"
 
 #ifndef FLUID_BLOCK
 #define FLUID_BLOCK 256  // threads per block of the density, displacement and XSPH passes
@@ -308,7 +309,14 @@ __device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb, 
     ay += __shfl_xor(ay, off, L);
     az += __shfl_xor(az, off, L);
   }
-  if (valid && lane == 0) gst(pos + x, make_float4(px.x + ax, px.y + ay, px.z + az, px.w));
+  if (lane == 0 && x < fp.ns) {
+    // the result also goes to spos_c[1], the snapshot the first constraint
+    // iteration reads (a single iteration finishes its solids into pos in
+    // the same launch); an inactive solid's position is copied unchanged
+    const float4 q = valid ? make_float4(px.x + ax, px.y + ay, px.z + az, px.w) : gld(pos + x);
+    if (valid) gst(pos + x, q);
+    gst(A.spos_c[1] + x, q);
+  }
   __syncthreads();  // the block's next item reuses lds_bonds
 }
 
@@ -837,8 +845,9 @@ __device__ __forceinline__ int fluid_block(int j, int o, int a) {
   return first + ((j - ((x - c) & 7)) >> 3);
 }
 
-#define FLUID_ITEMS(S, W)                                                                        \
-  for (int w_ = (int)blockIdx.x, nw_ = items_count(W); w_ < nw_; w_ += (int)gridDim.x)             \
+#define FLUID_ITEMS(S, W) FLUID_ITEMS_AT(S, W, (int)blockIdx.x, (int)gridDim.x)
+#define FLUID_ITEMS_AT(S, W, b0, G)                                                              \
+  for (int w_ = (b0), nw_ = items_count(W); w_ < nw_; w_ += (G))                                  \
     if (const int2 it_ = (W)[1 + w_]; true)                                                          \
       if (const SceneArgs& A = (S)[it_.x]; true)                                                     \
         if (const int lb = fluid_block(it_.y, w_ - it_.y,                                            \
@@ -987,6 +996,55 @@ __global__ void __launch_bounds__(FLUID_BLOCK) WAVES_ATTR(XSPH_WAVES) kg_xsph(co
   FLUID_ITEMS(S, W) xsph_body(A, lb);
 }
 
+// ---- the solid chain in the fluid chain's launches ----------------------------
+//
+// After the cell grid, a substep's solid chain (solid contacts, then the
+// constraint iterations) and its fluid chain write disjoint data: each pass
+// writes its own particles' positions (and velocities, active flags), the
+// constraints their own CSR entries and ping-pong buffers; the only shared
+// words (the non-finite flag, a tear flag) take the same store from every
+// writer.  A group that does not fork runs the solid passes on workgroups of
+// their own inside the fluid passes' launches: the solid contacts beside the
+// fluid contacts, the constraint iterations beside the fluid rank, the density
+// and the displacement.  Each particle's arithmetic is unchanged, so the
+// results are the same bits as on separate launches; a small scene's
+// substep takes 12 launches instead of 16.  Solid workgroups come first.
+
+template <int L>
+__global__ void __launch_bounds__(BLOCK) kg_contacts_both(const SceneArgs* __restrict__ S,
+                                                          const int2* __restrict__ WS,
+                                                          const int2* __restrict__ WP, int g0) {
+  __shared__ int lds_bonds[(BLOCK / L) * MAXB];
+  if ((int)blockIdx.x < g0) {
+    FOR_ITEMS_AT(S, WS, (int)blockIdx.x, g0) contacts_solid_body<L>(A, lb, lds_bonds);
+  } else {
+    FOR_ITEMS_AT(S, WP, (int)blockIdx.x - g0, (int)gridDim.x - g0) contacts_fluid_body(A, lb);
+  }
+}
+
+template <bool kStage>
+__global__ void __launch_bounds__(FLUID_BLOCK) DENSITY_ATTR kg_density_cons(const SceneArgs* __restrict__ S,
+                                                                           const int2* __restrict__ WC,
+                                                                           const int2* __restrict__ WF, int g0,
+                                                                           int it) {
+  __shared__ uint32_t lds[kStage ? DENSITY_STAGE * FLUID_BLOCK : 1];
+  if ((int)blockIdx.x < g0) {
+    FOR_ITEMS_AT(S, WC, (int)blockIdx.x, g0) constraints_body(A, lb, it);
+  } else {
+    FLUID_ITEMS_AT(S, WF, (int)blockIdx.x - g0, (int)gridDim.x - g0) density_body<kStage>(A, lb, lds);
+  }
+}
+
+__global__ void __launch_bounds__(FLUID_BLOCK) WAVES_ATTR(DISPLACE_WAVES)
+    kg_displace_cons(const SceneArgs* __restrict__ S, const int2* __restrict__ WC, const int2* __restrict__ WF,
+                     int g0, int it) {
+  if ((int)blockIdx.x < g0) {
+    FOR_ITEMS_AT(S, WC, (int)blockIdx.x, g0) constraints_body(A, lb, it);
+  } else {
+    FLUID_ITEMS_AT(S, WF, (int)blockIdx.x - g0, (int)gridDim.x - g0) displace_body(A, lb);
+  }
+}
+
 // ---- launchers ------------------------------------------------------------------
 
 FluidParams fp_fluid_params(const fp_solver* s) {
@@ -1097,6 +1155,45 @@ int fp_group_contacts(fp_group* g) {
   int rc = fp_group_contacts_solid(g, g->stream);
   if (rc) return rc;
   return fp_group_contacts_fluid(g);
+}
+
+// The solid and the fluid contacts in one launch (the group's lane count).
+int fp_group_contacts_both(fp_group* g) {
+  PassScope ps(g->prof, g->stream, P_CONTACTS);
+  switch (g->clanes) {
+    case 32:
+      GROUP_LAUNCH2(g, kg_contacts_both<32>, W_SOLID, W_PARTICLE, BLOCK);
+      break;
+    case 16:
+      GROUP_LAUNCH2(g, kg_contacts_both<16>, W_SOLID, W_PARTICLE, BLOCK);
+      break;
+    case 8:
+      GROUP_LAUNCH2(g, kg_contacts_both<8>, W_SOLID, W_PARTICLE, BLOCK);
+      break;
+    default:
+      GROUP_LAUNCH2(g, kg_contacts_both<4>, W_SOLID, W_PARTICLE, BLOCK);
+  }
+  FP_GTRY(g, hipGetLastError());
+  return FP_OK;
+}
+
+int fp_group_density_with(fp_group* g, int it) {
+  if (it < 0) return fp_group_density(g);
+  PassScope ps(g->prof, g->stream, P_DENSITY);
+  if (g->stage_on)
+    GROUP_LAUNCH2(g, kg_density_cons<true>, W_CONS, W_FLUID, FLUID_BLOCK, it);
+  else
+    GROUP_LAUNCH2(g, kg_density_cons<false>, W_CONS, W_FLUID, FLUID_BLOCK, it);
+  FP_GTRY(g, hipGetLastError());
+  return FP_OK;
+}
+
+int fp_group_displace_with(fp_group* g, int it) {
+  if (it < 0) return fp_group_displace(g);
+  PassScope ps(g->prof, g->stream, P_DISPLACE);
+  GROUP_LAUNCH2(g, kg_displace_cons, W_CONS, W_FLUID, FLUID_BLOCK, it);
+  FP_GTRY(g, hipGetLastError());
+  return FP_OK;
 }
 
 int fp_group_density(fp_group* g) {

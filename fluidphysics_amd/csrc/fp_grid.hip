@@ -34,6 +34,7 @@
 
 #include "fp_internal.h"
 #include "fp_kernels.h"
+#include "fp_cons.h"
 
 // ---- scan (shared by both grids) ---------------------------------------------
 
@@ -328,6 +329,28 @@ __device__ __forceinline__ void fluid_rank_body(const SceneArgs& A, int lb) {
 
 __global__ void __launch_bounds__(BLOCK) kg_fluid_rank(const SceneArgs* __restrict__ S, const int2* __restrict__ W) {
   FOR_ITEMS(S, W) fluid_rank_body(A, lb);
+}
+
+// The fluid rank with constraint iteration `it` on workgroups of their own
+// (the solid chain of a group that does not fork, fp_fluid.hip).
+__global__ void __launch_bounds__(BLOCK) kg_fluid_rank_cons(const SceneArgs* __restrict__ S,
+                                                            const int2* __restrict__ WC,
+                                                            const int2* __restrict__ WP, int g0, int it) {
+  if ((int)blockIdx.x < g0) {
+    FOR_ITEMS_AT(S, WC, (int)blockIdx.x, g0) constraints_body(A, lb, it);
+  } else {
+    FOR_ITEMS_AT(S, WP, (int)blockIdx.x - g0, (int)gridDim.x - g0) fluid_rank_body(A, lb);
+  }
+}
+
+int fp_group_resort_with(fp_group* g, int it) {
+  if (it < 0) return fp_group_resort(g);
+  PassScope ps(g->prof, g->stream, P_RESORT);
+  GROUP_LAUNCH(g, kg_scan_apply, W_SCAN, BLOCK, 1);
+  GROUP_LAUNCH(g, kg_fluid_scatter, W_PARTICLE, BLOCK);
+  GROUP_LAUNCH2(g, kg_fluid_rank_cons, W_CONS, W_PARTICLE, BLOCK, it);
+  FP_GTRY(g, hipGetLastError());
+  return FP_OK;
 }
 
 int fp_group_resort(fp_group* g) {

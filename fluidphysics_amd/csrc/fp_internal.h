@@ -29,7 +29,8 @@
 //     1/max(deg_a, deg_b)); erest f32[2C], ealive u8[2C]: each CSR entry's
 //     copy of its constraint's rest length and alive flag (the two copies are
 //     updated identically, in place, each by its own endpoint's lane);
-//     spos_c[2] float4[Ns] between iterations
+//     spos_c[2] float4[Ns] between iterations (spos_c[1] first holds the
+//     solid contacts' result, the first iteration's snapshot)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -454,6 +455,7 @@ struct fp_group {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool capturing = false;
   bool fork = false;  // the frame being captured forks
+  bool ride = false;  // the solid chain rides in the fluid chain's launches (a group that never forks)
   // the members or their parameters changed for this frame: it runs eagerly
   // (a graph captured now might be replayed once; continuous batching
   // changes members every few frames)
@@ -565,6 +567,19 @@ static inline int fp_grid8(int n) { return (n + 7) & ~7; }
                          ##__VA_ARGS__);                                                             \
   } while (0)
 
+// One launch over two work lists: kind k0's workgroups first, then kind
+// k1's (each count as GROUP_LAUNCH would take it); the kernel gets both lists
+// and g0, kind k0's workgroup count.  g0 is a multiple of 8, so a workgroup
+// keeps its XCD (b % 8) at its place b - g0 in the second list.
+#define GROUP_LAUNCH2(g, kernel, k0, k1, block, ...)                                               \
+  do {                                                                                               \
+    const int g0_ = (g)->capturing ? (g)->grid_cap[k0] : fp_grid8((g)->n_items[k0]);               \
+    const int g1_ = (g)->capturing ? (g)->grid_cap[k1] : fp_grid8((g)->n_items[k1]);               \
+    if (g0_ + g1_ > 0)                                                                               \
+      hipLaunchKernelGGL(kernel, dim3(g0_ + g1_), dim3(block), 0, (g)->stream, (g)->d_args,          \
+                         (g)->d_items[k0], (g)->d_items[k1], g0_, ##__VA_ARGS__);                    \
+  } while (0)
+
 // The number of items of work list W (the same in every lane).
 __device__ __forceinline__ int items_count(const int2* __restrict__ W) {
   return __builtin_amdgcn_readfirstlane(*(const FP_GLOBAL int*)&W[0].x);
@@ -595,8 +610,12 @@ __device__ __forceinline__ int items_run(const int2* __restrict__ W) {
   return __builtin_amdgcn_readfirstlane(*(const FP_GLOBAL int*)&W[0].y);
 }
 
-#define FOR_ITEMS(S, W)                                                                    \
-  for (int w_ = (int)blockIdx.x, nw_ = items_count(W), k_ = items_run(W); w_ < nw_; w_ += (int)gridDim.x) \
+#define FOR_ITEMS(S, W) FOR_ITEMS_AT(S, W, (int)blockIdx.x, (int)gridDim.x)
+// The same over list positions b0, b0 + G, ...: the workgroups of a launch
+// that serves two work lists (GROUP_LAUNCH2) start their list at b0 and
+// stride by their own count G.
+#define FOR_ITEMS_AT(S, W, b0, G)                                                          \
+  for (int w_ = (b0), nw_ = items_count(W), k_ = items_run(W); w_ < nw_; w_ += (G))        \
     if (const int2 it_ = (W)[1 + xcd_item(w_, nw_, k_)]; true)                             \
       if (const SceneArgs& A = (S)[it_.x]; true)                                           \
         if (const int lb = it_.y; true)
@@ -614,6 +633,11 @@ int fp_group_cell_grid(fp_group* g, bool count_done);
 int fp_group_resort(fp_group* g);
 // fp_fluid.hip
 int fp_group_contacts(fp_group* g);
+// The solid chain riding in the fluid chain's launches (groups that do not fork):
+int fp_group_contacts_both(fp_group* g);              // solid + fluid contacts, one launch
+int fp_group_resort_with(fp_group* g, int it);        // the fluid rank + constraint iteration it
+int fp_group_density_with(fp_group* g, int it);       // (it < 0: the pass alone)
+int fp_group_displace_with(fp_group* g, int it);
 int fp_group_contacts_solid(fp_group* g, hipStream_t st);
 int fp_group_contacts_fluid(fp_group* g);
 int fp_group_density(fp_group* g);

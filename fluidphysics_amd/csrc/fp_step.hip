@@ -303,6 +303,14 @@ static hipError_t reserve(fp_group* g, T*& p, int& cap, int count, bool& moved) 
   return e;
 }
 
+// 1: captured frames of groups with FP_SIDE_MIN particle slots or more run
+// the solid chain on the group's side stream.
+#ifndef FP_SIDE_STREAM
+#define FP_SIDE_STREAM 1
+#endif
+#ifndef FP_SIDE_MIN
+#define FP_SIDE_MIN 131072
+#endif
 // Scene table and work lists for substep h: rewritten when the members,
 // their parameters or h changed.  A captured frame survives a rewrite (its
 // kernels read the table and the lists' counts on the device); it is dropped
@@ -346,6 +354,11 @@ int fp_group_prepare(fp_group* g, double h) {
     g->any_tets |= args[k].sm.tets != 0;
   }
   g->any_extra = g->any_cache || g->any_tets;
+  {
+    int64_t slots = 0;
+    for (fp_solver* s : g->scenes) slots += s->n;
+    g->ride = !g->any_extra && !(FP_SIDE_STREAM && slots >= FP_SIDE_MIN);
+  }
   int64_t snap = 0;
   for (int k = 0; k < ns; k++) {
     args[k].snap_off = snap;
@@ -410,14 +423,6 @@ int fp_group_prepare(fp_group* g, double h) {
   return FP_OK;
 }
 
-// 1: captured frames of groups with FP_SIDE_MIN particle slots or more run
-// the solid chain on the group's side stream.
-#ifndef FP_SIDE_STREAM
-#define FP_SIDE_STREAM 1
-#endif
-#ifndef FP_SIDE_MIN
-#define FP_SIDE_MIN 131072
-#endif
 // 1: the first frame after the members (or their parameters) changed runs
 // eagerly; capture waits for a frame whose members match the last one's.
 #ifndef FP_GRAPH_DEFER
@@ -446,7 +451,20 @@ static int group_substep(fp_group* g) {
     }
   }
   if ((rc = fp_group_cell_grid(g, !g->any_grabs))) return rc;
-  if (g->capturing && g->fork) {
+  if (g->ride && !g->tune_record && !(g->prof && g->prof->on)) {
+    // a group that does not fork: its solid chain rides in the fluid chain's
+    // launches (fp_fluid.hip), the last three constraint iterations beside
+    // the fluid rank, the density and the displacement, any earlier ones on
+    // launches of their own after the contacts (a tuning frame times the
+    // solid contacts alone, and per-pass profiling times every pass on its
+    // own launch: both take the serial order below)
+    const int I = g->iters, own = I > 3 ? I - 3 : 0, k = I - own;
+    if ((rc = fp_group_contacts_both(g))) return rc;
+    if ((rc = fp_group_constraints(g, own, g->stream))) return rc;
+    if ((rc = fp_group_resort_with(g, k >= 3 ? own : -1))) return rc;
+    if ((rc = fp_group_density_with(g, k >= 2 ? I - 2 : -1))) return rc;
+    if ((rc = fp_group_displace_with(g, k >= 1 ? I - 1 : -1))) return rc;
+  } else if (g->capturing && g->fork) {
     // the solid chain on the side stream, the fluid chain here; they join
     // before XSPH (each chain ends its particles' substep: the last
     // constraint iteration its solids', the displacement its fluid's).  The

@@ -281,6 +281,45 @@ def test_forked_frames_equal_serial_frames():
             _same(runs[0][f][k], runs[1][f][k], "%s frame %d" % (scs[k].name, f))
 
 
+@pytest.mark.parametrize("iters", [1, 2, 3, 4])
+def test_riding_solid_chain_equals_its_own_launches(iters):
+    """A group below 131,072 particle slots runs its solid chain inside the
+    fluid chain's launches: the solid contacts beside the fluid contacts, the
+    last three constraint iterations beside the fluid rank, the density and
+    the displacement, earlier ones on launches of their own.  A larger group
+    runs every solid pass on its own launch (one stream when eager, a second
+    stream in a captured frame).  Scenes stepped alone (riding) equal the same
+    scenes in such a batch, bit for bit, whatever the iteration count."""
+    from fluidphysics_amd import Batch, from_scene
+    frames = TUNE_FRAMES + 3
+    scs, slots = [], 0
+    for k in range(40, 80):
+        sc = _scenes([k], frames)[0]
+        sc.params["constraint_iters"] = iters
+        scs.append(sc)
+        slots += sc.cube[3] ** 3 + sc.max_fluid
+        if slots >= 140000:
+            break
+    assert slots >= 131072
+    log("case riding solid chain, %d iterations: %d scenes, %d slots" % (iters, len(scs), slots))
+    picks = [0, len(scs) - 1]
+    ref = {k: _alone(scs[k], frames) for k in picks}
+    for use_graph in (True, False):
+        b = Batch()
+        b.graph(use_graph)
+        ss = [from_scene(sc) for sc in scs]
+        for s in ss:
+            b.add(s)
+        for f in range(frames):
+            for s, sc in zip(ss, scs):
+                _cuts(s, sc, f)
+            b.step(DT, SUB)
+            if f >= TUNE_FRAMES:
+                for k in picks:
+                    _same(_snap(ss[k]), ref[k][f], "%s frame %d (graph %s)" % (scs[k].name, f, use_graph))
+        b.close()
+
+
 def test_unconstrained_scene_finishes_its_solids():
     """A scene with solids but no constraints (a one-particle cube) has no
     constraint iteration to end its solids' substep: kg_bounds_velocity does

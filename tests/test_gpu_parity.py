@@ -190,6 +190,27 @@ def test_one_substep_mid_run_vs_fp64_jacobi():
     tol_report("gap to reference GS order", st["pos"][act], gs["pos"][act], 1e-4 * 0.19)
 
 
+@pytest.mark.parametrize("iters", [1, 2])
+def test_one_substep_few_iterations_vs_fp64_jacobi(iters):
+    """The frame-150 state with one and two constraint iterations.  A single
+    iteration reads its snapshot from the solid contacts' copy and finishes
+    the solids into pos in the same launch; two iterations take the
+    ping-pong buffers.  Both within 1e-4 h of the FP64 Jacobi oracle."""
+    log("case one-substep f150, %d constraint iterations" % iters)
+    d = G.scenario("plain_cpu")
+    ref = _state_from_dump("plain_cpu", "f150")
+    g = make_gpu(d, iters=iters, omega=8.0)
+    o = G.make_oracle(d, storage="f64", mode="jacobi", iters=iters, jacobi_omega=8.0)
+    inv_mass = g.read_state()["inv_mass"]
+    for s in (g, o):
+        s.write_state(ref["pos"], ref["prev"], ref["vel"], inv_mass, ref["active"])
+    g.write_constraints(ref["rest"].astype(np.float32), ref["alive"])
+    o.write_constraints(ref["rest"], ref["alive"])
+    g.step(SUB, 1)
+    o.step(SUB, 1)
+    _compare_substep(g, o, 0.19, "%d iterations vs jacobi" % iters)
+
+
 @pytest.mark.parametrize("scene_name,frames", [("single_gpu", 0), ("single_gpu", 30), ("large", 0), ("large", 12)])
 def test_one_substep_at_scale_vs_fp64_jacobi(scene_name, frames):
     from oracle.oracle import OracleSolver
