@@ -4,12 +4,14 @@ scene, for the GPU's multi-frame statistics test:
 
     python3 tests/golden/make_oracle_stats.py single_gpu [frames]
     python3 tests/golden/make_oracle_stats.py large [frames]
+    python3 tests/golden/make_oracle_stats.py batched_<index> [frames]
 
 writes tests/golden/<scene>_jacobi_trace.json: per-frame kinetic energy,
 mean fluid density, alive constraints and fluid count of the oracle
 (float32 stores, Jacobi mode, the scene's constraint_iters / jacobi_omega),
 with the scene's wedges and hose bursts (the large scene: about 27 minutes
-for 300 frames on one core).  The reference (src/solver.js) itself is not usable
+for 300 frames on one core; a scene of the batched workload, with its tilted
+gravity and slices, one to two minutes).  The reference (src/solver.js) itself is not usable
 as the target at these constants: its single Gauss-Seidel sweep goes
 unstable on the 20^3 cube within 7 frames (single_gpu_stats_trace.json,
 written by make_golden.mjs from the reference)."""
@@ -44,7 +46,10 @@ def frame_stats(o, ns):
 def main():
     name = sys.argv[1] if len(sys.argv) > 1 else "single_gpu"
     frames = int(sys.argv[2]) if len(sys.argv) > 2 else 300
-    sc = scenes.SCENES[name]()
+    if name.startswith("batched_"):
+        sc = scenes.batched_scene(int(name.split("_")[1]), frames=frames)
+    else:
+        sc = scenes.SCENES[name]()
     p = sc.params_with()
     o = from_scene(sc, storage="f32", mode="jacobi")
     sp = 1.91 * p["fluid_radius"]

@@ -417,6 +417,68 @@ def test_config2_300_frames_statistics_vs_jacobi_oracle():
     assert abs(alive / ral - 1) < 0.01
 
 
+BATCHED_STATS_SCENES = (2, 10, 21, 27)
+
+
+def test_config4_scenes_300_frames_statistics_vs_jacobi_oracle():
+    """Four scenes of the batched workload (cubes 11^3 to 17^3, 2.9k to 8.9k
+    fluid, gravity tilted by 4 to 14 degrees, one or two slices each),
+    stepped together as one batch for 300 frames: per scene, the time-mean
+    kinetic energy and fluid density and the final alive and fluid counts
+    within 1% of the oracle's Jacobi formulation
+    (tests/golden/batched_<index>_jacobi_trace.json, float32 stores, the
+    scenes' own iterations, relaxation and slices)."""
+    from fluidphysics_amd import Batch, from_scene, scenes
+    scs = [scenes.batched_scene(k, frames=300) for k in BATCHED_STATS_SCENES]
+    ts = [G.trace("batched_%d_jacobi" % k) for k in BATCHED_STATS_SCENES]
+    log("case 300 frames of %s in one batch vs the Jacobi oracle" % [sc.name for sc in scs])
+    for sc, t in zip(scs, ts):
+        assert t["constraint_iters"] == sc.params_with()["constraint_iters"]
+        assert t["jacobi_omega"] == sc.params_with()["jacobi_omega"]
+    ss = [from_scene(sc) for sc in scs]
+    b = Batch()
+    for s in ss:
+        b.add(s)
+    frames = len(ts[0]["frames"])
+    ke = [[] for _ in scs]
+    rho = [[] for _ in scs]
+    cuts = [[] for _ in scs]
+    for f in range(frames):
+        for k, (s, sc) in enumerate(zip(ss, scs)):
+            for w in sc.wedges:
+                if w.frame == f:
+                    cuts[k].append(s.cut(*w.o, *w.d1, *w.d2))
+        b.step(scs[0].dt, scs[0].substeps)
+        for k, (s, sc) in enumerate(zip(ss, scs)):
+            st = s.read_state()
+            r, _ = s.read_density()
+            fs = G.frame_stats(st, sc.solid_count, r)
+            ke[k].append(fs["ke"])
+            rho[k].append(fs["meanRho"])
+            assert np.isfinite(st["pos"]).all(), (sc.name, f)
+    alive = [int(s.read_constraints()["alive"].sum()) for s in ss]
+    fluid = [s.fluidCount() for s in ss]
+    b.close()
+    for s in ss:
+        s.close()
+    for k, (sc, t) in enumerate(zip(scs, ts)):
+        rke = np.mean([fr["stats"]["ke"] for fr in t["frames"]])
+        rrho = np.mean([fr["stats"]["meanRho"] for fr in t["frames"]])
+        ral = t["frames"][-1]["stats"]["alive"]
+        rfl = t["frames"][-1]["stats"]["fluid"]
+        log("  %s: mean KE %.5f vs %.5f (%+.2f%%), mean rho %.5f vs %.5f (%+.2f%%), alive %d vs %d, fluid %d vs %d, "
+            "cuts %s (oracle %s)" % (sc.name, np.mean(ke[k]), rke, 100 * (np.mean(ke[k]) / rke - 1), np.mean(rho[k]),
+                                     rrho, 100 * (np.mean(rho[k]) / rrho - 1), alive[k], ral, fluid[k], rfl, cuts[k],
+                                     [c["n"] for c in t["cuts"]]))
+    for k, (sc, t) in enumerate(zip(scs, ts)):
+        rke = np.mean([fr["stats"]["ke"] for fr in t["frames"]])
+        rrho = np.mean([fr["stats"]["meanRho"] for fr in t["frames"]])
+        assert abs(np.mean(ke[k]) / rke - 1) < 0.01, sc.name
+        assert abs(np.mean(rho[k]) / rrho - 1) < 0.01, sc.name
+        assert abs(alive[k] / t["frames"][-1]["stats"]["alive"] - 1) < 0.01, sc.name
+        assert abs(fluid[k] / t["frames"][-1]["stats"]["fluid"] - 1) < 0.01, sc.name
+
+
 def test_labels_due_from_a_tear_survive_a_cut_before_they_are_read():
     """A step that tore constraints makes new group labels due (solver.js
     relabels at the step's end, :238-241); a cut before the caller reads
