@@ -218,12 +218,13 @@ __device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb, 
   const int t = lb * BLOCK + threadIdx.x;
   const int x = t / L, lane = t % L;
   int* held_ids = lds_bonds + (threadIdx.x / L) * MAXB;
-  const bool valid = x < fp.ns && gld(A.active + x);
-  int k0 = 0, k1 = 0;
-  if (valid) {
-    k0 = gld(cc.adj_off + x);
-    k1 = gld(cc.adj_off + x + 1);
-  }
+  // the solid's flag, CSR row and position load together (a thread past the
+  // solids reads solid 0's and is dropped)
+  const int xc = x < fp.ns ? x : 0;
+  const bool valid = x < fp.ns && gld(A.active + xc);
+  int k0 = gld(cc.adj_off + xc), k1 = gld(cc.adj_off + xc + 1);
+  const float4 px0 = gld(pos + xc);
+  if (!valid) k0 = k1 = 0;
   const bool held = k1 - k0 <= MAXB;
 #pragma unroll
   for (int j = lane; j < MAXB; j += L) {
@@ -239,7 +240,7 @@ __device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb, 
   for (int j = 0; j < M; j++) px_[j] = py_[j] = pz_[j] = 0.f;
   float4 px = make_float4(0.f, 0.f, 0.f, 0.f);
   if (valid) {
-    px = gld(pos + x);
+    px = px0;
     const Cell c0 = cell_of(px.x, px.y, px.z, fp.cg.inv_cell);
     const uint32_t kx = cell_key(fp.cg, c0);
     const FaceDist fd = face_dist(px.x, px.y, px.z, c0, fp.cs);
@@ -712,17 +713,40 @@ __device__ __forceinline__ void visit_rows_simple(const FluidParams& fp, const P
 // Entries are read one batch of REPLAY_BATCH ahead; a batch's gathers (the
 // neighbour's position when F::kNeedPos, and f.fetch(q)) are issued
 // together, before the next batch's entries, and before any of them is used.
+// The loads a replay pass starts with, indexed by the lane alone and issued
+// together with the slot's own loads: the lane's pair count and its first
+// REPLAY_BATCH entries (rows past the block's sink are not read; a lane past
+// the lane arrays reads their last lane and is dropped by its caller).
+struct ReplayHead {
+  int cnt;
+  uint32_t v[REPLAY_BATCH];
+};
+
+__device__ __forceinline__ ReplayHead replay_head(const FluidParams& fp, const PairList& pl, int v) {
+  ReplayHead r;
+  const int vc = min(v, fp.n * (1 << fp.fl_log2) - 1);
+  const uint32_t* base = pair_block(pl, vc);
+  const uint32_t lo = pair_lane_off(vc);
+  r.cnt = gld(pl.ncount + vc);
+#pragma unroll
+  for (int i = 0; i < REPLAY_BATCH; i++) r.v[i] = gld_at(base, lo + (uint32_t)min(i, pl.cap) * NB_ROWB);
+  return r;
+}
+
 template <class F>
-__device__ __forceinline__ void replay_pairs(const PairSource& src, const PairList& pl, const PairSelf& me, int cnt,
-                                             F& f) {
+__device__ __forceinline__ void replay_pairs(const PairSource& src, const PairList& pl, const PairSelf& me,
+                                             const ReplayHead& rh, F& f) {
+  const int cnt = rh.cnt;
   const uint32_t* base = pair_block(pl, me.v);
   const uint32_t lo = pair_lane_off(me.v);
   const int last = cnt - 1;
   // entry k of this lane at byte lo + k * NB_ROWB of the wave's block
   const uint32_t olast = lo + (uint32_t)max(last, 0) * NB_ROWB;
+  // entries past the count are not this substep's: entry 0 stands in (a
+  // valid slot to gather from; its result is not used)
   uint32_t v[REPLAY_BATCH];
 #pragma unroll
-  for (int i = 0; i < REPLAY_BATCH; i++) v[i] = cnt > 0 ? gld_at(base, min(lo + (uint32_t)i * NB_ROWB, olast)) : 0u;
+  for (int i = 0; i < REPLAY_BATCH; i++) v[i] = i < cnt ? rh.v[i] : 0u;
   for (int k = 0; k < cnt; k += REPLAY_BATCH) {
     // this batch's gathers first, then the next batch's entries: the loads
     // complete in issue order, so waiting for the gathers leaves the
@@ -764,11 +788,10 @@ __device__ __forceinline__ void replay_pairs(const PairSource& src, const PairLi
 // grid again (global reads) for a particle whose list overflowed.
 template <class F>
 __device__ __forceinline__ void recorded_pairs(const FluidParams& fp, const PairSource& src, const PairList& pl,
-                                               const PairSelf& me, F& f) {
-  const int cnt = gld(pl.ncount + me.v);
+                                               const PairSelf& me, const ReplayHead& rh, F& f) {
   // a slot whose lanes did not all fit is gathered again by all its lanes
-  if (__builtin_expect(lanes_sum(cnt > pl.cap ? 1 : 0, 1 << fp.fl_log2) == 0, 1)) {
-    replay_pairs(src, pl, me, cnt, f);
+  if (__builtin_expect(lanes_sum(rh.cnt > pl.cap ? 1 : 0, 1 << fp.fl_log2) == 0, 1)) {
+    replay_pairs(src, pl, me, rh, f);
   } else {
     visit_rows_simple(fp, src, me, f);
   }
@@ -779,9 +802,11 @@ __device__ __forceinline__ bool pair_self(const FluidParams& fp, const PairSourc
   me.v = v;
   me.h = v & ((1 << fp.fl_log2) - 1);
   me.e = v >> fp.fl_log2;
-  if (me.e >= gld(src.lstart + fp.cg.M)) return false;
-  me.p = gld(src.spos2 + me.e);
-  return true;
+  // the position loads with the fluid order's size (a lane past the last
+  // slot reads an in-bounds one and is dropped)
+  const int occupied = gld(src.lstart + fp.cg.M);
+  me.p = gld(src.spos2 + min(me.e, fp.n - 1));
+  return me.e < occupied;
 }
 
 // The outputs of a gathered slot (every lane of the slot calls this).
@@ -910,13 +935,17 @@ __device__ __forceinline__ void displace_body(const SceneArgs& A, int lb) {
   const float2* __restrict__ sP = A.sP;
   float4* __restrict__ pos = A.pos;
   PairSelf me;
+  const ReplayHead rh = replay_head(fp, pl, fluid_lane(lb));
+  const float2 Px = gld(sP + min(fluid_lane(lb) >> fp.fl_log2, fp.n - 1));
   if (!pair_self(fp, src, fluid_lane(lb), me)) return;
+  // the inverse mass (pos.w) loads beside the replay
+  const float w = gld(pos + me.x()).w;
   DisplaceAcc acc;
   acc.sP = sP;
-  acc.Px = gld(sP + me.e);
+  acc.Px = Px;
   acc.inv_H = fp.inv_H;
   acc.max_pair = fp.max_pair;
-  recorded_pairs(fp, src, pl, me, acc);
+  recorded_pairs(fp, src, pl, me, rh, acc);
   const int L = 1 << fp.fl_log2;
   float sx = lanes_sum(acc.sx, L), sy = lanes_sum(acc.sy, L), sz = lanes_sum(acc.sz, L);
   if (me.h != 0) return;
@@ -927,7 +956,6 @@ __device__ __forceinline__ void displace_body(const SceneArgs& A, int lb) {
     sy *= g;
     sz *= g;
   }
-  const float w = gld(pos + me.x()).w;
   // the particle's substep ends here: bounds, then its velocity for XSPH
   fluid_finish(A, me.x(), me.e, make_float4(me.p.x + sx, me.p.y + sy, me.p.z + sz, w));
 }
@@ -973,16 +1001,19 @@ __device__ __forceinline__ void xsph_body(const SceneArgs& A, int lb) {
   // goes to the scene's host-mapped word, which the host reads after a sync
   if (lb == 0 && threadIdx.x == 0 && gld(A.dflags + DF_NONFINITE)) A.tear[1] = 1;
   PairSelf me;
+  const ReplayHead rh = replay_head(fp, pl, fluid_lane(lb));
+  const float4 v = gld(svtmp + min(fluid_lane(lb) >> fp.fl_log2, fp.n - 1));
   if (!pair_self(fp, src, fluid_lane(lb), me)) return;
-  if (!gld(active + me.x())) return;
+  // a particle culled since the density pass keeps its velocity: its flag is
+  // read beside the replay and decides only the store
+  const bool act = gld(active + me.x()) != 0;
   XsphAcc acc;
   acc.svtmp = svtmp;
-  recorded_pairs(fp, src, pl, me, acc);
+  recorded_pairs(fp, src, pl, me, rh, acc);
   const int L = 1 << fp.fl_log2;
   const int m = lanes_sum(acc.m, L);
   const float sx = lanes_sum(acc.sx, L), sy = lanes_sum(acc.sy, L), sz = lanes_sum(acc.sz, L);
-  if (me.h != 0) return;
-  const float4 v = gld(svtmp + me.e);
+  if (me.h != 0 || !act) return;
   if (m == 0) {
     gst(vel + me.x(), v);
     return;
