@@ -645,7 +645,7 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
     hoff[s->h_cons_b[c] + 1]++;
   }
   for (int i = 0; i < ns; i++) hoff[i + 1] += hoff[i];
-  std::vector<int2> hadj(std::max(1, 2 * C));
+  std::vector<uint32_t> hadj(std::max(1, 2 * C));
   std::vector<float> herest(std::max(1, 2 * C));
   std::vector<int> fill(ns, 0);
   std::vector<int2> hab(std::max(1, C));
@@ -653,12 +653,10 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
   auto deg = [&](int i) { return hoff[i + 1] - hoff[i]; };
   for (int c = 0; c < C; c++) {
     const int a = s->h_cons_a[c], b = s->h_cons_b[c];
-    float sc = 1.f / (float)std::max(deg(a), deg(b));
-    int scb;
-    std::memcpy(&scb, &sc, 4);
+    const uint32_t md = (uint32_t)std::max(deg(a), deg(b)) << 24;
     const int ka = hoff[a] + fill[a]++, kb = hoff[b] + fill[b]++;
-    hadj[ka] = make_int2(b, scb);
-    hadj[kb] = make_int2(a, scb);
+    hadj[ka] = ((uint32_t)(b - a) & 0xFFFFFFu) | md;
+    hadj[kb] = ((uint32_t)(a - b) & 0xFFFFFFu) | md;
     herest[ka] = herest[kb] = hrest[c];
     s->h_cent[c] = make_int2(ka, kb);
     hab[c] = make_int2(a, b);
@@ -751,7 +749,7 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
     FP_TRY(s, h2d(s, s->cent, s->h_cent.data(), C * sizeof(int2)));
     FP_TRY(s, h2d(s, s->erest, herest.data(), 2 * C * sizeof(float)));
     FP_TRY(s, h2d(s, s->ealive, ones.data(), 2 * C));
-    FP_TRY(s, h2d(s, s->adj, hadj.data(), 2 * C * sizeof(int2)));
+    FP_TRY(s, h2d(s, s->adj, hadj.data(), 2 * C * sizeof(uint32_t)));
   }
   FP_TRY(s, h2d(s, s->adj_off, hoff.data(), (ns + 1) * sizeof(int)));
   if (s->tets_on && s->NT) {

@@ -48,7 +48,7 @@ __device__ __forceinline__ void constraints_body(const SceneArgs& A, int lb, int
     // the neighbours' positions (a dead entry's neighbour index is still
     // valid; entries past the row repeat its last one and are not used)
     bool alive[CCHUNK];
-    int2 e[CCHUNK];
+    uint32_t e[CCHUNK];
     float rest[CCHUNK];
     float4 pb[CCHUNK];
 #pragma unroll
@@ -59,7 +59,7 @@ __device__ __forceinline__ void constraints_body(const SceneArgs& A, int lb, int
       rest[u] = gld(erest + k);
     }
 #pragma unroll
-    for (int u = 0; u < CCHUNK; u++) pb[u] = gld(sin + e[u].x);
+    for (int u = 0; u < CCHUNK; u++) pb[u] = gld(sin + adj_other(a, e[u]));
 #pragma unroll
     for (int u = 0; u < CCHUNK; u++) {
       const int k = kb + u * FP_CLANES;
@@ -73,7 +73,7 @@ __device__ __forceinline__ void constraints_body(const SceneArgs& A, int lb, int
         } else {
           const float C = d - rest[u];
           const float dl = -C / (w + cp.alpha);
-          const float s = dl / d * pa.w * (cp.omega * __int_as_float(e[u].y));
+          const float s = dl / d * pa.w * (cp.omega * adj_scale(e[u]));
           cx += dx * s;
           cy += dy * s;
           cz += dz * s;

@@ -189,7 +189,7 @@ __device__ __forceinline__ bool bonded(int x, int y, bool held, const int* held_
   }
   const int e = gld(cc.adj_off + x + 1);
   bool hit = false;
-  for (int k = gld(cc.adj_off + x); k < e; k++) hit |= gld(cc.adj + k).x == y && gld(cc.ealive + k) != 0;
+  for (int k = gld(cc.adj_off + x); k < e; k++) hit |= adj_other(x, gld(cc.adj + k)) == y && gld(cc.ealive + k) != 0;
   return hit;
 }
 
@@ -230,7 +230,7 @@ __device__ __forceinline__ void contacts_solid_body(const SceneArgs& A, int lb, 
   for (int j = lane; j < MAXB; j += L) {
     int v = -1;
     if (valid && held && k0 + j < k1) {
-      if (gld(cc.ealive + k0 + j)) v = gld(cc.adj + k0 + j).x;
+      if (gld(cc.ealive + k0 + j)) v = adj_other(x, gld(cc.adj + k0 + j));
     }
     held_ids[j] = v;
   }

@@ -185,7 +185,7 @@ struct fp_solver {
   uint8_t* ealive = nullptr;  // per CSR entry
   uint8_t* ealive_snap = nullptr;  // ealive when due labels were deferred past a cut (groups_from_snap)
   int* adj_off = nullptr;
-  int2* adj = nullptr;
+  uint32_t* adj = nullptr;  // per CSR entry: the other endpoint - this solid (24-bit) | max(deg) << 24
   float4* spos_c[2] = {nullptr, nullptr};
   // the GPU twin's optional solid modes (fp_params.tet_volume / solid_contact_cache)
   bool tets_on = false, cache_on = false;
@@ -302,7 +302,7 @@ struct ContactCtx {
   const int* __restrict__ lstart;   // grid-order cell starts
   const int* __restrict__ lnsol;    // solids of each non-empty cell (they come first)
   const int* __restrict__ adj_off;
-  const int2* __restrict__ adj;
+  const uint32_t* __restrict__ adj;  // packed entries (adj_other, adj_scale)
   const uint8_t* __restrict__ ealive;  // per CSR entry
   const int* __restrict__ sbox;  // solids' bounding box at grid time (ordered-int bits)
 };
@@ -365,7 +365,7 @@ struct SceneArgs {
   uint8_t* ealive;
   int2* cent;
   int* adj_off;
-  int2* adj;
+  uint32_t* adj;
   float4* spos_c[2];
   SolidModes sm;
 };

@@ -176,6 +176,21 @@ __device__ __forceinline__ uint32_t row_key(const CellGrid& g, uint32_t k, int a
   return (k + (uint32_t)a * g.mx + (uint32_t)b * g.my - (uint32_t)NB_R) & g.kmask;
 }
 
+// ---- the constraint CSR -----------------------------------------------------
+//
+// An entry of solid a's row packs the other endpoint as an offset from a
+// (24-bit two's complement: lattice neighbours of a cube the 32-bit CSR can
+// hold lie within n^2 + n + 1 < 2^23) and max(deg_a, deg_b) in the top byte;
+// the constraint's scale 1 / max(deg) is read from a table of correctly
+// rounded reciprocals (the host's float division gives the same values).
+__device__ __forceinline__ int adj_other(int a, uint32_t pk) { return a + ((int)(pk << 8) >> 8); }
+__device__ static const float kInvDeg[32] = {
+    0.f,        1.f / 1,  1.f / 2,  1.f / 3,  1.f / 4,  1.f / 5,  1.f / 6,  1.f / 7,
+    1.f / 8,    1.f / 9,  1.f / 10, 1.f / 11, 1.f / 12, 1.f / 13, 1.f / 14, 1.f / 15,
+    1.f / 16,   1.f / 17, 1.f / 18, 1.f / 19, 1.f / 20, 1.f / 21, 1.f / 22, 1.f / 23,
+    1.f / 24,   1.f / 25, 1.f / 26, 1.f / 27, 1.f / 28, 1.f / 29, 1.f / 30, 1.f / 31};
+__device__ __forceinline__ float adj_scale(uint32_t pk) { return kInvDeg[(pk >> 24) & 31u]; }
+
 // ---- global-memory access -------------------------------------------------------
 //
 // The kernels read their arrays' addresses from a device table of scenes, so

@@ -50,7 +50,7 @@ __device__ __forceinline__ bool lattice_adjacent(int i, int j, int n) {
 __device__ __forceinline__ bool bonded_csr(const SceneArgs& A, int x, int y) {
   const int e = gld(A.adj_off + x + 1);
   bool hit = false;
-  for (int k = gld(A.adj_off + x); k < e; k++) hit |= gld(A.adj + k).x == y && gld(A.ealive + k) != 0;
+  for (int k = gld(A.adj_off + x); k < e; k++) hit |= adj_other(x, gld(A.adj + k)) == y && gld(A.ealive + k) != 0;
   return hit;
 }
 

@@ -47,7 +47,9 @@ def test_scene_bytes_without_device():
     p = scenes.gpu_scale_params()
     a = scene_bytes(p, 4000, 10)
     assert a > 0
-    assert scene_bytes(p, 8000, 10) > a and scene_bytes(p, 4000, 12) > a
+    # sizes are rounded up to the device pool's size classes: a little more
+    # never takes less, a lot more takes more
+    assert scene_bytes(p, 4000, 12) >= a and scene_bytes(p, 8000, 10) > a and scene_bytes(p, 4000, 20) > a
     assert scene_bytes(p, 0, 6) > 0
     with pytest.raises(Exception):
         scene_bytes(p, 100, 0)
