@@ -91,8 +91,15 @@ __device__ __forceinline__ void fluid_finish(const SceneArgs& A, int i, int fs, 
   bounds_apply(bp, false, p, pv);
   gst(A.pos + i, p);
   if (p.y < -3.f || p.x * p.x + p.z * p.z > bp.cull_r2) {
+    // culled: the velocity stays the substep's integrated one, vel + g h
+    // (integrate does not store it; the same sums here)
     gst(A.active + i, (uint8_t)0);
-    gst(A.svtmp + fs, gld(A.vel + i));
+    float4 v = gld(A.vel + i);
+    v.y += A.gvy;
+    if (A.gvx != 0.f) v.x += A.gvx;
+    if (A.gvz != 0.f) v.z += A.gvz;
+    gst(A.vel + i, v);
+    gst(A.svtmp + fs, v);
     return;
   }
   const float4 v = substep_velocity(bp, false, p, pv);

@@ -53,7 +53,6 @@ __device__ __forceinline__ void integrate_body(const SceneArgs& A, int lb) {
     p.y = fmaf(v.y, A.h, p.y);
     p.z = fmaf(v.z, A.h, p.z);
     gst(A.pos + i, p);
-    gst(A.vel + i, v);
     if (COUNT) {
       gst(A.phash + i, cell_hash(cell_of(p.x, p.y, p.z, A.fp.cg.ref_inv), A.fp.cg.hmask));
       k = cell_key(A.fp.cg, cell_of(p.x, p.y, p.z, A.fp.cg.inv_cell));
