@@ -142,7 +142,7 @@ void scene_buffers(fp_solver* s, F& take) {
   take(s->lblock_sums, (size_t)s->lscan_blocks), take(s->lblock_f, (size_t)s->lscan_blocks);
   take(s->ltmp, N), take(s->lkey, N), take(s->lslot, N), take(s->sid, N), take(s->spos, N), take(s->spos2, N);
   take(s->svtmp, N), take(s->sphash, N), take(s->sP, N), take(s->tpos, N), take(s->tkey, N), take(s->tslot, N);
-  take(s->fsid, N), take(s->fslot, N), take(s->sphash2, N), take(s->sbox, 6);
+  take(s->sphash2, N), take(s->sbox, 6);
   const size_t lanes = N * (size_t)s->fluid_lanes, lanes64 = (lanes + 63) / 64 * 64;
   take(s->nbr, (size_t)(s->pair_slots / s->fluid_lanes + 1) * lanes64), take(s->ncount, lanes);
   take(s->pslot, N), take(s->count, T), take(s->cell_start, T + 1), take(s->entries_tmp, N);
@@ -1229,8 +1229,22 @@ int fp_read_cells(fp_solver* s, int32_t which, double* desc, int32_t* start, int
     desc[4] = (double)slots;
   }
   if (start) FP_TRY(s, d2h(s, start, dstart, (M + 1) * sizeof(int32_t)));
-  if (ids) FP_TRY(s, d2h(s, ids, which ? s->fsid : s->sid, (size_t)s->n * sizeof(int32_t)));
-  if (pos) FP_TRY(s, d2h(s, pos, which ? s->spos2 : s->spos, (size_t)s->n * sizeof(float4)));
+  if (ids && !which) FP_TRY(s, d2h(s, ids, s->sid, (size_t)s->n * sizeof(int32_t)));
+  if (which && (ids || pos)) {
+    // the fluid order keeps each slot's particle in its position's w (the
+    // index below PF_SHIFT, flags above)
+    std::vector<float4> p2(s->n);
+    FP_TRY(s, d2h(s, p2.data(), s->spos2, (size_t)s->n * sizeof(float4)));
+    if (ids)
+      for (int k = 0; k < s->n; k++) {
+        uint32_t w;
+        std::memcpy(&w, &p2[k].w, 4);
+        ids[k] = (int32_t)(w & SID_MASK);
+      }
+    if (pos) std::memcpy(pos, p2.data(), (size_t)s->n * sizeof(float4));
+  } else if (pos) {
+    FP_TRY(s, d2h(s, pos, s->spos, (size_t)s->n * sizeof(float4)));
+  }
   return FP_OK;
 }
 

@@ -321,8 +321,6 @@ __device__ __forceinline__ void fluid_rank_body(const SceneArgs& A, int lb) {
     for (int u = 0; u < 4; u++) r += (q0 + u < t && v[u] < i) ? 1 : 0;
   }
   const int d = s + r;
-  gst(A.fsid + d, i);
-  gst(A.fslot + i, d);
   gst(A.spos2 + d, tp);
   gst(A.sphash2 + d, hb);
 }

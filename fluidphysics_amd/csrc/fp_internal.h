@@ -162,7 +162,6 @@ struct fp_solver {
   uint32_t* sphash = nullptr;
   float4* tpos = nullptr;
   uint32_t *tkey = nullptr, *tslot = nullptr;
-  int *fsid = nullptr, *fslot = nullptr;
   float4 *spos2 = nullptr, *svtmp = nullptr;
   uint32_t* sphash2 = nullptr;
   float2* sP = nullptr;
@@ -354,7 +353,6 @@ struct SceneArgs {
   uint32_t* sphash;
   float4* tpos;
   uint32_t *tkey, *tslot;
-  int *fsid, *fslot;
   float4 *spos2, *svtmp;
   uint32_t* sphash2;
   float2* sP;

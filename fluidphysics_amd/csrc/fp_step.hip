@@ -200,8 +200,6 @@ SceneArgs fp_scene_args(const fp_solver* s, double h) {
   a.tpos = s->tpos;
   a.tkey = s->tkey;
   a.tslot = s->tslot;
-  a.fsid = s->fsid;
-  a.fslot = s->fslot;
   a.spos2 = s->spos2;
   a.svtmp = s->svtmp;
   a.sphash2 = s->sphash2;
