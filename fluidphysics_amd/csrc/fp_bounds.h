@@ -82,14 +82,14 @@ __device__ __forceinline__ void solid_finish(const SceneArgs& A, int i, float4 p
 }
 
 // Active fluid particle i, in fluid slot fs, whose position after the
-// displacement is p: pos, and its velocity into svtmp[fs] where XSPH reads
-// it; a particle culled here is deactivated and leaves its velocity as it
-// was, in vel and in svtmp.
+// displacement is p (x, y, z; its inverse mass stays in pos.w): pos, and its
+// velocity into svtmp[fs] where XSPH reads it; a particle culled here is
+// deactivated and leaves its velocity as it was, in vel and in svtmp.
 __device__ __forceinline__ void fluid_finish(const SceneArgs& A, int i, int fs, float4 p) {
   const BoundsParams& bp = A.bp;
   const float4 pv = gld(A.prev + i);
   bounds_apply(bp, false, p, pv);
-  gst(A.pos + i, p);
+  gst_xyz(A.pos + i, p);
   if (p.y < -3.f || p.x * p.x + p.z * p.z > bp.cull_r2) {
     // culled: the velocity stays the substep's integrated one, vel + g h
     // (integrate does not store it; the same sums here)

@@ -926,20 +926,17 @@ struct DisplaceAcc {
   }
 };
 
-// spos2 + the clamped displacement -> pos for the fluid; the inverse mass
-// is kept from pos.w.
+// spos2 + the clamped displacement -> pos for the fluid (x, y, z only: the
+// inverse mass in pos.w is not read or written).
 __device__ __forceinline__ void displace_body(const SceneArgs& A, int lb) {
   const FluidParams& fp = A.fp;
   const PairSource& src = A.src;
   const PairList& pl = A.pl;
   const float2* __restrict__ sP = A.sP;
-  float4* __restrict__ pos = A.pos;
   PairSelf me;
   const ReplayHead rh = replay_head(fp, pl, fluid_lane(lb));
   const float2 Px = gld(sP + min(fluid_lane(lb) >> fp.fl_log2, fp.n - 1));
   if (!pair_self(fp, src, fluid_lane(lb), me)) return;
-  // the inverse mass (pos.w) loads beside the replay
-  const float w = gld(pos + me.x()).w;
   DisplaceAcc acc;
   acc.sP = sP;
   acc.Px = Px;
@@ -957,7 +954,7 @@ __device__ __forceinline__ void displace_body(const SceneArgs& A, int lb) {
     sz *= g;
   }
   // the particle's substep ends here: bounds, then its velocity for XSPH
-  fluid_finish(A, me.x(), me.e, make_float4(me.p.x + sx, me.p.y + sy, me.p.z + sz, w));
+  fluid_finish(A, me.x(), me.e, make_float4(me.p.x + sx, me.p.y + sy, me.p.z + sz, 0.f));
 }
 
 __global__ void __launch_bounds__(FLUID_BLOCK) WAVES_ATTR(DISPLACE_WAVES) kg_displace(const SceneArgs* __restrict__ S,

@@ -233,6 +233,14 @@ __device__ __forceinline__ float4 gld_xyz(const float4* p) {
   const fp_f3v v = *(const FP_GLOBAL fp_f3v*)p;
   return make_float4(v.x, v.y, v.z, 0.f);
 }
+// x, y, z of a float4 with a 12-byte store (w stays as it is in memory).
+__device__ __forceinline__ void gst_xyz(float4* p, const float4& v) {
+  fp_f3v t;
+  t.x = v.x;
+  t.y = v.y;
+  t.z = v.z;
+  *(FP_GLOBAL fp_f3v*)p = t;
+}
 template <class T>
 __device__ __forceinline__ void gst(T* p, T v) {
   *(FP_GLOBAL T*)p = v;
