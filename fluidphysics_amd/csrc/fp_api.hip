@@ -653,6 +653,10 @@ int fp_add_solid_cube(fp_solver* s, double cx, double cy, double cz, int32_t n, 
   auto deg = [&](int i) { return hoff[i + 1] - hoff[i]; };
   for (int c = 0; c < C; c++) {
     const int a = s->h_cons_a[c], b = s->h_cons_b[c];
+    // packed entries: a lattice neighbour lies within n^2 + n + 1 of a (n <=
+    // 436, so under 2^23) and a solid has at most 26 of them
+    if (std::abs(b - a) >= (1 << 23) || std::max(deg(a), deg(b)) > 31)
+      return fp_fail(s, FP_EINVAL, "fp_add_solid_cube: constraint outside the packed CSR's range");
     const uint32_t md = (uint32_t)std::max(deg(a), deg(b)) << 24;
     const int ka = hoff[a] + fill[a]++, kb = hoff[b] + fill[b]++;
     hadj[ka] = ((uint32_t)(b - a) & 0xFFFFFFu) | md;
