@@ -255,7 +255,7 @@ int fp_read_tets(fp_solver* s, int32_t* verts, uint8_t* alive, int32_t* num_tets
 
 /* No counterpart in the reference: read access to the substep's own cell
  * sort, for tests.  The last substep sorted the particles twice by cell
- * (sort cells of cell_size / 2, keyed (cx * mx + cy * my + cz) mod M with the
+ * (sort cells of cell_size, keyed (cx * mx + cy * my + cz) mod M with the
  * cell coordinates floor(p * inv_cell) as 32-bit integers): which = 0 is the
  * grid order (every active particle by its position after integrate and the
  * grab pull), which = 1 the fluid order (every active fluid particle by its
